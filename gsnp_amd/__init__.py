@@ -1,0 +1,187 @@
+"""gsnp_amd -- MI355X (gfx950) SOAPsnp SNP-calling path.
+
+The product is the C-ABI library ``gsnp_amd/_build/libgsnp.so`` (declared in
+``include/gsnp_abi.h``) with HIP kernels for the site matrix, segmented sort,
+FP64 genotype likelihood, posterior, rank-sum test and .cns formatting, plus
+the command-line tools ``gsnp`` and ``gsnp-decompress`` beside it.  This
+module is a thin ctypes binding for Python callers; it loads the built
+library and raises if it is missing (there is no CPU fallback).
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+import os
+from pathlib import Path
+
+PKG = Path(__file__).resolve().parent
+LIB_PATH = PKG / "_build" / "libgsnp.so"
+CLI_PATH = PKG / "_build" / "gsnp"
+DECOMPRESS_PATH = PKG / "_build" / "gsnp-decompress"
+WIN = 262144
+
+GSNP_OK = 0
+ERRORS = {1: "io", 2: "format", 3: "unsorted", 4: "chrom", 5: "qual", 6: "range", 7: "nomem", 8: "arg", 9: "device"}
+
+
+class GsnpError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"gsnp error {code} ({ERRORS.get(code, '?')}): {msg}")
+        self.code = code
+
+
+class Params(ctypes.Structure):
+    """Mirror of gsnp_params (cpp/common.h:123-154)."""
+
+    _fields_ = [
+        ("q_min", ctypes.c_int),
+        ("q_max", ctypes.c_int),
+        ("read_length", ctypes.c_int),
+        ("is_monoploid", ctypes.c_int),
+        ("refine_mode", ctypes.c_int),
+        ("rank_sum_mode", ctypes.c_int),
+        ("transition_dominant", ctypes.c_int),
+        ("althom_novel_r", ctypes.c_double),
+        ("het_novel_r", ctypes.c_double),
+        ("althom_val_r", ctypes.c_double),
+        ("het_val_r", ctypes.c_double),
+        ("althom_unval_r", ctypes.c_double),
+        ("het_unval_r", ctypes.c_double),
+        ("global_dependency", ctypes.c_double),
+        ("pcr_dependency", ctypes.c_double),
+    ]
+
+
+class WindowOut(ctypes.Structure):
+    _fields_ = [
+        ("base1", ctypes.c_void_p),
+        ("base2", ctypes.c_void_p),
+        ("type1", ctypes.c_void_p),
+        ("q_cns", ctypes.c_void_p),
+        ("rank_sum", ctypes.c_void_p),
+        ("lk10", ctypes.c_void_p),
+        ("depth", ctypes.c_void_p),
+        ("dep_uni", ctypes.c_void_p),
+    ]
+
+
+class ReadBatch(ctypes.Structure):
+    _fields_ = [
+        ("n", ctypes.c_uint64),
+        ("pitch", ctypes.c_uint32),
+        ("pos", ctypes.c_void_p),
+        ("len_hit", ctypes.c_void_p),
+        ("strand", ctypes.c_void_p),
+        ("seq", ctypes.c_void_p),
+        ("qual", ctypes.c_void_p),
+    ]
+
+
+class Snp(ctypes.Structure):
+    _fields_ = [
+        ("hapmap", ctypes.c_uint8),
+        ("validated", ctypes.c_uint8),
+        ("indel", ctypes.c_uint8),
+        ("pad", ctypes.c_uint8),
+        ("freq", ctypes.c_double * 4),
+    ]
+
+
+_lib = None
+
+# every entry point declared in include/gsnp_abi.h: (restype, argtypes)
+_P = ctypes.c_void_p
+SIGNATURES = {
+    "gsnp_params_default": (None, [ctypes.POINTER(Params)]),
+    "gsnp_last_error": (ctypes.c_char_p, []),
+    "gsnp_device_init": (ctypes.c_int, [ctypes.c_int]),
+    "gsnp_create": (ctypes.c_int, [ctypes.POINTER(Params), _P, ctypes.POINTER(_P)]),
+    "gsnp_destroy": (None, [_P]),
+    "gsnp_set_chromosome": (ctypes.c_int, [_P, ctypes.c_char_p, _P, ctypes.c_uint32, _P, _P, ctypes.c_uint32]),
+    "gsnp_load_reads": (ctypes.c_int, [_P, ctypes.POINTER(ReadBatch)]),
+    "gsnp_window_init": (ctypes.c_int, [_P, ctypes.c_uint32]),
+    "gsnp_window_update": (ctypes.c_int, [_P, ctypes.c_uint64, ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint32)]),
+    "gsnp_window_likelihood": (ctypes.c_int, [_P, ctypes.c_uint32]),
+    "gsnp_window_posterior": (ctypes.c_int, [_P, ctypes.c_uint32]),
+    "gsnp_window_text": (ctypes.c_int, [_P, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint64)]),
+    "gsnp_fetch_text": (ctypes.c_int, [_P, ctypes.c_char_p, ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint64)]),
+    "gsnp_window_results": (ctypes.c_int, [_P, ctypes.c_uint32, ctypes.POINTER(WindowOut)]),
+    "gsnp_window_recycle": (ctypes.c_int, [_P]),
+    "gsnp_batched_call": (ctypes.c_int, [_P, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int,
+                                         ctypes.POINTER(ctypes.c_uint64)]),
+    "gsnp_run": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p,
+                                ctypes.POINTER(Params), ctypes.c_char_p, ctypes.c_int, ctypes.c_int]),
+    "gsnp_decompress": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p]),
+    "gsnp_matrix_gen": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(Params), _P]),
+    "gsnp_dict_compress_i32": (ctypes.c_int, [_P, _P, ctypes.c_uint64, _P, ctypes.POINTER(ctypes.c_uint64), _P,
+                                              ctypes.POINTER(ctypes.c_uint64)]),
+    "gsnp_sort_likelihood": (ctypes.c_int, [_P, _P, ctypes.c_uint32, _P, ctypes.c_uint32, _P]),
+    "gsnp_format_g": (ctypes.c_int, [ctypes.c_double, ctypes.c_char_p]),
+    "gsnp_synth_write": (ctypes.c_longlong, [ctypes.c_int, ctypes.c_ulonglong, ctypes.c_double, ctypes.c_char_p,
+                                             ctypes.POINTER(ctypes.c_int)]),
+}
+
+
+def lib() -> ctypes.CDLL:
+    """The loaded libgsnp.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(f"{LIB_PATH} is not built; run `python -c 'import __graft_entry__ as g; g.build()'`")
+        L = ctypes.CDLL(str(LIB_PATH))
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc: int) -> None:
+    if rc != GSNP_OK:
+        raise GsnpError(rc, lib().gsnp_last_error().decode())
+
+
+def params(**kw) -> Params:
+    """Default SOAPsnp parameters (cpp/common.h:142-153), overridden by kw.
+    global_dependency / pcr_dependency are given as the -g / -p values and
+    stored log10'd, as cpp/main.cc:151-160 does."""
+    p = Params()
+    lib().gsnp_params_default(ctypes.byref(p))
+    for k, v in kw.items():
+        if k in ("global_dependency", "pcr_dependency"):
+            v = math.log10(v)
+        setattr(p, k, v)
+    return p
+
+
+def _b(s) -> bytes | None:
+    return None if s is None else os.fsencode(s)
+
+
+def run(fasta, soap, out, dbsnp=None, matrix_in=None, matrix_out=None, compress=False, batched=0, **kw) -> None:
+    """Whole pipeline (gsnp_run): FASTA + SOAP (+ known SNPs) to .cns / -C files."""
+    L = lib()
+    check(L.gsnp_device_init(0))
+    p = params(**kw)
+    check(L.gsnp_run(_b(fasta), _b(soap), _b(dbsnp), _b(matrix_in), _b(matrix_out), ctypes.byref(p), _b(out),
+                     int(bool(compress)), int(batched)))
+
+
+def decompress(prefix, out) -> None:
+    check(lib().gsnp_decompress(_b(prefix), _b(out)))
+
+
+def synth(config: int, seed: int, scale: float, prefix) -> tuple[int, int]:
+    """Seeded synthetic inputs for a benchmark configuration; returns (alignment lines, read length)."""
+    rl = ctypes.c_int()
+    n = lib().gsnp_synth_write(config, seed, scale, _b(prefix), ctypes.byref(rl))
+    if n < 0:
+        raise GsnpError(1, f"cannot write {prefix}.*")
+    return int(n), rl.value
+
+
+def format_g(v: float) -> str:
+    buf = ctypes.create_string_buffer(64)
+    n = lib().gsnp_format_g(v, buf)
+    return buf.raw[:n].decode()

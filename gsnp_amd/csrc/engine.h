@@ -1,0 +1,122 @@
+// engine.h -- device buffers and the calling context behind the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/gsnp_abi.h"
+#include "kernels.h"
+
+namespace gsnp {
+
+int fail(int code, const std::string& msg);
+struct Ctx;
+int dict_encode_device(Ctx& c, const int32_t* d_in, uint64_t n, uint64_t* n_words, uint64_t* n_dict);
+
+template <class T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { release(); }
+    hipError_t alloc(size_t count)
+    {
+        release();
+        if (!count) count = 1;
+        n = count;
+        return hipMalloc(reinterpret_cast<void**>(&p), count * sizeof(T));
+    }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+};
+
+struct SiteBufs {
+    DevBuf<uint32_t> rec;
+    DevBuf<int32_t> nrec, depth, repeat;
+    DevBuf<int4> cu, qs, ca;
+    void reset()
+    {
+        rec.release();
+        nrec.release();
+        depth.release();
+        repeat.release();
+        cu.release();
+        qs.release();
+        ca.release();
+    }
+};
+
+struct Ctx {
+    gsnp_params p{};
+    hipStream_t stream = nullptr;
+    // tables
+    DevBuf<double> score, lprior, prank;
+    DevBuf<uint8_t> qadj;
+    int td = 1, gd = 1;
+    std::vector<double> h_prank, h_pprior;
+    // chromosome
+    std::string chr_name;
+    uint32_t chr_len = 0;
+    DevBuf<uint8_t> ref;
+    DevBuf<uint32_t> snp_pos;
+    DevBuf<double> snp_lp;
+    uint32_t n_snp = 0;
+    DevBuf<char> d_name;
+    // reads
+    uint64_t n_reads = 0;
+    uint32_t pitch = 0;
+    DevBuf<int32_t> rpos;
+    DevBuf<uint32_t> rmeta;
+    DevBuf<uint8_t> rseq;
+    std::vector<int32_t> h_rpos;
+    // sites of the current window (or batched span)
+    uint32_t ws = 0;
+    uint32_t win_cap = 0;
+    bool built = false;
+    SiteBufs sites;
+    DevBuf<double> lk;
+    DevBuf<uint8_t> base1, base2, type1;
+    DevBuf<int32_t> qcns;
+    DevBuf<double> rank;
+    DevBuf<uint32_t> lists, counts;
+    DevBuf<RankJob> jobs;
+    DevBuf<uint32_t> job_count;
+    DevBuf<FixJob> fixes;
+    DevBuf<uint32_t> fix_count;
+    uint32_t fix_offset = 0;
+    uint32_t last_fixups = 0;
+    // text
+    DevBuf<uint32_t> rowlen;
+    DevBuf<uint64_t> blocksum, total;
+    DevBuf<char> text;
+    uint64_t text_cap = 0, text_len = 0;
+    DevBuf<uint32_t> err;
+    DevBuf<int4> head_qs;  // q_sum of window-boundary sites before their own reads (batched span)
+    // device dictionary encoder scratch
+    DevBuf<int32_t> dict_mm, dict_vals;
+    DevBuf<uint32_t> dict_bits, dict_wpre, dict_out;
+    DevBuf<uint64_t> dict_bsum;
+
+    int alloc_window(uint32_t n_sites);
+    SiteArrays site_view(uint32_t offset);
+    Results result_view(uint32_t offset);
+    Tables tables() const;
+    RefView ref_view() const;
+    ReadView read_view() const;
+    int host_fixups();
+    int check_err();
+    int format_text(uint32_t offset, uint32_t ws_abs, uint32_t n_sites, uint32_t pos_base, uint64_t* nbytes);
+};
+
+}  // namespace gsnp
+
+// the opaque handle of the C ABI is the context itself
+struct gsnp_ctx : public gsnp::Ctx {};

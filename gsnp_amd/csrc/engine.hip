@@ -1,0 +1,619 @@
+// engine.hip -- context, C ABI and the per-window driver of the calling path.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/gsnp_abi.h"
+#include "cwriter.h"
+#include "engine.h"
+#include "gsnp_common.h"
+#include "inputs.h"
+#include "kernels.h"
+#include "tables.h"
+
+namespace gsnp {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg)
+{
+    g_err = msg;
+    return code;
+}
+
+#define HIPCHK(x)                                                                                 \
+    do {                                                                                          \
+        hipError_t e_ = (x);                                                                      \
+        if (e_ != hipSuccess) return fail(GSNP_E_DEVICE, std::string(#x) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+int Ctx::alloc_window(uint32_t n_sites)
+{
+    if (n_sites <= win_cap) return GSNP_OK;
+    HIPCHK(hipStreamSynchronize(stream));  // nothing queued may still use the old buffers
+    sites.reset();
+    win_cap = 0;
+    const size_t n = n_sites;
+    HIPCHK(sites.rec.alloc(n * kSlots));
+    HIPCHK(sites.nrec.alloc(n));
+    HIPCHK(sites.depth.alloc(n));
+    HIPCHK(sites.repeat.alloc(n));
+    HIPCHK(sites.cu.alloc(n));
+    HIPCHK(sites.qs.alloc(n));
+    HIPCHK(sites.ca.alloc(n));
+    HIPCHK(lk.alloc(n * 10));
+    HIPCHK(base1.alloc(n));
+    HIPCHK(base2.alloc(n));
+    HIPCHK(type1.alloc(n));
+    HIPCHK(qcns.alloc(n));
+    HIPCHK(rank.alloc(n));
+    HIPCHK(lists.alloc(n * 6));
+    HIPCHK(jobs.alloc(n));
+    HIPCHK(fixes.alloc(n));
+    HIPCHK(rowlen.alloc(n));
+    HIPCHK(blocksum.alloc(n / kRowBlock + 2));
+    HIPCHK(hipMemsetAsync(sites.nrec.p, 0, n * 4, stream));
+    HIPCHK(hipMemsetAsync(sites.depth.p, 0, n * 4, stream));
+    HIPCHK(hipMemsetAsync(sites.repeat.p, 0, n * 4, stream));
+    HIPCHK(hipMemsetAsync(sites.cu.p, 0, n * 16, stream));
+    HIPCHK(hipMemsetAsync(sites.qs.p, 0, n * 16, stream));
+    HIPCHK(hipMemsetAsync(sites.ca.p, 0, n * 16, stream));
+    win_cap = n_sites;
+    return GSNP_OK;
+}
+
+SiteArrays Ctx::site_view(uint32_t offset)
+{
+    SiteArrays a;
+    a.rec = sites.rec.p + (size_t)offset * kSlots;
+    a.nrec = sites.nrec.p + offset;
+    a.depth = sites.depth.p + offset;
+    a.repeat = sites.repeat.p + offset;
+    a.cu = sites.cu.p + offset;
+    a.qs = sites.qs.p + offset;
+    a.ca = sites.ca.p + offset;
+    return a;
+}
+
+Results Ctx::result_view(uint32_t offset)
+{
+    Results r;
+    r.base1 = base1.p + offset;
+    r.base2 = base2.p + offset;
+    r.type1 = type1.p + offset;
+    r.q_cns = qcns.p + offset;
+    r.rank = rank.p + offset;
+    r.lk = lk.p + (size_t)offset * 10;
+    return r;
+}
+
+Tables Ctx::tables() const
+{
+    Tables t;
+    t.score = score.p;
+    t.qadj = qadj.p;
+    t.td = td;
+    t.gd = gd;
+    t.lprior = lprior.p;
+    t.prank = prank.p;
+    t.read_len = p.read_length;
+    t.q_min = p.q_min;
+    t.q_max_min = p.q_max - p.q_min;
+    t.monoploid = p.is_monoploid;
+    t.refine = p.refine_mode;
+    t.rank_mode = p.rank_sum_mode;
+    return t;
+}
+
+RefView Ctx::ref_view() const
+{
+    RefView r;
+    r.code = ref.p;
+    r.len = chr_len;
+    r.snp_pos = snp_pos.p;
+    r.snp_lp = snp_lp.p;
+    r.n_snp = n_snp;
+    return r;
+}
+
+ReadView Ctx::read_view() const
+{
+    ReadView v;
+    v.pos = rpos.p;
+    v.meta = rmeta.p;
+    v.seq = rseq.p;
+    v.pitch = pitch;
+    return v;
+}
+
+// (int)x as x86-64 cvttsd2si
+static int trunc_x86(double x)
+{
+    if (!(x > -2147483649.0 && x < 2147483648.0)) return (int)0x80000000;
+    return (int)x;
+}
+
+// cpp/normal_dis.cc:3-24, cpp/rank_sum.cc:44-49 with the host libm
+static double host_normal_value(double z)
+{
+    if (z > 6.0 || z < -6.0) return 0.0;
+    const double b1 = 0.31938153, b2 = -0.356563782, b3 = 1.781477937, b4 = -1.821255978, b5 = 1.330274429;
+    const double pp = 0.2316419, c2 = 0.39894228;
+    double a = std::fabs(z);
+    double t = 1.0 / (1.0 + a * pp);
+    double b = c2 * std::exp((-z) * (z / 2.0));
+    double n = ((((b5 * t + b4) * t + b3) * t + b2) * t + b1) * t;
+    n = 1.0 - b * n;
+    if (z < 0.0) n = 1.0 - n;
+    return n > 0.5 ? 1 - n : n;
+}
+
+static double host_normal_test(int n1, int n2, double T1, double T2)
+{
+    double u1 = (T1 - n1 * (n1 + n2 + 1) / 2) / std::sqrt(n1 * n2 * (n1 + n2 + 1) / (double)12);
+    double u2 = (T2 - n2 * (n1 + n2 + 1) / 2) / std::sqrt(n1 * n2 * (n1 + n2 + 1) / (double)12);
+    return host_normal_value(std::fabs(u1) > std::fabs(u2) ? u1 : u2);
+}
+
+static double host_table_test(const double* p, int n1, int n2, double T1, double T2)
+{
+    if (n1 <= n2)
+        return p[(n1 + n2) << 17 | n1 << 11 | (int)(T1)] +
+               (T1 - (int)T1) * (p[(n1 + n2) << 16 | n1 << 11 | (int)(T1 + 1)] - p[(n1 + n2) << 17 | n1 << 11 | (int)(T1)]);
+    return p[(n1 + n2) << 17 | n2 << 11 | (int)(T2)] +
+           (T2 - (int)T2) * (p[(n1 + n2) << 16 | n2 << 11 | (int)(T2 + 1)] - p[(n1 + n2) << 17 | n2 << 11 | (int)(T2)]);
+}
+
+// Sites whose rank-sum value or consensus quality depends on a host-libm
+// transcendental are finished here with the host libm, then patched back.
+int Ctx::host_fixups()
+{
+    uint32_t nfix = 0;
+    HIPCHK(hipMemcpyAsync(&nfix, fix_count.p, 4, hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+    last_fixups = nfix;
+    if (!nfix) return GSNP_OK;
+    std::vector<FixJob> fx(nfix);
+    HIPCHK(hipMemcpyAsync(fx.data(), fixes.p, sizeof(FixJob) * nfix, hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+    std::vector<double> r(nfix);
+    std::vector<int32_t> q(nfix);
+    for (uint32_t i = 0; i < nfix; i++) {
+        const FixJob& f = fx[i];
+        double rk = f.normal ? host_normal_test(f.n1, f.n2, f.T1, f.T2)
+                             : host_table_test(h_prank.data(), f.n1, f.n2, f.T1, f.T2);
+        int qc = rk == 0.0 ? 0 : trunc_x86(10 * f.dtp + 10 * std::log10(rk));
+        if (f.het_ok) {
+            if (qc > f.cap) qc = f.cap;
+        } else {
+            qc = 0;
+        }
+        if (qc > 99) qc = 99;
+        if (qc < 0) qc = 0;
+        r[i] = rk;
+        q[i] = qc;
+    }
+    DevBuf<double> dr;
+    DevBuf<int32_t> dq;
+    HIPCHK(dr.alloc(nfix));
+    HIPCHK(dq.alloc(nfix));
+    HIPCHK(hipMemcpyAsync(dr.p, r.data(), 8 * nfix, hipMemcpyHostToDevice, stream));
+    HIPCHK(hipMemcpyAsync(dq.p, q.data(), 4 * nfix, hipMemcpyHostToDevice, stream));
+    Results res = result_view(fix_offset);
+    launch_apply_fix(stream, fixes.p, dr.p, dq.p, nfix, res);
+    HIPCHK(hipStreamSynchronize(stream));
+    return GSNP_OK;
+}
+
+int Ctx::check_err()
+{
+    uint32_t e = 0;
+    HIPCHK(hipMemcpyAsync(&e, err.p, 4, hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+    if (e & kErrQadjRange) return fail(GSNP_E_RANGE, "adjusted quality exceeds the score table");
+    return GSNP_OK;
+}
+
+}  // namespace gsnp
+
+using namespace gsnp;
+
+extern "C" {
+
+void gsnp_params_default(gsnp_params* p)
+{
+    std::memset(p, 0, sizeof *p);
+    p->q_min = 64;
+    p->q_max = 64 + 40;
+    p->read_length = 45;
+    p->althom_novel_r = 0.0005;
+    p->het_novel_r = 0.0010;
+    p->althom_val_r = 0.05;
+    p->het_val_r = 0.10;
+    p->althom_unval_r = 0.01;
+    p->het_unval_r = 0.02;
+    p->global_dependency = std::log10(0.9);
+    p->pcr_dependency = std::log10(0.5);
+}
+
+const char* gsnp_last_error(void) { return g_err.c_str(); }
+
+int gsnp_device_init(int device)
+{
+    HIPCHK(hipSetDevice(device));
+    return GSNP_OK;
+}
+
+int gsnp_create(const gsnp_params* p, const double* p_matrix, gsnp_ctx** out)
+{
+    if (!p || !p_matrix || !out) return fail(GSNP_E_ARG, "null argument");
+    if (p->read_length < 1 || p->read_length > 256) return fail(GSNP_E_ARG, "read length outside [1,256]");
+    if (p->q_min < 1 || p->q_max < p->q_min || p->q_max > 126 || p->q_max - p->q_min > 62)
+        return fail(GSNP_E_ARG, "quality characters out of range");
+    std::unique_ptr<gsnp_ctx> c(new gsnp_ctx());
+    c->p = *p;
+    HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    // score table, adjusted qualities, priors, rank table
+    std::vector<double> sc = build_score_table(p_matrix);
+    QadjTable qt = build_qadj_table(p->pcr_dependency, p->global_dependency);
+    c->td = qt.td;
+    c->gd = qt.gd;
+    c->h_pprior.resize(128);
+    prior_gen(*p, c->h_pprior.data());
+    std::vector<double> lp = log_prior_rows(c->h_pprior.data());
+    c->h_prank = rank_table();
+    HIPCHK(c->score.alloc(sc.size()));
+    HIPCHK(c->qadj.alloc(qt.v.size()));
+    HIPCHK(c->lprior.alloc(lp.size()));
+    HIPCHK(c->prank.alloc(c->h_prank.size()));
+    HIPCHK(c->err.alloc(1));
+    HIPCHK(c->counts.alloc(8));
+    HIPCHK(c->job_count.alloc(1));
+    HIPCHK(c->fix_count.alloc(1));
+    HIPCHK(c->total.alloc(1));
+    HIPCHK(hipMemcpyAsync(c->score.p, sc.data(), sc.size() * 8, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->qadj.p, qt.v.data(), qt.v.size(), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->lprior.p, lp.data(), lp.size() * 8, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->prank.p, c->h_prank.data(), c->h_prank.size() * 8, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemsetAsync(c->err.p, 0, 4, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    *out = c.release();
+    return GSNP_OK;
+}
+
+void gsnp_destroy(gsnp_ctx* c)
+{
+    if (!c) return;
+    if (c->stream) hipStreamSynchronize(c->stream);
+    hipStream_t s = c->stream;
+    delete c;
+    if (s) hipStreamDestroy(s);
+}
+
+int gsnp_set_chromosome(gsnp_ctx* c, const char* name, const uint8_t* codes, uint32_t len, const uint32_t* snp_pos,
+                        const gsnp_snp* snps, uint32_t n_snp)
+{
+    if (!c || !name || (!codes && len)) return fail(GSNP_E_ARG, "null argument");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->chr_name = name;
+    c->chr_len = len;
+    HIPCHK(c->ref.alloc((size_t)len + GSNP_WIN + 512));
+    HIPCHK(hipMemsetAsync(c->ref.p, 4, (size_t)len + GSNP_WIN + 512, c->stream));  // beyond the end reads as N
+    if (len) HIPCHK(hipMemcpyAsync(c->ref.p, codes, len, hipMemcpyHostToDevice, c->stream));
+    c->n_snp = n_snp;
+    if (n_snp) {
+        std::vector<double> lp((size_t)n_snp * 10);
+        for (uint32_t i = 0; i < n_snp; i++) {
+            if (snp_pos[i] >= len || (i && snp_pos[i] <= snp_pos[i - 1]))
+                return fail(GSNP_E_ARG, "known-SNP positions must be sorted, unique and inside the chromosome");
+            snp_log_prior(c->p, c->h_pprior.data(), codes[snp_pos[i]], snps[i], &lp[(size_t)i * 10]);
+        }
+        HIPCHK(c->snp_pos.alloc(n_snp));
+        HIPCHK(c->snp_lp.alloc(lp.size()));
+        HIPCHK(hipMemcpyAsync(c->snp_pos.p, snp_pos, 4ull * n_snp, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipMemcpyAsync(c->snp_lp.p, lp.data(), 8 * lp.size(), hipMemcpyHostToDevice, c->stream));
+    }
+    const size_t nl = std::strlen(name);
+    HIPCHK(c->d_name.alloc(nl + 1));
+    HIPCHK(hipMemcpyAsync(c->d_name.p, name, nl + 1, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return GSNP_OK;
+}
+
+static bool is_acgt(char ch)
+{
+    switch (ch) {
+    case 'A': case 'C': case 'G': case 'T': case 'a': case 'c': case 'g': case 't':
+        return true;
+    default:
+        return false;
+    }
+}
+
+int gsnp_load_reads(gsnp_ctx* c, const gsnp_read_batch* b)
+{
+    if (!c || !b) return fail(GSNP_E_ARG, "null argument");
+    const uint32_t L = (uint32_t)c->p.read_length;
+    if (b->n && (b->pitch < 1 || !b->pos || !b->len_hit || !b->strand || !b->seq || !b->qual))
+        return fail(GSNP_E_ARG, "incomplete read batch");
+    // validate and pack: byte = (q << 2) | base, 0 when the base is not observed
+    std::vector<uint32_t> meta(b->n);
+    std::vector<uint8_t> seq((size_t)b->n * L, 0);
+    int32_t last = 0;
+    for (uint64_t i = 0; i < b->n; i++) {
+        const uint32_t lh = (uint32_t)b->len_hit[i];
+        const uint32_t len = lh >> 16, hit = lh & 0xffff;
+        const int32_t pos = b->pos[i];
+        if (len < 1 || len > L || len > b->pitch) return fail(GSNP_E_RANGE, "read length outside [1, read_length]");
+        if (pos < 0 || (uint64_t)pos + len >= c->chr_len)
+            return fail(GSNP_E_RANGE, "read outside the chromosome (filter pos >= 0 and pos+len < length first)");
+        if (pos < last) return fail(GSNP_E_UNSORTED, "Errors in sorting:" + std::to_string(pos) + "<" + std::to_string(last));
+        last = pos;
+        const bool rev = b->strand[i] != '+';
+        meta[i] = hit | (len << 16) | (rev ? 0x80000000u : 0u);
+        const char* s = b->seq + i * b->pitch;
+        const char* q = b->qual + i * b->pitch;
+        uint8_t* o = &seq[i * L];
+        for (uint32_t k = 0; k < len; k++) {
+            const int qc = (unsigned char)q[k];
+            if (!is_acgt(s[k]) || qc < c->p.q_min) continue;
+            int qq = qc - c->p.q_min;
+            if (hit == 1) {
+                if (qq < 1 || qq > 63) return fail(GSNP_E_QUAL, "quality outside [1,63] on a unique-hit base");
+            } else {
+                qq = 1;  // repeats only count the base; any non-zero quality marks it observed
+            }
+            o[k] = (uint8_t)((qq << 2) | (((unsigned)s[k] >> 1) & 3));
+        }
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->n_reads = b->n;
+    c->pitch = L;
+    c->h_rpos.assign(b->pos, b->pos + b->n);
+    HIPCHK(c->rpos.alloc(b->n + 1));
+    HIPCHK(c->rmeta.alloc(b->n + 1));
+    HIPCHK(c->rseq.alloc(seq.size() + 64));
+    if (b->n) {
+        HIPCHK(hipMemcpyAsync(c->rpos.p, b->pos, 4 * b->n, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipMemcpyAsync(c->rmeta.p, meta.data(), 4 * b->n, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipMemcpyAsync(c->rseq.p, seq.data(), seq.size(), hipMemcpyHostToDevice, c->stream));
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return GSNP_OK;
+}
+
+int gsnp_window_init(gsnp_ctx* c, uint32_t start)
+{
+    if (!c) return fail(GSNP_E_ARG, "null context");
+    const uint32_t L = (uint32_t)c->p.read_length;
+    int rc = c->alloc_window(GSNP_WIN + L + 1);
+    if (rc) return rc;
+    c->ws = start;
+    // only the carried head [0, L) is read before being written
+    HIPCHK(hipMemsetAsync(c->sites.nrec.p, 0, 4 * (L + 1), c->stream));
+    HIPCHK(hipMemsetAsync(c->sites.depth.p, 0, 4 * (L + 1), c->stream));
+    HIPCHK(hipMemsetAsync(c->sites.repeat.p, 0, 4 * (L + 1), c->stream));
+    HIPCHK(hipMemsetAsync(c->sites.cu.p, 0, 16 * (L + 1), c->stream));
+    HIPCHK(hipMemsetAsync(c->sites.qs.p, 0, 16 * (L + 1), c->stream));
+    HIPCHK(hipMemsetAsync(c->sites.ca.p, 0, 16 * (L + 1), c->stream));
+    c->built = false;
+    return GSNP_OK;
+}
+
+int gsnp_window_update(gsnp_ctx* c, uint64_t r0, uint64_t r1, uint32_t* max_records)
+{
+    if (!c || !c->win_cap) return fail(GSNP_E_ARG, "window not initialised");
+    if (r0 > r1 || r1 > c->n_reads) return fail(GSNP_E_ARG, "read range outside the loaded batch");
+    const uint32_t L = (uint32_t)c->p.read_length;
+    launch_site_build(c->stream, c->read_view(), r0, r1, c->ws, GSNP_WIN + L, (int64_t)c->ws,
+                      (int64_t)c->ws + GSNP_WIN - 1, L, c->site_view(0), nullptr, c->err.p);
+    HIPCHK(hipGetLastError());
+    c->built = true;
+    if (max_records) {
+        std::vector<int32_t> n(GSNP_WIN);
+        HIPCHK(hipMemcpyAsync(n.data(), c->sites.nrec.p, 4ull * GSNP_WIN, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        *max_records = (uint32_t)*std::max_element(n.begin(), n.end());
+    }
+    return GSNP_OK;
+}
+
+int gsnp_window_likelihood(gsnp_ctx* c, uint32_t C)
+{
+    if (!c || !c->win_cap) return fail(GSNP_E_ARG, "window not initialised");
+    if (C > GSNP_WIN) return fail(GSNP_E_ARG, "call length above the window size");
+    if (!c->built) {  // a window without reads still carries the previous tail
+        int rc = gsnp_window_update(c, 0, 0, nullptr);
+        if (rc) return rc;
+    }
+    SiteArrays a = c->site_view(0);
+    launch_bucket(c->stream, a.nrec, C, c->lists.p, c->win_cap, c->counts.p);
+    launch_sort(c->stream, a.rec, a.nrec, c->lists.p, c->win_cap, c->counts.p, C);
+    launch_likelihood(c->stream, c->tables(), a, C, c->lk.p, c->err.p);
+    HIPCHK(hipGetLastError());
+    return GSNP_OK;
+}
+
+int gsnp_window_posterior(gsnp_ctx* c, uint32_t C)
+{
+    if (!c || !c->win_cap) return fail(GSNP_E_ARG, "window not initialised");
+    if (C > GSNP_WIN) return fail(GSNP_E_ARG, "call length above the window size");
+    SiteArrays a = c->site_view(0);
+    Results res = c->result_view(0);
+    HIPCHK(hipMemsetAsync(c->job_count.p, 0, 4, c->stream));
+    HIPCHK(hipMemsetAsync(c->fix_count.p, 0, 4, c->stream));
+    launch_posterior(c->stream, c->tables(), c->ref_view(), c->ws, a, C, nullptr, res, c->jobs.p, c->job_count.p);
+    if (c->p.rank_sum_mode) {
+        launch_rank(c->stream, c->tables(), a, c->jobs.p, c->job_count.p, C, res, c->fixes.p, c->fix_count.p);
+        HIPCHK(hipGetLastError());
+        c->fix_offset = 0;
+        int rc = c->host_fixups();
+        if (rc) return rc;
+    }
+    HIPCHK(hipGetLastError());
+    return c->check_err();
+}
+
+int gsnp_window_text(gsnp_ctx* c, uint32_t C, uint64_t* nbytes)
+{
+    if (!c || !c->win_cap) return fail(GSNP_E_ARG, "window not initialised");
+    return c->format_text(0, c->ws, C, c->ws + 1, nbytes);
+}
+
+int gsnp_fetch_text(gsnp_ctx* c, char* dst, uint64_t cap, uint64_t* nbytes)
+{
+    if (!c) return fail(GSNP_E_ARG, "null context");
+    if (nbytes) *nbytes = c->text_len;
+    if (!dst) return GSNP_OK;
+    if (cap < c->text_len) return fail(GSNP_E_ARG, "destination smaller than the formatted text");
+    if (c->text_len) HIPCHK(hipMemcpyAsync(dst, c->text.p, c->text_len, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return GSNP_OK;
+}
+
+int gsnp_window_results(gsnp_ctx* c, uint32_t C, const gsnp_window_out* o)
+{
+    if (!c || !o) return fail(GSNP_E_ARG, "null argument");
+    auto cp = [&](void* dst, const void* src, size_t bytes) -> hipError_t {
+        if (!dst || !bytes) return hipSuccess;
+        return hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream);
+    };
+    HIPCHK(cp(o->base1, c->base1.p, C));
+    HIPCHK(cp(o->base2, c->base2.p, C));
+    HIPCHK(cp(o->type1, c->type1.p, C));
+    HIPCHK(cp(o->q_cns, c->qcns.p, 4ull * C));
+    HIPCHK(cp(o->rank_sum, c->rank.p, 8ull * C));
+    HIPCHK(cp(o->lk10, c->lk.p, 80ull * C));
+    HIPCHK(cp(o->depth, c->sites.depth.p, 4ull * C));
+    HIPCHK(cp(o->dep_uni, c->sites.nrec.p, 4ull * C));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return GSNP_OK;
+}
+
+int gsnp_window_recycle(gsnp_ctx* c)
+{
+    if (!c || !c->win_cap) return fail(GSNP_E_ARG, "window not initialised");
+    if (!c->built) {
+        int rc = gsnp_window_update(c, 0, 0, nullptr);
+        if (rc) return rc;
+    }
+    launch_recycle(c->stream, c->site_view(0), GSNP_WIN, (uint32_t)c->p.read_length);
+    HIPCHK(hipGetLastError());
+    c->ws += GSNP_WIN;
+    c->built = false;
+    return GSNP_OK;
+}
+
+int gsnp_batched_call(gsnp_ctx* c, uint32_t start, uint32_t n, int emit_text, uint64_t* nbytes)
+{
+    if (!c) return fail(GSNP_E_ARG, "null context");
+    if ((uint64_t)start + n > c->chr_len) return fail(GSNP_E_ARG, "span outside the chromosome");
+    if (start % GSNP_WIN) return fail(GSNP_E_ARG, "span must start on a window boundary");
+    int rc = c->alloc_window(n + 2);
+    if (rc) return rc;
+    const uint32_t nheads = n / GSNP_WIN + 2;
+    if (c->head_qs.n < nheads) HIPCHK(c->head_qs.alloc(nheads));
+    HIPCHK(hipMemsetAsync(c->head_qs.p, 0, sizeof(int4) * nheads, c->stream));
+    SiteArrays a = c->site_view(0);
+    Results res = c->result_view(0);
+    // every read that covers [start, start+n], no window batches and no carry
+    launch_site_build(c->stream, c->read_view(), 0, c->n_reads, start, n + 1, INT64_MIN, (int64_t)start + n, 0, a,
+                      c->head_qs.p, c->err.p);
+    launch_bucket(c->stream, a.nrec, n, c->lists.p, c->win_cap, c->counts.p);
+    launch_sort(c->stream, a.rec, a.nrec, c->lists.p, c->win_cap, c->counts.p, n);
+    launch_likelihood(c->stream, c->tables(), a, n, c->lk.p, c->err.p);
+    HIPCHK(hipMemsetAsync(c->job_count.p, 0, 4, c->stream));
+    HIPCHK(hipMemsetAsync(c->fix_count.p, 0, 4, c->stream));
+    launch_posterior(c->stream, c->tables(), c->ref_view(), start, a, n, c->head_qs.p, res, c->jobs.p, c->job_count.p);
+    if (c->p.rank_sum_mode) {
+        launch_rank(c->stream, c->tables(), a, c->jobs.p, c->job_count.p, n, res, c->fixes.p, c->fix_count.p);
+        HIPCHK(hipGetLastError());
+        c->fix_offset = 0;
+        rc = c->host_fixups();
+        if (rc) return rc;
+    }
+    HIPCHK(hipGetLastError());
+    rc = c->check_err();
+    if (rc) return rc;
+    c->ws = start;
+    c->built = false;
+    if (emit_text) return c->format_text(0, start, n, start + 1, nbytes);
+    if (nbytes) *nbytes = 0;
+    return GSNP_OK;
+}
+
+int gsnp_sort_likelihood(gsnp_ctx* c, uint32_t* records, uint32_t pitch, const uint32_t* counts, uint32_t n_sites,
+                         double* lk10)
+{
+    if (!c || !records || !counts || !lk10) return fail(GSNP_E_ARG, "null argument");
+    if (pitch > kSlots) return fail(GSNP_E_ARG, "pitch above 256");
+    int rc = c->alloc_window(n_sites + 1);
+    if (rc) return rc;
+    std::vector<uint32_t> rows((size_t)n_sites * kSlots, 0xFFFFFFFFu);
+    std::vector<int32_t> n(n_sites);
+    for (uint32_t i = 0; i < n_sites; i++) {
+        if (counts[i] > pitch) return fail(GSNP_E_ARG, "count above pitch");
+        n[i] = (int32_t)counts[i];
+        std::memcpy(&rows[(size_t)i * kSlots], records + (size_t)i * pitch, 4ull * counts[i]);
+    }
+    HIPCHK(hipMemcpyAsync(c->sites.rec.p, rows.data(), rows.size() * 4, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->sites.nrec.p, n.data(), 4ull * n_sites, hipMemcpyHostToDevice, c->stream));
+    SiteArrays a = c->site_view(0);
+    launch_bucket(c->stream, a.nrec, n_sites, c->lists.p, c->win_cap, c->counts.p);
+    launch_sort(c->stream, a.rec, a.nrec, c->lists.p, c->win_cap, c->counts.p, n_sites);
+    launch_likelihood(c->stream, c->tables(), a, n_sites, c->lk.p, c->err.p);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(rows.data(), c->sites.rec.p, rows.size() * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(lk10, c->lk.p, 80ull * n_sites, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (uint32_t i = 0; i < n_sites; i++)
+        std::memcpy(records + (size_t)i * pitch, &rows[(size_t)i * kSlots], 4ull * counts[i]);
+    c->built = false;
+    return c->check_err();
+}
+
+int gsnp_format_g(double v, char* out)
+{
+    int n = fmt_g(v, out);
+    out[n] = 0;
+    return n;
+}
+
+}  // extern "C"
+
+namespace gsnp {
+
+int Ctx::format_text(uint32_t offset, uint32_t ws_abs, uint32_t n_sites, uint32_t pos_base, uint64_t* nbytes)
+{
+    const size_t name_len = chr_name.size();
+    const uint64_t need = (uint64_t)n_sites * (name_len + 160) + 64;
+    if (need > text_cap) {
+        HIPCHK(text.alloc(need));
+        text_cap = need;
+    }
+    RowCtx rc;
+    rc.name = d_name.p;
+    rc.name_len = (int)name_len;
+    rc.pos_base = pos_base;
+    SiteArrays a = site_view(offset);
+    Results res = result_view(offset);
+    const uint32_t nb = (n_sites + kRowBlock - 1) / kRowBlock;
+    launch_rowlen(stream, ref_view(), ws_abs, a, res, n_sites, rc, rowlen.p, blocksum.p);
+    launch_scan_blocks(stream, blocksum.p, nb, total.p);
+    launch_format(stream, ref_view(), ws_abs, a, res, n_sites, rc, rowlen.p, blocksum.p, text.p);
+    HIPCHK(hipGetLastError());
+    uint64_t t = 0;
+    HIPCHK(hipMemcpyAsync(&t, total.p, 8, hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+    text_len = n_sites ? t : 0;
+    if (nbytes) *nbytes = text_len;
+    return GSNP_OK;
+}
+
+}  // namespace gsnp

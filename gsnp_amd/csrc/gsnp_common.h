@@ -147,24 +147,11 @@ GSNP_HD uint64_t big_shr_u64(const BigNum& b, int s)
     return r;
 }
 
-GSNP_HD uint64_t round_scaled(uint64_t m, int e, int k, bool trunc)
+// Multiword path of round_scaled for scalings outside 10^0..10^27 (values
+// below 1e-22 or from 1e6 up); kept out of line so the common path stays in
+// registers.
+__host__ __device__ __noinline__ inline uint64_t round_scaled_wide(uint64_t m, int e, int k, bool trunc)
 {
-    if (k >= 0 && k <= 27) {
-        unsigned __int128 p5 = 1;
-        for (int i = 0; i < k; i++) p5 *= 5u;
-        unsigned __int128 N = (unsigned __int128)m * p5;  // < 2^116
-        int s = -(e + k);
-        if (s <= 0) {
-            if (-s >= 11) return (1ull << 63);
-            return (uint64_t)(N << (-s));
-        }
-        if (s >= 128) return 0;
-        unsigned __int128 q = N >> s;
-        unsigned __int128 rem = N - (q << s);
-        unsigned __int128 half = (unsigned __int128)1 << (s - 1);
-        if (!trunc && (rem > half || (rem == half && (q & 1)))) q += 1;
-        return (uint64_t)q;
-    }
     // general path: v = A / 2^s with A = m << max(e,0), s = max(-e,0)
     BigNum b;
     int s = e < 0 ? -e : 0;
@@ -193,6 +180,28 @@ GSNP_HD uint64_t round_scaled(uint64_t m, int e, int k, bool trunc)
     uint64_t q = big_shr_u64(I, 0);
     if (!trunc && (top > 5 || (top == 5 && (sticky || (q & 1))))) q += 1;
     return q;
+}
+
+GSNP_HD uint64_t round_scaled(uint64_t m, int e, int k, bool trunc)
+{
+    if (k >= 0 && k <= 27) {
+        unsigned __int128 p5 = 1;
+        for (int i = 0; i < k; i++) p5 *= 5u;
+        unsigned __int128 N = (unsigned __int128)m * p5;  // < 2^116
+        int s = -(e + k);
+        if (s <= 0) {  // integral result; saturate, callers only compare it with 10^5..10^6
+            if (-s >= 11) return (1ull << 63);
+            const unsigned __int128 v = N << (-s);
+            return (v >> 63) ? (1ull << 63) : (uint64_t)v;
+        }
+        if (s >= 128) return 0;
+        unsigned __int128 q = N >> s;
+        unsigned __int128 rem = N - (q << s);
+        unsigned __int128 half = (unsigned __int128)1 << (s - 1);
+        if (!trunc && (rem > half || (rem == half && (q & 1)))) q += 1;
+        return (uint64_t)q;
+    }
+    return round_scaled_wide(m, e, k, trunc);
 }
 
 GSNP_HD int fmt_g(double v, char* o)

@@ -1,0 +1,439 @@
+// inputs.cpp -- FASTA, known-SNP and SOAP alignment readers (host).
+#include "inputs.h"
+
+#include <algorithm>
+#include <cctype>
+#include <climits>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+
+namespace gsnp {
+
+namespace {
+
+// Buffered line reader over a FILE*.
+class LineReader {
+public:
+    explicit LineReader(FILE* f) : f_(f), buf_(1 << 22) {}
+    // returns false at end of file; line excludes the '\n'
+    bool next(const char*& s, size_t& n)
+    {
+        for (;;) {
+            char* nl = (char*)std::memchr(buf_.data() + beg_, '\n', end_ - beg_);
+            if (nl) {
+                s = buf_.data() + beg_;
+                n = (size_t)(nl - s);
+                beg_ = (size_t)(nl - buf_.data()) + 1;
+                return true;
+            }
+            if (eof_) {
+                if (beg_ < end_) {
+                    s = buf_.data() + beg_;
+                    n = end_ - beg_;
+                    beg_ = end_;
+                    return true;
+                }
+                return false;
+            }
+            if (beg_ > 0) {
+                std::memmove(buf_.data(), buf_.data() + beg_, end_ - beg_);
+                end_ -= beg_;
+                beg_ = 0;
+            }
+            if (end_ == buf_.size()) buf_.resize(buf_.size() * 2);
+            size_t got = std::fread(buf_.data() + end_, 1, buf_.size() - end_, f_);
+            if (got == 0) eof_ = true;
+            end_ += got;
+        }
+    }
+
+private:
+    FILE* f_;
+    std::vector<char> buf_;
+    size_t beg_ = 0, end_ = 0;
+    bool eof_ = false;
+};
+
+struct FileCloser {
+    void operator()(FILE* f) const
+    {
+        if (f) std::fclose(f);
+    }
+};
+using FilePtr = std::unique_ptr<FILE, FileCloser>;
+
+inline bool acgt(char c)
+{
+    switch (c) {
+    case 'A': case 'C': case 'G': case 'T': case 'a': case 'c': case 'g': case 't':
+        return true;
+    default:
+        return false;
+    }
+}
+
+// Token scanner with std::istream extraction semantics for the SOAP fields.
+struct Scan {
+    const char* p;
+    const char* e;
+    void ws()
+    {
+        while (p < e && std::isspace((unsigned char)*p)) p++;
+    }
+    bool word(const char*& s, int& n)
+    {
+        ws();
+        if (p >= e) return false;
+        s = p;
+        while (p < e && !std::isspace((unsigned char)*p)) p++;
+        n = (int)(p - s);
+        return true;
+    }
+    bool integer(int& v)
+    {
+        ws();
+        const char* q = p;
+        bool neg = false;
+        if (q < e && (*q == '+' || *q == '-')) neg = *q++ == '-';
+        if (q >= e || !std::isdigit((unsigned char)*q)) return false;
+        long long x = 0;
+        while (q < e && std::isdigit((unsigned char)*q)) {
+            x = x * 10 + (*q++ - '0');
+            if (x > 2147483648LL) return false;
+        }
+        if (neg) x = -x;
+        if (x > INT_MAX || x < INT_MIN) return false;
+        v = (int)x;
+        p = q;
+        return true;
+    }
+    bool character(char& c)
+    {
+        ws();
+        if (p >= e) return false;
+        c = *p++;
+        return true;
+    }
+};
+
+}  // namespace
+
+int load_genome(const char* fasta, const char* dbsnp, Genome& g, std::string& err)
+{
+    FilePtr f(std::fopen(fasta, "r"));
+    if (!f) {
+        err = std::string("cannot open ") + fasta;
+        return GSNP_E_IO;
+    }
+    LineReader lr(f.get());
+    const char* s;
+    size_t n;
+    int cur = -1;
+    while (lr.next(s, n)) {
+        if (n && s[0] == '>') {
+            size_t i = 1;
+            while (i < n && !std::isspace((unsigned char)s[i])) i++;
+            std::string name(s + 1, i - 1);
+            if (g.index.count(name)) {
+                err = "duplicate chromosome " + name;
+                return GSNP_E_FORMAT;
+            }
+            g.index[name] = (int)g.chr.size();
+            g.chr.emplace_back();
+            g.chr.back().name = name;
+            cur = (int)g.chr.size() - 1;
+        } else if (cur >= 0) {
+            auto& code = g.chr[cur].code;
+            size_t at = code.size();
+            code.resize(at + n);
+            for (size_t k = 0; k < n; k++) code[at + k] = (uint8_t)(((unsigned)(signed char)s[k] >> 1) & 7);
+        }
+    }
+    if (dbsnp && dbsnp[0]) {
+        FilePtr fs(std::fopen(dbsnp, "r"));
+        if (!fs) {
+            err = std::string("cannot open ") + dbsnp;
+            return GSNP_E_IO;
+        }
+        LineReader ls(fs.get());
+        struct Tmp {
+            uint32_t pos, order;
+            gsnp_snp s;
+        };
+        std::vector<std::vector<Tmp>> per(g.chr.size());
+        uint32_t order = 0;
+        while (ls.next(s, n)) {
+            std::string line(s, n);
+            char name[256];
+            long long pos;
+            int hm, va, in;
+            double fr[4];
+            if (std::sscanf(line.c_str(), "%255s %lld %d %d %d %lf %lf %lf %lf", name, &pos, &hm, &va, &in, &fr[0],
+                            &fr[1], &fr[2], &fr[3]) != 9)
+                continue;
+            auto it = g.index.find(name);
+            if (it == g.index.end()) continue;
+            Chrom& c = g.chr[it->second];
+            if (pos < 1 || (uint64_t)pos > c.code.size()) {
+                err = "known-SNP position outside chromosome " + c.name;
+                return GSNP_E_RANGE;
+            }
+            Tmp t;
+            t.pos = (uint32_t)(pos - 1);
+            t.order = order++;
+            std::memset(&t.s, 0, sizeof t.s);
+            t.s.hapmap = hm != 0;
+            t.s.validated = va != 0;
+            t.s.indel = in != 0;
+            std::memcpy(t.s.freq, fr, sizeof fr);
+            per[it->second].push_back(t);
+        }
+        for (size_t ci = 0; ci < g.chr.size(); ci++) {
+            auto& v = per[ci];
+            std::stable_sort(v.begin(), v.end(), [](const Tmp& a, const Tmp& b) { return a.pos < b.pos; });
+            Chrom& c = g.chr[ci];
+            for (const Tmp& t : v) {
+                if (!c.snp_pos.empty() && c.snp_pos.back() == t.pos) continue;  // first insertion wins
+                c.snp_pos.push_back(t.pos);
+                c.snp.push_back(t.s);
+                c.code[t.pos] |= 8;
+            }
+        }
+    }
+    return GSNP_OK;
+}
+
+int load_alignments(const char* soap, const gsnp_params& p, const Genome& g, std::vector<ReadBlock>& blocks,
+                    std::string& err)
+{
+    FilePtr f(std::fopen(soap, "r"));
+    if (!f) {
+        err = std::string("cannot open ") + soap;
+        return GSNP_E_IO;
+    }
+    const int L = p.read_length;
+    LineReader lr(f.get());
+    const char* s;
+    size_t n;
+    std::vector<char> rd(1024), ql(1024);
+    std::vector<char> seen(g.chr.size(), 0);
+    int cur = -1;
+    int32_t last = 0;
+    while (lr.next(s, n)) {
+        Scan sc{s, s + n};
+        const char *id, *r0, *q0, *chr;
+        int nid, nr, nq, nchr, hit, rlen, pos, mm;
+        char ab, strand;
+        if (!sc.word(id, nid) || !sc.word(r0, nr) || !sc.word(q0, nq) || !sc.integer(hit) || !sc.character(ab) ||
+            !sc.integer(rlen) || !sc.character(strand) || !sc.word(chr, nchr) || !sc.integer(pos) || !sc.integer(mm))
+            continue;
+        // indel edits (cpp/soap_snp.h:139-158); the read keeps its length field
+        size_t cap = (size_t)nr + (size_t)nq + 2048;
+        if (rd.size() < cap) rd.resize(cap), ql.resize(cap);
+        int elen = nr, eqlen = nq;
+        const char* es = r0;
+        const char* eq = q0;
+        if (mm > 100) {
+            int ip;
+            if (!sc.integer(ip)) continue;
+            int il = mm > 200 ? mm - 200 : mm - 100;
+            bool ins_n = mm > 200;
+            bool bad = ins_n ? (ip < 0 || ip > nr || ip > nq || il > 1024) : (ip < 0 || ip + il > nr || ip + il > nq);
+            if (bad) {
+                err = "indel position out of range";
+                return GSNP_E_FORMAT;
+            }
+            int skip = ins_n ? 0 : il;
+            int tail = rlen - ip - skip;
+            if (tail < 0) tail = 0;
+            int tr = std::min(nr - ip - skip, tail), tq = std::min(nq - ip - skip, tail);
+            std::memcpy(rd.data(), r0, ip);
+            std::memcpy(ql.data(), q0, ip);
+            int at = ip;
+            if (ins_n) {
+                std::memset(rd.data() + at, 'N', il);
+                std::memset(ql.data() + at, 'N', il);
+                at += il;
+            }
+            std::memcpy(rd.data() + at, r0 + ip + skip, tr);
+            std::memcpy(ql.data() + at, q0 + ip + skip, tq);
+            elen = at + tr;
+            eqlen = at + tq;
+            es = rd.data();
+            eq = ql.data();
+        }
+        pos -= 1;
+        if (pos < 0) continue;
+        auto it = g.index.find(std::string(chr, nchr));
+        if (it == g.index.end()) {
+            err = "Chromosome: !" + std::string(chr, nchr) + "! NOT found";
+            return GSNP_E_CHROM;
+        }
+        if (rlen < 1 || rlen > L) {
+            err = "read length " + std::to_string(rlen) + " outside [1," + std::to_string(L) + "]";
+            return GSNP_E_RANGE;
+        }
+        if (hit < 0 || hit > 65535) {
+            err = "hit count " + std::to_string(hit) + " outside [0,65535]";
+            return GSNP_E_RANGE;
+        }
+        if (hit == 1) {
+            for (int k = 0; k < rlen && k < elen; k++) {
+                int qc = k < eqlen ? (unsigned char)eq[k] : 0;
+                if (!acgt(es[k]) || qc < p.q_min) continue;
+                int q = qc - p.q_min;
+                if (q < 1 || q > 63) {
+                    err = "quality " + std::to_string(q) + " outside [1,63] on a unique-hit base";
+                    return GSNP_E_QUAL;
+                }
+            }
+        }
+        const int ci = it->second;
+        if (ci != cur) {
+            if (seen[ci]) {
+                err = "reads of chromosome " + g.chr[ci].name + " are not contiguous";
+                return GSNP_E_UNSORTED;
+            }
+            seen[ci] = 1;
+            blocks.emplace_back();
+            blocks.back().chr = ci;
+            blocks.back().pitch = (uint32_t)L;
+            cur = ci;
+            last = 0;
+        }
+        if ((int64_t)pos + rlen >= (int64_t)g.chr[ci].code.size()) continue;
+        if (pos < last) {
+            err = "Errors in sorting:" + std::to_string(pos) + "<" + std::to_string(last);
+            return GSNP_E_UNSORTED;
+        }
+        last = pos;
+        ReadBlock& b = blocks.back();
+        b.pos.push_back(pos);
+        b.len_hit.push_back((int32_t)(((uint32_t)rlen << 16) | (uint32_t)hit));
+        b.strand.push_back(strand);
+        size_t at = b.seq.size();
+        b.seq.resize(at + L);
+        b.qual.resize(at + L);
+        for (int k = 0; k < L; k++) {
+            b.seq[at + k] = k < elen ? es[k] : 'N';
+            b.qual[at + k] = k < eqlen ? eq[k] : 0;
+        }
+    }
+    return GSNP_OK;
+}
+
+void matrix_gen(const std::vector<ReadBlock>& blocks, const Genome& g, const gsnp_params& p, double* pm)
+{
+    std::vector<uint64_t> cnt((size_t)256 * 256 * 16, 0);
+    for (const ReadBlock& b : blocks) {
+        const Chrom& c = g.chr[b.chr];
+        for (uint64_t i = 0; i < b.size(); i++) {
+            const uint32_t lh = (uint32_t)b.len_hit[i];
+            if ((lh & 0xffff) != 1) continue;
+            const int len = (int)(lh >> 16);
+            const char* sq = &b.seq[i * b.pitch];
+            const char* qu = &b.qual[i * b.pitch];
+            const bool fwd = b.strand[i] == '+';
+            for (int k = 0; k < len; k++) {
+                if (!acgt(sq[k])) continue;
+                const uint8_t ref = c.code[b.pos[i] + k];
+                if (ref & 12) continue;  // N or known SNP on the reference
+                const uint64_t cyc = fwd ? (uint64_t)k : (uint64_t)(len - 1 - k);
+                const uint64_t qc = (unsigned char)qu[k];
+                cnt[(qc << 12) | (cyc << 4) | ((uint64_t)(ref & 3) << 2) | (((unsigned)sq[k] >> 1) & 3)]++;
+            }
+        }
+    }
+    for (size_t i = 0; i < (size_t)256 * 256 * 16; i++) pm[i] = 1.0;
+    const uint64_t kPow = 10;  // "statistically powerful" count
+    for (int qc = p.q_min; qc <= p.q_max; qc++) {
+        const uint64_t* row = &cnt[(size_t)qc << 12];
+        uint64_t by_type[16] = {0}, by_ref[4] = {0}, total = 0, mism = 0;
+        for (int c = 0; c < p.read_length; c++)
+            for (int t = 0; t < 16; t++) {
+                uint64_t v = row[(c << 4) | t];
+                by_type[t] += v;
+                by_ref[t >> 2] += v;
+                total += v;
+                if (t % 5) mism += v;
+            }
+        const int q = qc - p.q_min;
+        for (int c = 0; c < p.read_length; c++) {
+            uint64_t sum[4] = {0};
+            for (int t = 0; t < 16; t++) sum[t >> 2] += row[(c << 4) | t];
+            for (int r = 0; r < 4; r++)
+                for (int o = 0; o < 4; o++) {
+                    double& v = pm[((size_t)q << 12) | ((size_t)c << 4) | (r << 2) | o];
+                    const uint64_t k = row[(c << 4) | (r << 2) | o];
+                    if (k > kPow)
+                        v = (double)k / (double)sum[r];
+                    else if (by_type[(r << 2) | o] > kPow)
+                        v = (double)by_type[(r << 2) | o] / (double)by_ref[r];
+                    else if (total > 0)
+                        v = (o == r) ? (double)(total - mism) / (double)total : (double)mism / (double)total;
+                    if (v == 0 || v == 1) {  // fall back to the Phred value
+                        double e = std::pow(10, -(q / 10.0));
+                        if (o == r) {
+                            v = 1 - e;
+                            if (v < 0.25) v = 0.25;
+                        } else {
+                            v = e / 3;
+                            if (v > 0.25) v = 0.25;
+                        }
+                    }
+                }
+        }
+    }
+}
+
+int matrix_read(const char* path, double* pm, std::string& err)
+{
+    FilePtr f(std::fopen(path, "r"));
+    if (!f) {
+        err = std::string("cannot open ") + path;
+        return GSNP_E_IO;
+    }
+    for (size_t i = 0; i < (size_t)256 * 256 * 16; i++) pm[i] = 1.0;
+    LineReader lr(f.get());
+    const char* s;
+    size_t n;
+    while (lr.next(s, n)) {
+        std::string line(s, n);
+        char* p = (char*)line.c_str();
+        char* e;
+        long q = std::strtol(p, &e, 10);
+        if (e == p) continue;
+        p = e;
+        long c = std::strtol(p, &e, 10);
+        if (e == p || q < 0 || q > 255 || c < 0 || c > 255) continue;
+        p = e;
+        for (int t = 0; t < 16; t++) {
+            double v = std::strtod(p, &e);
+            if (e == p) break;
+            p = e;
+            pm[((size_t)q << 12) | ((size_t)c << 4) | t] = v;
+        }
+    }
+    return GSNP_OK;
+}
+
+int matrix_write(const char* path, const double* pm, const gsnp_params& p, std::string& err)
+{
+    FilePtr f(std::fopen(path, "w"));
+    if (!f) {
+        err = std::string("cannot create ") + path;
+        return GSNP_E_IO;
+    }
+    for (int q = 0; q <= p.q_max - p.q_min; q++)
+        for (int c = 0; c < p.read_length; c++) {
+            std::fprintf(f.get(), "%d\t%d", q, c);
+            for (int t = 0; t < 16; t++) std::fprintf(f.get(), "\t%.16e", pm[((size_t)q << 12) | ((size_t)c << 4) | t]);
+            std::fputc('\n', f.get());
+        }
+    return GSNP_OK;
+}
+
+}  // namespace gsnp

@@ -1,0 +1,979 @@
+// kernels.hip -- CDNA4 (gfx950) kernels of the SOAPsnp calling path.
+//
+// Compiled with -ffp-contract=off: the likelihood is a sequential FP64 sum in
+// sorted-record order and every comparison downstream depends on its exact
+// rounding, so no FMA contraction anywhere.
+#include <hip/hip_runtime.h>
+
+#include "gsnp_common.h"
+#include "kernels.h"
+
+namespace gsnp {
+
+namespace {
+
+__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+
+// lower_bound / upper_bound over positions[lo, hi)
+__device__ uint64_t lower_pos(const int32_t* pos, uint64_t lo, uint64_t hi, int64_t v)
+{
+    while (lo < hi) {
+        uint64_t m = (lo + hi) >> 1;
+        if ((int64_t)pos[m] < v)
+            lo = m + 1;
+        else
+            hi = m;
+    }
+    return lo;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// (a) site matrix.  One thread per site, 256 consecutive sites per block.  The
+// block walks, in input order, every read of [r0, r1) that can cover one of
+// its sites; read position and metadata are block-uniform (scalar loads) and
+// the lanes of a covering read fetch consecutive bytes of it (coalesced).
+// Per-site semantics follow kernel/update_kernel.cu:233-326: depth and
+// repeat for every covering read; for an observed base (ACGT, quality >=
+// q_min) while dep_uni < 255: unique hits append a record and bump
+// count_uni/q_sum/dep_uni; every hit bumps count_all.
+// Sites below `carry` continue from the values already in memory (the
+// recycled read-length tail of the previous window); the others start at 0.
+// Only reads starting in [lo_min, hi_max] take part.  With head_qs set, a site
+// on a window boundary (absolute position a multiple of the window) also
+// records its q_sum before the reads that start on it, which is what the
+// per-window path holds in its first tail site when the window is called.
+__global__ __launch_bounds__(256) void k_site_build(ReadView r, uint64_t r0, uint64_t r1, uint32_t ws, uint32_t n_sites,
+                                                    int64_t lo_min, int64_t hi_max, uint32_t carry, SiteArrays a,
+                                                    int4* head_qs, uint32_t* err)
+{
+    __shared__ uint64_t range[2];
+    const uint32_t s0 = blockIdx.x * 256u;
+    const uint32_t s = s0 + threadIdx.x;
+    const int L = (int)r.pitch;
+    if (threadIdx.x == 0) {
+        int64_t lo = (int64_t)ws + (int64_t)s0 - (L - 1);
+        if (lo < lo_min) lo = lo_min;
+        int64_t hi = (int64_t)ws + (int64_t)s0 + 255;
+        if (hi > hi_max) hi = hi_max;
+        range[0] = lower_pos(r.pos, r0, r1, lo);
+        range[1] = lower_pos(r.pos, r0, r1, hi + 1);
+    }
+    __syncthreads();
+    // block-uniform bounds: keep the read loop on scalar registers
+    const uint64_t b = ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(range[0] >> 32)) << 32) |
+                       (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)range[0]);
+    const uint64_t e = ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(range[1] >> 32)) << 32) |
+                       (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)range[1]);
+    const bool live = s < n_sites;
+    int depth = 0, repeat = 0, nrec = 0;
+    uint32_t cu = 0;   // 4 x 8-bit count_uni
+    uint64_t qs = 0;   // 4 x 16-bit q_sum
+    int ca0 = 0, ca1 = 0, ca2 = 0, ca3 = 0;
+    if (live && s < carry) {
+        depth = a.depth[s];
+        repeat = a.repeat[s];
+        nrec = a.nrec[s];
+        int4 c = a.cu[s], q = a.qs[s], al = a.ca[s];
+        cu = (uint32_t)c.x | ((uint32_t)c.y << 8) | ((uint32_t)c.z << 16) | ((uint32_t)c.w << 24);
+        qs = (uint64_t)(uint32_t)q.x | ((uint64_t)(uint32_t)q.y << 16) | ((uint64_t)(uint32_t)q.z << 32) |
+             ((uint64_t)(uint32_t)q.w << 48);
+        ca0 = al.x, ca1 = al.y, ca2 = al.z, ca3 = al.w;
+    }
+    const int64_t p = (int64_t)ws + s;
+    uint32_t* row = a.rec + (size_t)s * kSlots;
+    const bool head = head_qs && live && (p % kWin) == 0;
+    bool snapped = false;
+    for (uint64_t i = b; i < e; i++) {
+        const int64_t start = r.pos[i];
+        const uint32_t m = r.meta[i];
+        if (head && !snapped && start >= p) {
+            head_qs[p / kWin - ws / kWin] = make_int4((int)(qs & 0xffff), (int)((qs >> 16) & 0xffff),
+                                                      (int)((qs >> 32) & 0xffff), (int)(qs >> 48));
+            snapped = true;
+        }
+        const int64_t coord = p - start;
+        const uint32_t len = rm_len(m);
+        if (!live || coord < 0 || coord >= (int64_t)len) continue;
+        const uint32_t hit = rm_hit(m);
+        depth += 1;
+        repeat += (int)hit;
+        const uint32_t byte = r.seq[i * r.pitch + (uint64_t)coord];
+        if (byte == 0 || nrec >= (int)kMaxDepUni) continue;
+        const uint32_t o = pb_base(byte);
+        if (hit == 1) {
+            const uint32_t q = pb_q(byte);
+            const uint32_t strand = rm_strand(m);
+            const uint32_t cyc = strand ? (uint32_t)(len - 1 - coord) : (uint32_t)coord;
+            row[nrec] = rec_encode(cyc, strand, o, q, 1);
+            nrec += 1;
+            cu += 1u << (8 * o);
+            qs += (uint64_t)q << (16 * o);
+        }
+        ca0 += o == 0;
+        ca1 += o == 1;
+        ca2 += o == 2;
+        ca3 += o == 3;
+    }
+    if (!live) return;
+    if (head && !snapped)
+        head_qs[p / kWin - ws / kWin] =
+            make_int4((int)(qs & 0xffff), (int)((qs >> 16) & 0xffff), (int)((qs >> 32) & 0xffff), (int)(qs >> 48));
+    a.depth[s] = depth;
+    a.repeat[s] = repeat;
+    a.nrec[s] = nrec;
+    a.cu[s] = make_int4((int)(cu & 0xff), (int)((cu >> 8) & 0xff), (int)((cu >> 16) & 0xff), (int)(cu >> 24));
+    a.qs[s] = make_int4((int)(qs & 0xffff), (int)((qs >> 16) & 0xffff), (int)((qs >> 32) & 0xffff), (int)(qs >> 48));
+    a.ca[s] = make_int4(ca0, ca1, ca2, ca3);
+    (void)err;
+}
+
+void launch_site_build(hipStream_t st, const ReadView& r, uint64_t r0, uint64_t r1, uint32_t ws, uint32_t n_sites,
+                       int64_t lo_min, int64_t hi_max, uint32_t carry, const SiteArrays& a, int4* head_qs,
+                       uint32_t* err)
+{
+    const uint32_t blocks = (n_sites + 255) / 256;
+    if (blocks)
+        hipLaunchKernelGGL(k_site_build, dim3(blocks), dim3(256), 0, st, r, r0, r1, ws, n_sites, lo_min, hi_max, carry,
+                           a, head_qs, err);
+}
+
+// ---------------------------------------------------------------------------
+// (b) segmented sort.  Sites are bucketed by record count (<=8, 16, 32, 64,
+// 128, 256; 0 or 1 record needs no sort) and each bucket runs a bitonic
+// network in registers: for widths up to 64 a site occupies `W` lanes of a
+// wave, exchanging partners with ds_swizzle (xor < 32) or ds_bpermute
+// (xor 32); widths 128/256 keep 2/4 records per lane and do the xor >= 64
+// steps inside the lane.  Padding lanes hold 0xFFFFFFFF, so ascending u32
+// order (base, quality descending, cycle, strand) comes out in the first n
+// slots, as kernel/bitonic_kernel.cu:119-176 produces.
+
+__global__ __launch_bounds__(256) void k_bucket(const int32_t* nrec, uint32_t n_sites, uint32_t* lists,
+                                                uint32_t pitch, uint32_t* counts)
+{
+    const uint32_t s = blockIdx.x * 256u + threadIdx.x;
+    int b = -1;
+    if (s < n_sites) {
+        const int n = nrec[s];
+        b = n <= 1 ? -1 : n <= 8 ? 0 : n <= 16 ? 1 : n <= 32 ? 2 : n <= 64 ? 3 : n <= 128 ? 4 : 5;
+    }
+    const uint32_t lane = lane_id();
+    const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    for (int k = 0; k < 6; k++) {
+        const uint64_t mask = __ballot(b == k);
+        if (!mask) continue;
+        const uint32_t leader = (uint32_t)__ffsll((long long)mask) - 1;
+        uint32_t base = 0;
+        if (lane == leader) base = atomicAdd(&counts[k], (uint32_t)__popcll(mask));
+        base = __shfl(base, (int)leader);
+        if (b == k) lists[(size_t)k * pitch + base + (uint32_t)__popcll(mask & below)] = s;
+    }
+}
+
+void launch_bucket(hipStream_t st, const int32_t* nrec, uint32_t n_sites, uint32_t* lists, uint32_t pitch,
+                   uint32_t* counts)
+{
+    hipMemsetAsync(counts, 0, 6 * sizeof(uint32_t), st);
+    const uint32_t blocks = (n_sites + 255) / 256;
+    if (blocks) hipLaunchKernelGGL(k_bucket, dim3(blocks), dim3(256), 0, st, nrec, n_sites, lists, pitch, counts);
+}
+
+template <int J>
+__device__ __forceinline__ uint32_t xor_lane(uint32_t v)
+{
+    if constexpr (J == 1) {
+        return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0xB1, 0xF, 0xF, false);  // quad_perm [1,0,3,2]
+    } else if constexpr (J == 2) {
+        return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x4E, 0xF, 0xF, false);  // quad_perm [2,3,0,1]
+    } else if constexpr (J < 32) {
+        return (uint32_t)__builtin_amdgcn_ds_swizzle((int)v, 0x1F | (J << 10));  // bitmask mode, xor J
+    } else {
+        return (uint32_t)__shfl_xor((int)v, J);
+    }
+}
+
+template <int K, int J>
+__device__ __forceinline__ void bitonic_step(uint32_t& v, uint32_t e)
+{
+    const uint32_t w = xor_lane<J>(v);
+    const bool up = (e & K) == 0;
+    const bool low = (e & J) == 0;
+    const uint32_t mn = v < w ? v : w, mx = v < w ? w : v;
+    v = (up == low) ? mn : mx;
+}
+
+template <int K, int J>
+struct Stage {
+    __device__ __forceinline__ static void run(uint32_t& v, uint32_t e)
+    {
+        bitonic_step<K, J>(v, e);
+        if constexpr (J > 1) Stage<K, J / 2>::run(v, e);
+    }
+};
+
+template <int K, int S>
+__device__ __forceinline__ void bitonic_all(uint32_t& v, uint32_t e)
+{
+    Stage<K, K / 2>::run(v, e);
+    if constexpr (K < S) bitonic_all<K * 2, S>(v, e);
+}
+
+// widths 8..64: 64/S sites per wave, one record per lane
+template <int S>
+__global__ __launch_bounds__(256) void k_sort_lanes(uint32_t* rec, const int32_t* nrec, const uint32_t* list,
+                                                    const uint32_t* count_ptr)
+{
+    const uint32_t count = *count_ptr;
+    const uint32_t lane = lane_id();
+    const uint32_t sub = lane / S, e = lane % S;
+    constexpr uint32_t kPerWave = 64 / S;
+    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint32_t nwave = (gridDim.x * blockDim.x) >> 6;
+    for (uint32_t g = wave * kPerWave; g < count; g += nwave * kPerWave) {
+        const uint32_t idx = g + sub;
+        const bool ok = idx < count;
+        uint32_t site = ok ? list[idx] : 0;
+        const uint32_t n = ok ? (uint32_t)nrec[site] : 0;
+        uint32_t* row = rec + (size_t)site * kSlots;
+        uint32_t v = (e < n) ? row[e] : 0xFFFFFFFFu;
+        bitonic_all<2, S>(v, e);
+        if (e < n) row[e] = v;
+    }
+}
+
+// one bitonic merge stage of width K >= 128 over R registers per lane
+// (element e = r*64 + lane): in-lane compare-exchanges for j = K/2 .. 64,
+// then cross-lane j = 32 .. 1; ascending where (e & K) == 0.
+template <int R, int K, int JR>
+__device__ __forceinline__ void regs_inlane(uint32_t* v, uint32_t lane)
+{
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        if (r & JR) continue;
+        const uint32_t e = (uint32_t)r * 64u + lane;
+        const bool up = (e & (uint32_t)K) == 0;
+        const uint32_t a = v[r], b = v[r | JR];
+        const uint32_t mn = a < b ? a : b, mx = a < b ? b : a;
+        v[r] = up ? mn : mx;
+        v[r | JR] = up ? mx : mn;
+    }
+    if constexpr (JR > 1) regs_inlane<R, K, JR / 2>(v, lane);
+}
+
+template <int R, int K>
+__device__ __forceinline__ void regs_merge(uint32_t* v, uint32_t lane)
+{
+    regs_inlane<R, K, K / 128>(v, lane);
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const uint32_t e = (uint32_t)r * 64u + lane;
+        const bool up = (e & (uint32_t)K) == 0;
+        uint32_t x = v[r];
+#define GSNP_XSTEP(J)                                          \
+    {                                                          \
+        const uint32_t w = xor_lane<J>(x);                     \
+        const bool low = (e & (J)) == 0;                       \
+        const uint32_t mn = x < w ? x : w, mx = x < w ? w : x; \
+        x = (up == low) ? mn : mx;                             \
+    }
+        GSNP_XSTEP(32) GSNP_XSTEP(16) GSNP_XSTEP(8) GSNP_XSTEP(4) GSNP_XSTEP(2) GSNP_XSTEP(1)
+#undef GSNP_XSTEP
+        v[r] = x;
+    }
+}
+
+// widths 128/256: one site per wave, R = S/64 records per lane at e = r*64+lane
+template <int S>
+__global__ __launch_bounds__(256) void k_sort_regs(uint32_t* rec, const int32_t* nrec, const uint32_t* list,
+                                                   const uint32_t* count_ptr)
+{
+    constexpr int R = S / 64;
+    const uint32_t count = *count_ptr;
+    const uint32_t lane = lane_id();
+    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint32_t nwave = (gridDim.x * blockDim.x) >> 6;
+    for (uint32_t g = wave; g < count; g += nwave) {
+        const uint32_t site = list[g];
+        const uint32_t n = (uint32_t)nrec[site];
+        uint32_t* row = rec + (size_t)site * kSlots;
+        uint32_t v[R];
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const uint32_t e = r * 64 + lane;
+            v[r] = e < n ? row[e] : 0xFFFFFFFFu;
+        }
+        // k = 2..64: every step is cross-lane with e = r*64+lane, (e & k) from lane bits
+#pragma unroll
+        for (int r = 0; r < R; r++) bitonic_all<2, 64>(v[r], (uint32_t)(r * 64) + lane);
+        // k = 128 (and 256): in-lane steps for j >= 64, then cross-lane j = 32..1
+        regs_merge<R, 128>(v, lane);
+        if constexpr (S == 256) regs_merge<R, 256>(v, lane);
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const uint32_t e = r * 64 + lane;
+            if (e < n) row[e] = v[r];
+        }
+    }
+}
+
+void launch_sort(hipStream_t st, uint32_t* rec, const int32_t* nrec, const uint32_t* lists, uint32_t pitch,
+                 const uint32_t* counts, uint32_t max_sites)
+{
+    if (!max_sites) return;
+    auto grid = [&](uint32_t per_wave) {
+        uint32_t waves = (max_sites + per_wave - 1) / per_wave;
+        uint32_t blocks = (waves + 3) / 4;
+        return blocks < 2048u ? (blocks ? blocks : 1u) : 2048u;
+    };
+    hipLaunchKernelGGL(k_sort_lanes<8>, dim3(grid(8)), dim3(256), 0, st, rec, nrec, lists + 0 * (size_t)pitch, counts + 0);
+    hipLaunchKernelGGL(k_sort_lanes<16>, dim3(grid(4)), dim3(256), 0, st, rec, nrec, lists + 1 * (size_t)pitch, counts + 1);
+    hipLaunchKernelGGL(k_sort_lanes<32>, dim3(grid(2)), dim3(256), 0, st, rec, nrec, lists + 2 * (size_t)pitch, counts + 2);
+    hipLaunchKernelGGL(k_sort_lanes<64>, dim3(grid(1)), dim3(256), 0, st, rec, nrec, lists + 3 * (size_t)pitch, counts + 3);
+    hipLaunchKernelGGL(k_sort_regs<128>, dim3(grid(1)), dim3(256), 0, st, rec, nrec, lists + 4 * (size_t)pitch, counts + 4);
+    hipLaunchKernelGGL(k_sort_regs<256>, dim3(grid(1)), dim3(256), 0, st, rec, nrec, lists + 5 * (size_t)pitch, counts + 5);
+}
+
+// ---------------------------------------------------------------------------
+// (c) likelihood.  One thread per site over its sorted records
+// (kernel/likelihood_kernel.cu:168-256).  Within a base segment, t is the
+// occurrence count of the record's (strand, cycle) and g the number of
+// distinct (strand, cycle) seen so far minus one; the adjusted quality comes
+// from the host-built table and the 10 genotype scores (80 contiguous bytes
+// of the score table) are added in record order.  A per-thread 512-bit LDS
+// set marks the (strand, cycle) keys seen in the segment; only a repeated key
+// rescans the segment to count its earlier occurrences.
+__global__ __launch_bounds__(256) void k_likelihood(Tables t, SiteArrays a, uint32_t n_sites, double* lk,
+                                                    uint32_t* err)
+{
+    __shared__ uint32_t seen_all[256 * 17];
+    uint32_t* seen = seen_all + threadIdx.x * 17;
+    const uint32_t s = blockIdx.x * 256u + threadIdx.x;
+    if (s >= n_sites) return;
+    const uint32_t n = (uint32_t)a.nrec[s];
+    const uint32_t* row = a.rec + (size_t)s * kSlots;
+    double acc[10];
+#pragma unroll
+    for (int g = 0; g < 10; g++) acc[g] = 0.0;
+    int last_o = -1, gdc = -1;
+    uint32_t seg = 0;
+    uint32_t bad = 0;
+    for (uint32_t i = 0; i < n; i++) {
+        const uint32_t w = row[i];
+        const uint32_t o = rec_base(w);
+        uint32_t q = rec_q(w);
+        const uint32_t cyc = rec_cycle(w), occ = rec_occ(w), key = rec_key(w);
+        if ((int)o != last_o) {
+#pragma unroll
+            for (int k = 0; k < 16; k++) seen[k] = 0;
+            gdc = -1;
+            seg = i;
+            last_o = (int)o;
+        }
+        const uint32_t word = seen[key >> 5], bit = 1u << (key & 31);
+        uint32_t prior = 0;
+        if (word & bit) {
+            for (uint32_t j = seg; j < i; j++)
+                if (rec_key(row[j]) == key) prior += rec_occ(row[j]);
+        } else {
+            seen[key >> 5] = word | bit;
+        }
+        if (q > 63) {
+            bad |= kErrQadjRange;
+            q = 63;
+        }
+        for (uint32_t k = 0; k < occ; k++) {
+            if (prior + k == 0) gdc += 1;
+            const uint32_t tm1 = prior + k < (uint32_t)t.td ? prior + k : (uint32_t)t.td - 1;
+            const uint32_t gi = (uint32_t)gdc < (uint32_t)t.gd ? (uint32_t)gdc : (uint32_t)t.gd - 1;
+            uint32_t qa = t.qadj[((size_t)q * t.td + tm1) * t.gd + gi];
+            if (qa == 0) {
+                bad |= kErrQadjRange;
+                qa = 1;
+            }
+            const double2* sr = reinterpret_cast<const double2*>(t.score + (size_t)((qa << 10) | (cyc << 2) | o) * 10);
+#pragma unroll
+            for (int h = 0; h < 5; h++) {
+                const double2 v = sr[h];
+                acc[2 * h] += v.x;
+                acc[2 * h + 1] += v.y;
+            }
+        }
+    }
+    double2* out = reinterpret_cast<double2*>(lk + (size_t)s * 10);
+#pragma unroll
+    for (int h = 0; h < 5; h++) out[h] = make_double2(acc[2 * h], acc[2 * h + 1]);
+    if (bad) atomicOr(err, bad);
+}
+
+void launch_likelihood(hipStream_t st, const Tables& t, const SiteArrays& a, uint32_t n_sites, double* lk, uint32_t* err)
+{
+    const uint32_t blocks = (n_sites + 255) / 256;
+    if (blocks) hipLaunchKernelGGL(k_likelihood, dim3(blocks), dim3(256), 0, st, t, a, n_sites, lk, err);
+}
+
+// ---------------------------------------------------------------------------
+// (d) posterior (kernel/posterior_kernel.cu:635-786 and the q_cns rules of
+// :833-864).  One thread per site.  Priors come as host-computed log10 rows
+// (reference code rows, or a known-SNP row found by binary search).  A het
+// call under the rank-sum mode is handed to k_rank through a job list.
+
+__device__ __forceinline__ int get4(const int4& v, uint32_t i)
+{
+    return i == 0 ? v.x : i == 1 ? v.y : i == 2 ? v.z : v.w;
+}
+
+__global__ __launch_bounds__(256) void k_posterior(Tables t, RefView ref, uint32_t ws, SiteArrays a, uint32_t n_sites,
+                                                   const int4* head_qs, Results out, RankJob* jobs,
+                                                   uint32_t* job_count)
+{
+    const uint32_t s = blockIdx.x * 256u + threadIdx.x;
+    if (s >= n_sites) return;
+    const uint32_t ori = ref.code[ws + s] & 0xF;
+    const int depth = a.depth[s];
+    if ((ori & 4) && depth == 0) return;  // N row, not called (nflag)
+    const int dep_uni = a.nrec[s];
+    const int4 qs = a.qs[s];
+    const int4 ca = a.ca[s];
+    // base1/base2 and qual1..3 (>= keeps the later base on ties)
+    uint32_t base1 = 0, base2 = 0;
+    int qual1 = -1, qual2 = -2, qual3 = -3;
+    const int4 key = dep_uni ? qs : ca;
+    int k1 = dep_uni ? -1 : 0, k2 = dep_uni ? -2 : 0, k3 = dep_uni ? -3 : 0;
+#pragma unroll
+    for (uint32_t i = 0; i < 4; i++) {
+        const int v = get4(key, i);
+        if (v >= k1) {
+            k3 = k2;
+            base2 = base1;
+            k2 = k1;
+            base1 = i;
+            k1 = v;
+        } else if (v >= k2) {
+            k3 = k2;
+            base2 = i;
+            k2 = v;
+        } else if (v >= k3) {
+            k3 = v;
+        }
+    }
+    if (k1 == 0)
+        base1 = ori & 7;
+    else if (k2 == 0 && base1 != (ori & 7))
+        base2 = ori & 7;
+    if (dep_uni) {
+        qual1 = k1;
+        qual2 = k2;
+        qual3 = k3;
+    }
+    // log10 priors
+    const double* lp = t.lprior + (ori & 7) * 10;
+    if ((ori & 8) && t.refine && ref.n_snp) {
+        uint32_t lo = 0, hi = ref.n_snp;
+        const uint32_t p = ws + s;
+        while (lo < hi) {
+            uint32_t m = (lo + hi) >> 1;
+            if (ref.snp_pos[m] < p)
+                lo = m + 1;
+            else
+                hi = m;
+        }
+        if (lo < ref.n_snp && ref.snp_pos[lo] == p) lp = ref.snp_lp + (size_t)lo * 10;
+    }
+    const double* lk = out.lk + (size_t)s * 10;
+    uint32_t type1 = 16, type2 = 16;
+    double tp1 = 0.0, tp2 = 0.0;
+#pragma unroll
+    for (int g = 0; g < 10; g++) {
+        const uint32_t gt = geno_code(g);
+        if (t.monoploid && (gt >> 2) != (gt & 3)) continue;
+        const double v = lk[g] + lp[g];
+        if (v >= tp1 || type1 == 16) {
+            type2 = type1;
+            tp2 = tp1;
+            type1 = gt;
+            tp1 = v;
+        } else if (v >= tp2 || type2 == 16) {
+            type2 = gt;
+            tp2 = v;
+        }
+    }
+    const double dtp = tp1 - tp2;
+    const bool hom = (type1 & 3) == ((type1 >> 2) & 3);
+    // HET cap inputs; q_sum is flat over sites, so base codes 4..7 read the next
+    // site -- for the last site of a window, as it stands before the reads
+    // that start on the window boundary (head_qs in the batched span)
+    auto qsum_flat = [&](uint32_t b) -> int {
+        if (b < 4) return get4(qs, b);
+        const uint32_t nx = ws + s + 1;
+        const int4 nq = (head_qs && nx % kWin == 0) ? head_qs[nx / kWin - ws / kWin] : a.qs[s + 1];
+        return get4(nq, b - 4);
+    };
+    bool het_ok = false;
+    if (!hom) {
+        const uint32_t het = base1 < base2 ? (base1 << 2 | base2) : (base2 << 2 | base1);
+        het_ok = qsum_flat(base1) > 0 && qsum_flat(base2) > 0 && type1 == het;
+    }
+    double rank = 1.0;
+    int q_cns;
+    if (t.rank_mode && !hom) {
+        const int4 cu = a.cu[s];
+        if (get4(cu, type1 & 3) == 0 || get4(cu, (type1 >> 2) & 3) == 0) {
+            rank = 0.0;
+        } else {
+            const uint32_t j = atomicAdd(job_count, 1u);
+            RankJob jb;
+            jb.site = s;
+            jb.type1 = (uint8_t)type1;
+            jb.het_ok = het_ok;
+            jb.pad0 = jb.pad1 = 0;
+            jb.cap = qual2 - qual3;
+            jb.dtp = dtp;
+            jobs[j] = jb;
+            out.base1[s] = (uint8_t)base1;
+            out.base2[s] = (uint8_t)base2;
+            out.type1[s] = (uint8_t)type1;
+            return;  // rank and q_cns come from k_rank
+        }
+    }
+    if (rank == 0.0)
+        q_cns = 0;
+    else
+        q_cns = trunc_int_x86(10 * dtp + 0.0);  // 10*log10(1) == 0
+    if (hom) {
+        if (qual1 > 0 && base1 != (type1 & 3))
+            q_cns = 0;
+        else if (q_cns > qual1 - qual2)
+            q_cns = qual1 - qual2;
+    } else {
+        if (het_ok) {
+            if (q_cns > qual2 - qual3) q_cns = qual2 - qual3;
+        } else {
+            q_cns = 0;
+        }
+    }
+    if (q_cns > 99) q_cns = 99;
+    if (q_cns < 0) q_cns = 0;
+    out.base1[s] = (uint8_t)base1;
+    out.base2[s] = (uint8_t)base2;
+    out.type1[s] = (uint8_t)type1;
+    out.q_cns[s] = q_cns;
+    out.rank[s] = rank;
+}
+
+void launch_posterior(hipStream_t st, const Tables& t, const RefView& ref, uint32_t ws, const SiteArrays& a,
+                      uint32_t n_sites, const int4* head_qs, Results& out, RankJob* jobs, uint32_t* job_count)
+{
+    const uint32_t blocks = (n_sites + 255) / 256;
+    if (blocks)
+        hipLaunchKernelGGL(k_posterior, dim3(blocks), dim3(256), 0, st, t, ref, ws, a, n_sites, head_qs, out, jobs,
+                           job_count);
+}
+
+// ---------------------------------------------------------------------------
+// (e) Wilcoxon rank-sum test on het calls (kernel/posterior_kernel.cu:506-572,
+// cpp/rank_sum.cc:51-58).  One wave per job: lane q counts the records of the
+// two alleles with quality q, a wave prefix sum gives the mid-ranks, and
+// rank sums T1/T2 (exact half-integers) are reduced across lanes.  N < 64
+// uses the exact table (including its <<16 interpolation index); N >= 64
+// (normal approximation, host exp) and any log10 within 1e-9 of an integer
+// boundary go to the host fix-up list.
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    return v;
+}
+
+__global__ __launch_bounds__(256) void k_rank(Tables t, SiteArrays a, const RankJob* jobs, const uint32_t* job_count,
+                                              Results out, FixJob* fixes, uint32_t* fix_count)
+{
+    const uint32_t count = *job_count;
+    const uint32_t lane = lane_id();
+    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint32_t nwave = (gridDim.x * blockDim.x) >> 6;
+    for (uint32_t j = wave; j < count; j += nwave) {
+        const RankJob jb = jobs[j];
+        const uint32_t s = jb.site;
+        const uint32_t A = jb.type1 & 3, B = (jb.type1 >> 2) & 3;  // n1 allele, n2 allele
+        const uint32_t n = (uint32_t)a.nrec[s];
+        const uint32_t* row = a.rec + (size_t)s * kSlots;
+        // same-quality counts: lane q
+        int cnt = 0;
+        for (uint32_t i = 0; i < n; i++) {
+            const uint32_t w = row[i];
+            const uint32_t o = rec_base(w);
+            if ((o == A || o == B) && rec_q(w) == lane) cnt += (int)rec_occ(w);
+        }
+        // exclusive prefix over lanes 0..q (only q <= q_max_min+1 take part)
+        const bool part = (int)lane <= t.q_max_min + 1;
+        int incl = part ? cnt : 0;
+        for (int off = 1; off < 64; off <<= 1) {
+            const int y = __shfl_up(incl, off);
+            if ((int)lane >= off) incl += y;
+        }
+        const int excl = incl - (part ? cnt : 0);
+        const double rank_q = part ? excl + (1 + cnt) / 2.0 : 0.0;
+        double ta = 0.0, tb = 0.0;
+        for (uint32_t i0 = 0; i0 < n; i0 += 64) {
+            const uint32_t i = i0 + lane;
+            const uint32_t w = i < n ? row[i] : 0u;
+            const uint32_t q = rec_q(w) & 63;
+            const double rq = __shfl(rank_q, (int)q);
+            if (i < n) {
+                const uint32_t o = rec_base(w);
+                const double c = rq * rec_occ(w);
+                if (o == A) ta += c;
+                if (o == B) tb += c;
+            }
+        }
+        ta = wave_sum(ta);
+        tb = wave_sum(tb);
+        if (lane != 0) continue;
+        const int4 cu = a.cu[s];
+        const int n1 = get4(cu, A), n2 = get4(cu, B);
+        bool need_host = false;
+        double rank = 0.0;
+        if (n1 + n2 < 64) {
+            const double* p = t.prank;
+            if (n1 <= n2) {
+                const int i1 = (n1 + n2) << 17 | n1 << 11 | (int)(ta);
+                const int i2 = (n1 + n2) << 16 | n1 << 11 | (int)(ta + 1);
+                rank = p[i1] + (ta - (int)ta) * (p[i2] - p[i1]);
+            } else {
+                const int i1 = (n1 + n2) << 17 | n2 << 11 | (int)(tb);
+                const int i2 = (n1 + n2) << 16 | n2 << 11 | (int)(tb + 1);
+                rank = p[i1] + (tb - (int)tb) * (p[i2] - p[i1]);
+            }
+        } else {
+            need_host = true;
+        }
+        int q_cns = 0;
+        if (!need_host) {
+            if (rank == 0.0) {
+                q_cns = 0;
+            } else {
+                const double l = rank == 1.0 ? 0.0 : log10(rank);
+                const double y = 10 * jb.dtp + 10 * l;
+                if (y == y) {  // NaN gives INT_MIN on both sides, no guard needed
+                    const double r = rint(y);
+                    const double band = 1e-9 * (1.0 + fabs(y) + fabs(10 * l));
+                    if (fabs(y - r) <= band || !(fabs(y) < 2.0e9)) need_host = true;
+                }
+                q_cns = trunc_int_x86(y);
+            }
+        }
+        if (need_host) {
+            const uint32_t f = atomicAdd(fix_count, 1u);
+            FixJob fx;
+            fx.site = s;
+            fx.n1 = n1;
+            fx.n2 = n2;
+            fx.het_ok = jb.het_ok;
+            fx.normal = n1 + n2 >= 64;
+            fx.pad0 = fx.pad1 = 0;
+            fx.cap = jb.cap;
+            fx.T1 = ta;
+            fx.T2 = tb;
+            fx.dtp = jb.dtp;
+            fx.table_rank = rank;
+            fixes[f] = fx;
+            continue;
+        }
+        if (jb.het_ok) {
+            if (q_cns > jb.cap) q_cns = jb.cap;
+        } else {
+            q_cns = 0;
+        }
+        if (q_cns > 99) q_cns = 99;
+        if (q_cns < 0) q_cns = 0;
+        out.q_cns[s] = q_cns;
+        out.rank[s] = rank;
+    }
+}
+
+void launch_rank(hipStream_t st, const Tables& t, const SiteArrays& a, const RankJob* jobs, const uint32_t* job_count,
+                 uint32_t max_jobs, Results& out, FixJob* fixes, uint32_t* fix_count)
+{
+    uint32_t blocks = (max_jobs + 3) / 4;
+    if (blocks > 4096) blocks = 4096;
+    if (blocks == 0) blocks = 1;
+    hipLaunchKernelGGL(k_rank, dim3(blocks), dim3(256), 0, st, t, a, jobs, job_count, out, fixes, fix_count);
+}
+
+__global__ void k_apply_fix(const FixJob* fixes, const double* rank, const int32_t* qcns, uint32_t n, Results out)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t s = fixes[i].site;
+    out.rank[s] = rank[i];
+    out.q_cns[s] = qcns[i];
+}
+
+void launch_apply_fix(hipStream_t st, const FixJob* fixes, const double* rank, const int32_t* qcns, uint32_t n,
+                      Results& out)
+{
+    if (n) hipLaunchKernelGGL(k_apply_fix, dim3((n + 255) / 256), dim3(256), 0, st, fixes, rank, qcns, n, out);
+}
+
+// ---------------------------------------------------------------------------
+// (f) .cns rows (cpp/call_genotype.cc:353-411).  k_rowlen sizes each row and
+// sums per block; a one-block scan turns block sums into offsets; k_format
+// writes each block's rows into LDS at their block-local offsets and copies
+// the contiguous block text out with wide stores (rows of very long
+// chromosome names fall back to direct stores).
+
+struct RowData {
+    bool nrow;
+    uint32_t ori, type1, b1, b2;
+    int q_cns, depth, repeat;
+    int avg1, cu1, ca1, avg2, cu2, ca2;
+    double rank;
+};
+
+__device__ __forceinline__ RowData load_row(const RefView& ref, uint32_t ws, const SiteArrays& a, const Results& r,
+                                            uint32_t s)
+{
+    RowData d;
+    d.ori = ref.code[ws + s] & 0xF;
+    d.depth = a.depth[s];
+    d.nrow = (d.ori & 4) && d.depth == 0;
+    d.b1 = d.nrow ? 4 : r.base1[s];
+    d.nrow = d.nrow || d.b1 >= 4;
+    if (d.nrow) return d;
+    d.b2 = r.base2[s];
+    d.type1 = r.type1[s];
+    d.q_cns = r.q_cns[s];
+    d.repeat = a.repeat[s];
+    d.rank = r.rank[s];
+    const int4 qs = a.qs[s], cu = a.cu[s], ca = a.ca[s];
+    const int q1 = get4(qs, d.b1);
+    d.cu1 = get4(cu, d.b1);
+    d.ca1 = get4(ca, d.b1);
+    d.avg1 = q1 == 0 ? 0 : q1 / d.cu1;
+    if (d.b2 < 4) {
+        const int q2 = get4(qs, d.b2);
+        d.cu2 = get4(cu, d.b2);
+        d.ca2 = get4(ca, d.b2);
+        d.avg2 = q2 == 0 ? 0 : q2 / d.cu2;
+    }
+    return d;
+}
+
+__constant__ char kNRow[] = "\tN\tN\t0\tN\t0\t0\t0\tN\t0\t0\t0\t0\t1.000\t255.000\t0\n";
+constexpr int kNRowLen = sizeof(kNRow) - 1;
+__constant__ char kACTGN[] = "ACTGNNNN";
+__constant__ char kAbbv[] = "AMWRMCYSWYTKRSKGN";
+
+// Writes the row through `put(pos, char)`; returns its length.
+template <class Put>
+__device__ __forceinline__ int emit_row(const RowData& d, const RowCtx& rc, uint32_t pos1, Put put)
+{
+    int n = 0;
+    for (int i = 0; i < rc.name_len; i++) put(n++, rc.name[i]);
+    put(n++, '\t');
+    char buf[24];
+    int k = fmt_u32(pos1, buf);
+    for (int i = 0; i < k; i++) put(n++, buf[i]);
+    if (d.nrow) {
+        for (int i = 0; i < kNRowLen; i++) put(n++, kNRow[i]);
+        return n;
+    }
+    put(n++, '\t');
+    put(n++, kACTGN[d.ori & 7]);
+    put(n++, '\t');
+    put(n++, kAbbv[d.type1]);
+    put(n++, '\t');
+    k = fmt_i32(d.q_cns, buf);
+    for (int i = 0; i < k; i++) put(n++, buf[i]);
+    put(n++, '\t');
+    put(n++, kACTGN[d.b1]);
+    const int v1[3] = {d.avg1, d.cu1, d.ca1};
+    for (int j = 0; j < 3; j++) {
+        put(n++, '\t');
+        k = fmt_i32(v1[j], buf);
+        for (int i = 0; i < k; i++) put(n++, buf[i]);
+    }
+    put(n++, '\t');
+    if (d.b2 < 4) {
+        put(n++, kACTGN[d.b2]);
+        const int v2[3] = {d.avg2, d.cu2, d.ca2};
+        for (int j = 0; j < 3; j++) {
+            put(n++, '\t');
+            k = fmt_i32(v2[j], buf);
+            for (int i = 0; i < k; i++) put(n++, buf[i]);
+        }
+    } else {
+        put(n++, 'N');
+        for (int j = 0; j < 3; j++) {
+            put(n++, '\t');
+            put(n++, '0');
+        }
+    }
+    put(n++, '\t');
+    k = fmt_i32(d.depth, buf);
+    for (int i = 0; i < k; i++) put(n++, buf[i]);
+    put(n++, '\t');
+    k = fmt_g(d.rank, buf);
+    for (int i = 0; i < k; i++) put(n++, buf[i]);
+    put(n++, '\t');
+    k = fmt_g(d.depth == 0 ? 255.0 : (double)d.repeat / d.depth, buf);
+    for (int i = 0; i < k; i++) put(n++, buf[i]);
+    put(n++, '\t');
+    put(n++, (d.ori & 8) ? '1' : '0');
+    put(n++, '\n');
+    return n;
+}
+
+__device__ __forceinline__ uint64_t block_sum_u64(uint64_t v, uint64_t* red)
+{
+    const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    if (lane == 0) red[w] = v;
+    __syncthreads();
+    uint64_t t = 0;
+    for (uint32_t i = 0; i < blockDim.x / 64; i++) t += red[i];
+    return t;
+}
+
+__global__ __launch_bounds__(256) void k_rowlen(RefView ref, uint32_t ws, SiteArrays a, Results res, uint32_t n_sites,
+                                                RowCtx rc, uint32_t* rowlen, uint64_t* block_sum)
+{
+    __shared__ uint64_t red[4];
+    const uint32_t s = blockIdx.x * 256u + threadIdx.x;
+    uint32_t len = 0;
+    if (s < n_sites) {
+        const RowData d = load_row(ref, ws, a, res, s);
+        len = (uint32_t)emit_row(d, rc, rc.pos_base + s, [](int, char) {});
+        rowlen[s] = len;
+    }
+    const uint64_t tot = block_sum_u64(len, red);
+    if (threadIdx.x == 0) block_sum[blockIdx.x] = tot;
+}
+
+void launch_rowlen(hipStream_t st, const RefView& ref, uint32_t ws, const SiteArrays& a, const Results& res,
+                   uint32_t n_sites, RowCtx rc, uint32_t* rowlen, uint64_t* block_sum)
+{
+    const uint32_t blocks = (n_sites + 255) / 256;
+    if (blocks)
+        hipLaunchKernelGGL(k_rowlen, dim3(blocks), dim3(256), 0, st, ref, ws, a, res, n_sites, rc, rowlen, block_sum);
+}
+
+// exclusive scan of block sums in place (one block of 1024 threads); writes the total
+__global__ __launch_bounds__(1024) void k_scan_blocks(uint64_t* v, uint32_t n, uint64_t* total)
+{
+    __shared__ uint64_t wsum[16];
+    __shared__ uint64_t carry;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
+    for (uint32_t base = 0; base < n; base += 1024) {
+        const uint32_t i = base + threadIdx.x;
+        const uint64_t x = i < n ? v[i] : 0;
+        uint64_t incl = x;
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint64_t y = __shfl_up(incl, off);
+            if ((int)lane >= off) incl += y;
+        }
+        if (lane == 63) wsum[w] = incl;
+        __syncthreads();
+        uint64_t before = 0;
+        for (uint32_t k = 0; k < w; k++) before += wsum[k];
+        const uint64_t c = carry;
+        if (i < n) v[i] = c + before + incl - x;
+        __syncthreads();
+        if (threadIdx.x == 1023) carry = c + before + incl;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *total = carry;
+}
+
+void launch_scan_blocks(hipStream_t st, uint64_t* block_sum, uint32_t n_blocks, uint64_t* total)
+{
+    hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(1024), 0, st, block_sum, n_blocks, total);
+}
+
+constexpr uint32_t kFmtLds = 40960;
+
+__global__ __launch_bounds__(256) void k_format(RefView ref, uint32_t ws, SiteArrays a, Results res, uint32_t n_sites,
+                                                RowCtx rc, const uint32_t* rowlen, const uint64_t* block_off, char* out)
+{
+    __shared__ __align__(16) char stage[kFmtLds];
+    __shared__ uint32_t wsum[4];
+    const uint32_t s = blockIdx.x * 256u + threadIdx.x;
+    const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
+    const uint32_t len = s < n_sites ? rowlen[s] : 0;
+    uint32_t incl = len;
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t y = __shfl_up(incl, off);
+        if ((int)lane >= off) incl += y;
+    }
+    if (lane == 63) wsum[w] = incl;
+    __syncthreads();
+    uint32_t before = 0, total = 0;
+    for (uint32_t k = 0; k < 4; k++) {
+        if (k < w) before += wsum[k];
+        total += wsum[k];
+    }
+    const uint32_t my = before + incl - len;
+    const uint64_t base = block_off[blockIdx.x];
+    if (total <= kFmtLds) {
+        if (s < n_sites) {
+            const RowData d = load_row(ref, ws, a, res, s);
+            emit_row(d, rc, rc.pos_base + s, [&](int i, char c) { stage[my + i] = c; });
+        }
+        __syncthreads();
+        // copy stage[0, total) to out[base, base+total): bytes up to 4-byte alignment, then words
+        const uint32_t head = (uint32_t)((4 - (base & 3)) & 3) < total ? (uint32_t)((4 - (base & 3)) & 3) : total;
+        for (uint32_t i = threadIdx.x; i < head; i += 256) out[base + i] = stage[i];
+        const uint32_t nw = (total - head) / 4;
+        uint32_t* dst = reinterpret_cast<uint32_t*>(out + base + head);
+        for (uint32_t i = threadIdx.x; i < nw; i += 256) {
+            const uint32_t o = head + 4 * i;
+            const uint32_t x = (uint32_t)(uint8_t)stage[o] | ((uint32_t)(uint8_t)stage[o + 1] << 8) |
+                               ((uint32_t)(uint8_t)stage[o + 2] << 16) | ((uint32_t)(uint8_t)stage[o + 3] << 24);
+            dst[i] = x;
+        }
+        for (uint32_t i = head + 4 * nw + threadIdx.x; i < total; i += 256) out[base + i] = stage[i];
+    } else if (s < n_sites) {
+        const RowData d = load_row(ref, ws, a, res, s);
+        char* o = out + base + my;
+        emit_row(d, rc, rc.pos_base + s, [&](int i, char c) { o[i] = c; });
+    }
+}
+
+void launch_format(hipStream_t st, const RefView& ref, uint32_t ws, const SiteArrays& a, const Results& res,
+                   uint32_t n_sites, RowCtx rc, const uint32_t* rowlen, const uint64_t* block_off, char* out)
+{
+    const uint32_t blocks = (n_sites + 255) / 256;
+    if (blocks)
+        hipLaunchKernelGGL(k_format, dim3(blocks), dim3(256), 0, st, ref, ws, a, res, n_sites, rc, rowlen, block_off,
+                           out);
+}
+
+// ---------------------------------------------------------------------------
+// window carry-over (kernel/recycle_kernel.cu:91-136): the read-length tail
+// [win, win+L) moves to [0, L), records included.
+__global__ __launch_bounds__(256) void k_recycle(SiteArrays a, uint32_t win, uint32_t L)
+{
+    const uint32_t i = blockIdx.x;  // one block per tail site
+    if (i >= L) return;
+    const uint32_t src = win + i;
+    const int n = a.nrec[src];
+    for (int k = threadIdx.x; k < n; k += blockDim.x) a.rec[(size_t)i * kSlots + k] = a.rec[(size_t)src * kSlots + k];
+    if (threadIdx.x == 0) {
+        a.nrec[i] = n;
+        a.depth[i] = a.depth[src];
+        a.repeat[i] = a.repeat[src];
+        a.cu[i] = a.cu[src];
+        a.qs[i] = a.qs[src];
+        a.ca[i] = a.ca[src];
+    }
+}
+
+void launch_recycle(hipStream_t st, const SiteArrays& a, uint32_t win, uint32_t L)
+{
+    if (L) hipLaunchKernelGGL(k_recycle, dim3(L), dim3(256), 0, st, a, win, L);
+}
+
+}  // namespace gsnp

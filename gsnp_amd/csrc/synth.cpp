@@ -13,6 +13,9 @@
 //      blocks at depth 200-255 with hits 1-3, 2% uncovered
 //   4  2 Mb 30x, errors from q - delta, delta = 3[q>30] + 2(c-1)/99
 //   5  248 Mb 30x 150 bp pairs (insert 400+-40), q decaying by cycle
+//   6  stress: 3 contigs (560 kb, 300 kb, 90 kb) with N runs, depth blocks from
+//      0 to 300x (past the 255 record cap), hits 0-3, N and low-quality read
+//      bases, insertion/deletion edits, 75 bp reads, a known-SNP file
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
@@ -76,6 +79,8 @@ struct Contig {
 };
 
 struct Read {
+    int indel;      // SOAP mismatch code > 100 (deletion) or > 200 (insertion), 0 if none
+    int indel_pos;
     int32_t pos;  // 0-based
     uint8_t strand;
     uint8_t hit;
@@ -97,6 +102,7 @@ struct Cfg {
     bool miscal = false; // config 4 error model
     bool gamma_depth = false;
     bool dbsnp = false;
+    bool stress = false;
 };
 
 int draw_q(Xoshiro256ss& g, const Cfg& c, int cycle)
@@ -150,6 +156,8 @@ void emit_read(Xoshiro256ss& g, const Cfg& c, const Contig& ct, int hap, int64_t
                uint8_t hit, uint32_t id, std::vector<Read>& out)
 {
     Read r;
+    r.indel = 0;
+    r.indel_pos = 0;
     r.pos = (int32_t)pos;
     r.strand = strand;
     r.hit = hit;
@@ -164,9 +172,15 @@ void emit_read(Xoshiro256ss& g, const Cfg& c, const Contig& ct, int hap, int64_t
         double qe = q;
         if (c.miscal) qe = q - (3.0 * (q > 30) + 2.0 * cycle / 99.0);
         uint8_t b = ct.hap[hap][pos + k];
+        if (b == 7) b = (uint8_t)g.below(4);  // reads over reference N carry real bases
         if (g.uniform() < std::pow(10.0, -qe / 10.0)) b = (uint8_t)((b + 1 + g.below(3)) & 3);
         r.seq[k] = kBase[b];
         r.qual[k] = (char)(64 + q);
+        if (c.stress) {
+            double u = g.uniform();
+            if (u < 0.01) r.seq[k] = 'N';
+            else if (u < 0.02) r.qual[k] = (char)(64 - 1 - (int)g.below(3));  // below q_min: not observed
+        }
     }
     out.push_back(std::move(r));
 }
@@ -189,6 +203,8 @@ void reads_for_span(Xoshiro256ss& g, const Cfg& c, const Contig& ct, uint64_t lo
             hit = (uint8_t)(1 + g.below(3));
         else if (c.frac_hit2 > 0 && g.uniform() < c.frac_hit2)
             hit = 2;
+        else if (c.stress && g.uniform() < 0.02)
+            hit = (uint8_t)(g.below(2) ? 0 : 3);
         if (!c.paired) {
             int64_t pos = (int64_t)(lo + g.below(hi - lo));
             if (pos + L + 1 > (int64_t)len) pos = (int64_t)len - L - 1;
@@ -246,6 +262,14 @@ Cfg config_for(int id, double scale, Xoshiro256ss& g)
         c.insert_sd = 40;
         c.qual_model = 2;
         break;
+    case 6:
+        c.lens = {sc(560000), sc(300000), sc(90000)};
+        c.read_len = 75;
+        c.depth = 25;
+        c.stress = true;
+        c.dbsnp = true;
+        c.qual_model = 1;
+        break;
     default:
         c.lens = {sc(100000)};
     }
@@ -276,15 +300,28 @@ long long gsnp_synth_write(int config, unsigned long long seed, double scale, co
         Contig ct;
         ct.name = c.lens.size() == 1 ? std::string("chr1") : ("chr" + std::to_string(ci + 1));
         make_contig(g, ct, c.lens[ci]);
+        if (c.stress) {  // a few N runs in the reference (code 7 marks 'N' when written)
+            for (int k = 0; k < 4; k++) {
+                uint64_t at = g.below(ct.ref.size() - 2000), run = 50 + g.below(1500);
+                for (uint64_t i = at; i < at + run; i++) ct.ref[i] = 7;
+            }
+        }
         std::fprintf(fa, ">%s\n", ct.name.c_str());
         for (uint64_t i = 0; i < ct.ref.size(); i += 60) {
             uint64_t m = std::min<uint64_t>(60, ct.ref.size() - i);
-            for (uint64_t k = 0; k < m; k++) line[k] = kBase[ct.ref[i + k]];
+            for (uint64_t k = 0; k < m; k++) line[k] = ct.ref[i + k] == 7 ? 'N' : kBase[ct.ref[i + k]];
             line[m] = '\n';
             std::fwrite(line.data(), 1, m + 1, fa);
         }
         std::vector<Read> reads;
-        if (c.gamma_depth) {
+        if (c.stress) {
+            for (uint64_t lo = 0; lo < ct.ref.size(); lo += 5000) {
+                uint64_t hi = std::min<uint64_t>(ct.ref.size(), lo + 5000);
+                double u = g.uniform();
+                double d = u < 0.05 ? 0 : u < 0.12 ? 260 + (double)g.below(40) : u < 0.2 ? 80 + (double)g.below(60) : 5 + 30 * g.uniform();
+                reads_for_span(g, c, ct, lo, hi, d, u >= 0.05 && u < 0.09, id, reads);
+            }
+        } else if (c.gamma_depth) {
             for (uint64_t lo = 0; lo < ct.ref.size(); lo += 10000) {
                 uint64_t hi = std::min<uint64_t>(ct.ref.size(), lo + 10000);
                 double u = g.uniform();
@@ -298,16 +335,28 @@ long long gsnp_synth_write(int config, unsigned long long seed, double scale, co
             reads_for_span(g, c, ct, 0, ct.ref.size(), c.depth, false, id, reads);
         }
         std::stable_sort(reads.begin(), reads.end(), [](const Read& a, const Read& b) { return a.pos < b.pos; });
+        for (Read& r : reads) {
+            if (c.stress && g.uniform() < 0.01) {
+                r.indel = g.below(2) ? 100 + 1 + (int)g.below(3) : 200 + 1 + (int)g.below(3);
+                r.indel_pos = 5 + (int)g.below(c.read_len - 10);
+            }
+        }
         for (const Read& r : reads) {
             int mm = 0;
             for (int k = 0; k < c.read_len; k++) {
-                char rb = kBase[ct.ref[r.pos + k]];
+                char rb = ct.ref[r.pos + k] == 7 ? 'N' : kBase[ct.ref[r.pos + k]];
                 mm += r.seq[k] != rb;
             }
             if (mm > 99) mm = 99;
-            int n = std::snprintf(line.data(), line.size(), "r%u/%c\t%s\t%s\t%d\t%c\t%d\t%c\t%s\t%d\t%d\n", r.id,
-                                  r.ab == 'a' ? '1' : '2', r.seq.c_str(), r.qual.c_str(), (int)r.hit, r.ab,
-                                  c.read_len, r.strand ? '-' : '+', ct.name.c_str(), r.pos + 1, mm);
+            int n;
+            if (r.indel)
+                n = std::snprintf(line.data(), line.size(), "r%u/%c\t%s\t%s\t%d\t%c\t%d\t%c\t%s\t%d\t%d\t%d\n", r.id,
+                                  r.ab == 'a' ? '1' : '2', r.seq.c_str(), r.qual.c_str(), (int)r.hit, r.ab, c.read_len,
+                                  r.strand ? '-' : '+', ct.name.c_str(), r.pos + 1, r.indel, r.indel_pos);
+            else
+                n = std::snprintf(line.data(), line.size(), "r%u/%c\t%s\t%s\t%d\t%c\t%d\t%c\t%s\t%d\t%d\n", r.id,
+                                  r.ab == 'a' ? '1' : '2', r.seq.c_str(), r.qual.c_str(), (int)r.hit, r.ab, c.read_len,
+                                  r.strand ? '-' : '+', ct.name.c_str(), r.pos + 1, mm);
             std::fwrite(line.data(), 1, n, so);
             nlines++;
         }
@@ -323,6 +372,7 @@ long long gsnp_synth_write(int config, unsigned long long seed, double scale, co
             pos.erase(std::unique(pos.begin(), pos.end()), pos.end());
             for (uint64_t p : pos) {
                 uint8_t r0 = ct.ref[p];
+                if (r0 > 3) continue;  // no known SNPs on reference N
                 uint8_t alt = ct.hap[0][p] != r0 ? ct.hap[0][p] : (ct.hap[1][p] != r0 ? ct.hap[1][p] : mutate(g, r0));
                 double f[4] = {0, 0, 0, 0};
                 double af = 0.05 + 0.45 * g.uniform();

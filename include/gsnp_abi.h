@@ -1,0 +1,197 @@
+/*
+ * gsnp_abi.h -- C ABI of the MI355X SOAPsnp calling library (libgsnp.so).
+ *
+ * This is the boundary the reference's window driver (cpp/call_genotype.cc)
+ * crosses into the GPU.  Each entry point names the reference interface it
+ * replaces (paths relative to the reference root).  The reference passes
+ * implicit process-global device state; here an explicit context owns all
+ * device memory.  Every function returns GSNP_OK or an error code, with a
+ * message from gsnp_last_error().  A context is used by one host thread at a
+ * time (the reference is single-threaded and not reentrant either).
+ *
+ * Host pointers are plain caller-owned arrays; device memory is owned by the
+ * context.  No torch or HIP types appear in the signatures.
+ */
+#ifndef GSNP_ABI_H
+#define GSNP_ABI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* error codes (the reference aborts instead; cpp/gsnp.h has no status codes) */
+#define GSNP_OK 0
+#define GSNP_E_IO 1
+#define GSNP_E_FORMAT 2
+#define GSNP_E_UNSORTED 3   /* "Errors in sorting", cpp/call_genotype.cc:516-519 */
+#define GSNP_E_CHROM 4      /* "Chromosome NOT found", cpp/matrix.cc:71-78 */
+#define GSNP_E_QUAL 5       /* unique-hit base quality outside [1,63] (record field overflow) */
+#define GSNP_E_RANGE 6      /* read length, hit count, adjusted quality out of range */
+#define GSNP_E_NOMEM 7
+#define GSNP_E_ARG 8
+#define GSNP_E_DEVICE 9     /* HIP runtime error */
+
+#define GSNP_WIN 262144     /* sites per window, cpp/common.h:16 */
+
+/* Calling parameters; mirrors Parameter, cpp/common.h:123-154 (defaults :142-153). */
+typedef struct gsnp_params {
+    int q_min;               /* char standing for quality 0 (-z), default 64 */
+    int q_max;               /* q_min + 40 (-Q) */
+    int read_length;         /* -L, default 45 */
+    int is_monoploid;        /* -m */
+    int refine_mode;         /* -2 */
+    int rank_sum_mode;       /* -u */
+    int transition_dominant; /* -t */
+    double althom_novel_r;   /* -r */
+    double het_novel_r;      /* -e */
+    double althom_val_r;
+    double het_val_r;
+    double althom_unval_r;
+    double het_unval_r;
+    double global_dependency; /* log10 scale (-g value is log10'd, cpp/main.cc:151-155) */
+    double pcr_dependency;    /* log10 scale (-p, cpp/main.cc:156-160) */
+} gsnp_params;
+
+/* Known-SNP entry; mirrors Snp_info, cpp/soap_snp.h:714-761. freq is A C T G. */
+typedef struct gsnp_snp {
+    uint8_t hapmap, validated, indel, pad;
+    double freq[4];
+} gsnp_snp;
+
+/* A batch of alignments for one chromosome, structure of arrays; mirrors
+ * st_update, cpp/gsnp.h:134-310.  Reads must be position sorted and already
+ * filtered to pos >= 0 and pos + len < chromosome length
+ * (cpp/call_genotype.cc:495-514).  seq/qual hold `pitch` chars per read. */
+typedef struct gsnp_read_batch {
+    uint64_t n;
+    uint32_t pitch;
+    const int32_t* pos;      /* 0-based */
+    const int32_t* len_hit;  /* (len << 16) | hit, cpp/gsnp.h:199-202 */
+    const char* strand;      /* '+' forward, anything else reverse */
+    const char* seq;
+    const char* qual;
+} gsnp_read_batch;
+
+/* Per-site results of one called window (host copies). */
+typedef struct gsnp_window_out {
+    uint8_t* base1;     /* may be NULL for any field not wanted */
+    uint8_t* base2;
+    uint8_t* type1;
+    int32_t* q_cns;
+    double* rank_sum;
+    double* lk10;       /* 10 genotype log-likelihoods per site */
+    int32_t* depth;
+    int32_t* dep_uni;
+} gsnp_window_out;
+
+typedef struct gsnp_ctx gsnp_ctx;
+
+void gsnp_params_default(gsnp_params* p);
+const char* gsnp_last_error(void);
+
+/* Select the HIP device.  Replaces GPUInit, cpp/gsnp.h:26 (kernel/gsnp_kernel.cu:20-22). */
+int gsnp_device_init(int device);
+
+/* Build the score table, adjusted-quality table, priors and rank-sum table
+ * and allocate window buffers.  p_matrix is 256*256*16 doubles indexed
+ * q<<12|cycle<<4|ref<<2|obs.  Replaces GSNPKernelInit, cpp/gsnp.h:127-129
+ * (kernel/gsnp_kernel.cu:24-33). */
+int gsnp_create(const gsnp_params* p, const double* p_matrix, gsnp_ctx** out);
+
+/* Replaces GSNPKernelFinalize, kernel/gsnp_kernel.cu:36-42. */
+void gsnp_destroy(gsnp_ctx* ctx);
+
+/* Reference sequence of the chromosome to call, as 4-bit codes per base
+ * (cpp/soap_snp.h:763-789: bit3 known SNP, bit2 N, bits1:0 A0 C1 T2 G3), and
+ * its known-SNP entries (sorted positions).  Replaces the per-window host
+ * gathers of get_bin_base and find_snp, cpp/call_genotype.cc:222-262. */
+int gsnp_set_chromosome(gsnp_ctx* ctx, const char* name, const uint8_t* codes, uint32_t len,
+                        const uint32_t* snp_pos, const gsnp_snp* snps, uint32_t n_snp);
+
+/* Validate, pack and upload the chromosome's reads into device memory
+ * (2-bit base + 6-bit quality per byte).  Replaces the per-window
+ * ~218 MB st_update copy of cuda_updateSparse, kernel/update_kernel.cu:346-348. */
+int gsnp_load_reads(gsnp_ctx* ctx, const gsnp_read_batch* batch);
+
+/* Reset window state to start at `start`.  Replaces cuda_init, cpp/gsnp.h:492-498. */
+int gsnp_window_init(gsnp_ctx* ctx, uint32_t start);
+
+/* Add reads [r0, r1) of the loaded batch (all starting inside the current
+ * window) to the site matrix; returns the largest record count per site.
+ * Replaces gold_pos_match + cuda_updateSparse, cpp/gsnp.h:385, :403-417. */
+int gsnp_window_update(gsnp_ctx* ctx, uint64_t r0, uint64_t r1, uint32_t* max_records);
+
+/* Segmented sort of each site's records and genotype likelihoods for sites
+ * [0, call_length).  Replaces cuda_likelihoodSparse, cpp/gsnp.h:398-401. */
+int gsnp_window_likelihood(gsnp_ctx* ctx, uint32_t call_length);
+
+/* Posterior, best/second genotype, consensus quality and (-u) rank-sum test.
+ * Replaces cuda_posterior, cpp/gsnp.h:470-489. */
+int gsnp_window_posterior(gsnp_ctx* ctx, uint32_t call_length);
+
+/* Format the window's .cns rows in device memory; *nbytes receives their
+ * size.  Replaces the text branch of Call_win::call_cns2,
+ * cpp/call_genotype.cc:353-411. */
+int gsnp_window_text(gsnp_ctx* ctx, uint32_t call_length, uint64_t* nbytes);
+
+/* Copy the last formatted rows (gsnp_window_text / gsnp_batched_call) to host. */
+int gsnp_fetch_text(gsnp_ctx* ctx, char* dst, uint64_t cap, uint64_t* nbytes);
+
+/* Copy per-site results of the last called window to host. */
+int gsnp_window_results(gsnp_ctx* ctx, uint32_t call_length, const gsnp_window_out* out);
+
+/* Carry the read-length tail to the front of the next window.  Replaces
+ * cuda_recycle / cuda_quickRecycle, cpp/gsnp.h:499-516. */
+int gsnp_window_recycle(gsnp_ctx* ctx);
+
+/* Batched form: call sites [start, start+n_sites) of the chromosome straight
+ * from the device read store in one fused launch per depth bucket, without
+ * the per-window recycle; emit_text formats the rows.  Same outputs as the
+ * per-window path (no reference counterpart; it replaces the window loop of
+ * Call_win::new_soap2cns, cpp/call_genotype.cc:474-644). */
+int gsnp_batched_call(gsnp_ctx* ctx, uint32_t start, uint32_t n_sites, int emit_text, uint64_t* nbytes);
+
+/* Whole pipeline on files: FASTA + sorted SOAP alignments (+ known SNPs) to
+ * .cns text or, with compress, the -C column files <out>.1_2 ... <out>.15_17.
+ * matrix_in/matrix_out are the -I/-M calibration files (may be NULL).
+ * batched selects gsnp_batched_call instead of the per-window path.
+ * Replaces main + Call_win::new_soap2cns, cpp/main.cc:91-429. */
+int gsnp_run(const char* fasta, const char* soap, const char* dbsnp, const char* matrix_in, const char* matrix_out,
+             const gsnp_params* p, const char* out, int compress, int batched);
+
+/* -C files back to .cns text.  Replaces decompressAll, cpp/decompression.cpp:130-255. */
+int gsnp_decompress(const char* prefix, const char* out);
+
+/* Counting-trained calibration matrix from a SOAP file, FASTA and optional
+ * known-SNP file (whose sites are excluded from counting).  Replaces
+ * Prob_matrix::matrix_gen, cpp/matrix.cc:42-213. */
+int gsnp_matrix_gen(const char* fasta, const char* soap, const char* dbsnp, const gsnp_params* p, double* p_matrix);
+
+/* Dictionary encoding of a 32-bit integer column on the device: sorted
+ * unique dictionary plus MSB-first bit packing.  Replaces cuda_dictCompression
+ * (int overload), cpp/dict_compression.h:26-31.  words needs n entries, dict n. */
+int gsnp_dict_compress_i32(gsnp_ctx* ctx, const int32_t* in, uint64_t n, uint32_t* words, uint64_t* n_words,
+                           int32_t* dict, uint64_t* n_dict);
+
+/* Segmented sort + likelihood on caller-supplied records: n_sites rows of
+ * `pitch` u32 records, counts[i] valid in row i; writes 10 log-likelihoods
+ * per site.  Exposes the sort and likelihood kernels on their own
+ * (cpp/gsnp.h:387-401). */
+int gsnp_sort_likelihood(gsnp_ctx* ctx, uint32_t* records, uint32_t pitch, const uint32_t* counts,
+                         uint32_t n_sites, double* lk10);
+
+/* %#g text of one double as written in .cns rows (host build of the device
+ * formatter). Returns the length. */
+int gsnp_format_g(double v, char* out);
+
+/* Seeded synthetic inputs: <prefix>.fa, .soap (and .snp for config 2).
+ * Returns the number of alignment lines. */
+long long gsnp_synth_write(int config, unsigned long long seed, double scale, const char* prefix, int* read_len);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

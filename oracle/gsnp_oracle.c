@@ -1061,6 +1061,10 @@ int orc_call_window(const orc_reads* r, const int64_t* idx, int64_t nidx, const 
         for (int coord = 0; coord < r->len[i]; coord++) {
             int64_t site = (int64_t)r->pos[i] + coord - ws;
             if (site < 0 || site >= NS) continue;
+            /* the extra site C of a full window is the first tail site, which at
+             * call time holds only reads that start inside the window
+             * (kernel/update_kernel.cu:269-274 stops at global_win_size) */
+            if (site == C && r->pos[i] >= (int64_t)ws + ORC_WIN) continue;
             orc_site* st = &out[site];
             st->depth += 1;
             st->repeat += r->hit[i];

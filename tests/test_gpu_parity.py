@@ -1,0 +1,185 @@
+"""GPU parity: the HIP path against the CPU oracle, byte for byte.
+
+Every test here runs the product through the C ABI on the device and compares
+with the oracle on the same seeded inputs: .cns text (plain per-window path and
+the batched span path), the -C column files, and per-site unit entry points.
+"""
+from __future__ import annotations
+
+import ctypes
+import filecmp
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import oracle_lib
+
+pytestmark = pytest.mark.gpu
+
+SUFFIXES = [".1_2", ".3_4_6", ".5_7_8_9_14_16", ".10", ".11_12_13", ".15_17"]
+
+
+def _first_diff(a: Path, b: Path) -> str:
+    la = a.read_bytes().split(b"\n")
+    lb = b.read_bytes().split(b"\n")
+    for i, (x, y) in enumerate(zip(la, lb)):
+        if x != y:
+            return f"line {i + 1}:\n gpu    {x!r}\n oracle {y!r}"
+    return f"line count {len(la)} vs {len(lb)}"
+
+
+def _gpu_run(ds, out: Path, **kw):
+    import gsnp_amd
+
+    flags = dict(ds.flags)
+    flags.update(kw.pop("flags", {}))
+    gsnp_amd.run(ds.fasta, ds.soap, out, dbsnp=ds.dbsnp, **kw, **flags)
+
+
+@pytest.mark.parametrize("name", ["c1", "c2w3", "stress", "c3s", "c5s"])
+def test_text_matches_oracle(datasets, tmp_path, name):
+    ds = datasets(name)
+    print(f"[parity] {name}: {ds.n_reads} reads, flags {ds.flags}", flush=True)
+    out = tmp_path / f"{name}.cns"
+    _gpu_run(ds, out)
+    ref = ds.oracle_text()
+    assert out.read_bytes() == ref.read_bytes(), _first_diff(out, ref)
+
+
+@pytest.mark.parametrize("name,per", [("c2w3", 1), ("c2w3", 8), ("stress", 2), ("c3s", 4)])
+def test_batched_matches_oracle(datasets, tmp_path, name, per):
+    ds = datasets(name)
+    print(f"[parity] batched {name} x{per}", flush=True)
+    out = tmp_path / f"{name}.b{per}.cns"
+    _gpu_run(ds, out, batched=per)
+    ref = ds.oracle_text()
+    assert out.read_bytes() == ref.read_bytes(), _first_diff(out, ref)
+
+
+@pytest.mark.parametrize("name,flags", [("c1", {"is_monoploid": 1}), ("stress", {"rank_sum_mode": 0}),
+                                        ("c2w3", {"pcr_dependency": 1.0, "global_dependency": 1.0}),
+                                        ("c2w3", {"pcr_dependency": 0.8, "global_dependency": 0.95,
+                                                  "het_novel_r": 0.01})])
+def test_parameter_variants(datasets, tmp_path, name, flags):
+    ds = datasets(name)
+    print(f"[parity] {name} {flags}", flush=True)
+    out = tmp_path / f"{name}.var.cns"
+    _gpu_run(ds, out, flags=flags)
+    ref = ds.oracle_text(**flags)
+    assert out.read_bytes() == ref.read_bytes(), _first_diff(out, ref)
+
+
+@pytest.mark.parametrize("name", ["stress", "c2w3"])
+def test_compressed_output(datasets, tmp_path, name):
+    import gsnp_amd
+
+    ds = datasets(name)
+    print(f"[parity] -C {name}", flush=True)
+    gpu = tmp_path / f"{name}.gpu"
+    _gpu_run(ds, gpu, compress=True)
+    cpu = tmp_path / f"{name}.cpu"
+    oracle_lib.run(ds.fasta, ds.soap, cpu, dbsnp=ds.dbsnp, compress=True, **ds.flags)
+    for sfx in SUFFIXES:
+        a, b = Path(str(gpu) + sfx), Path(str(cpu) + sfx)
+        assert filecmp.cmp(a, b, shallow=False), f"{sfx} differs"
+    # decompressing the GPU files gives the plain text
+    txt = tmp_path / f"{name}.dec.cns"
+    gsnp_amd.decompress(gpu, txt)
+    ref = ds.oracle_text()
+    assert txt.read_bytes() == ref.read_bytes(), _first_diff(txt, ref)
+
+
+def test_sort_likelihood_unit(built):
+    """Random record sets through the sort + likelihood kernels vs the oracle."""
+    import gsnp_amd
+
+    L = gsnp_amd.lib()
+    o = oracle_lib.orc()
+    rng = np.random.default_rng(5)
+    p = gsnp_amd.params(read_length=100)
+    pm = np.full(256 * 256 * 16, 1.0)
+    # a plausible calibration: match 1-e, mismatch e/3 by quality
+    q = np.arange(256)
+    e = 10.0 ** (-q / 10.0)
+    for t in range(4):
+        for ob in range(4):
+            col = np.where(t == ob, 1 - e, e / 3)
+            idx = (q[:, None] << 12) | (np.arange(256)[None, :] << 4) | (t << 2) | ob
+            pm[idx] = col[:, None]
+    ctx = ctypes.c_void_p()
+    gsnp_amd.check(L.gsnp_device_init(0))
+    gsnp_amd.check(L.gsnp_create(ctypes.byref(p), pm.ctypes.data, ctypes.byref(ctx)))
+    try:
+        n_sites = 3000
+        counts = rng.integers(0, 256, n_sites).astype(np.uint32)
+        counts[:40] = [0, 1, 2, 8, 9, 16, 17, 32, 33, 64, 65, 128, 129, 255, 256, 7, 15, 31, 63, 127] * 2
+        recs = np.zeros((n_sites, 256), dtype=np.uint32)
+        for i in range(n_sites):
+            n = counts[i]
+            base = rng.integers(0, 4, n)
+            qs = rng.integers(1, 41, n)
+            cyc = rng.integers(0, 100, n)
+            st = rng.integers(0, 2, n)
+            if i % 3 == 0:  # force duplicate (strand, cycle) keys
+                cyc = rng.integers(0, 4, n)
+            recs[i, :n] = (base << 23) | ((64 - qs) << 17) | (cyc << 9) | (st << 8) | 1
+        lk = np.zeros((n_sites, 10))
+        work = recs.copy()
+        gsnp_amd.check(L.gsnp_sort_likelihood(ctx, work.ctypes.data, 256, counts.ctypes.data, n_sites, lk.ctypes.data))
+        score = np.zeros(256 * 256 * 4 * 10)
+        o.orc_score_table(pm.ctypes.data, score.ctypes.data)
+        for i in range(n_sites):
+            n = int(counts[i])
+            want = np.sort(recs[i, :n])
+            assert np.array_equal(work[i, :n], want), f"site {i} (n={n}) not sorted"
+            ref = np.zeros(10)
+            rr = np.ascontiguousarray(want)
+            assert o.orc_likelihood_sorted(rr.ctypes.data, n, 100, p.pcr_dependency, p.global_dependency,
+                                           score.ctypes.data, ref.ctypes.data) == 0
+            assert np.array_equal(lk[i].view(np.uint64), ref.view(np.uint64)), f"site {i} likelihood differs"
+    finally:
+        L.gsnp_destroy(ctx)
+
+
+def test_device_dictionary_matches_reference_codec(built):
+    """gsnp_dict_compress_i32 vs the reference's own dictCompression (and the oracle)."""
+    import gsnp_amd
+
+    L = gsnp_amd.lib()
+    ref = oracle_lib.dictref()
+    o = oracle_lib.orc()
+    p = gsnp_amd.params(read_length=100)
+    pm = np.full(256 * 256 * 16, 0.5)
+    ctx = ctypes.c_void_p()
+    gsnp_amd.check(L.gsnp_device_init(0))
+    gsnp_amd.check(L.gsnp_create(ctypes.byref(p), pm.ctypes.data, ctypes.byref(ctx)))
+    rng = np.random.default_rng(9)
+    try:
+        cases = [np.array([7], np.int32), np.array([3, 1, 3, 7, 1], np.int32), np.arange(1024, dtype=np.int32),
+                 rng.integers(0, 2, 5000).astype(np.int32), rng.integers(-50, 50, 10001).astype(np.int32),
+                 rng.integers(0, 65537, 70000).astype(np.int32), rng.integers(-2**31, 2**31 - 1, 3000).astype(np.int32)]
+        for k in [1, 2, 3, 4, 5, 8, 9, 16, 17, 255, 256, 257, 1023, 1024, 1025]:
+            cases.append(rng.permutation(np.repeat(np.arange(k, dtype=np.int32) * 3 - 7, 3)))
+        for v in cases:
+            n = len(v)
+            w = np.zeros(n + 1, np.uint32)
+            d = np.zeros(n + 1, np.int32)
+            nw, nd = ctypes.c_uint64(), ctypes.c_uint64()
+            gsnp_amd.check(L.gsnp_dict_compress_i32(ctx, v.ctypes.data, n, w.ctypes.data, ctypes.byref(nw),
+                                                    d.ctypes.data, ctypes.byref(nd)))
+            w2 = np.zeros(n + 1, np.uint32)
+            d2 = np.zeros(n + 1, np.int32)
+            nd2 = ctypes.c_uint64()
+            nw2 = o.orc_dict_encode_i32(v.ctypes.data, n, w2.ctypes.data, d2.ctypes.data, ctypes.byref(nd2))
+            assert (nw.value, nd.value) == (nw2, nd2.value)
+            assert np.array_equal(w[:nw2], w2[:nw2]) and np.array_equal(d[:nd2.value], d2[:nd2.value])
+            if ref is not None:
+                w3 = np.zeros(n + 1, np.uint32)
+                d3 = np.zeros(n + 1, np.int32)
+                nd3 = ctypes.c_ulong()
+                nw3 = ref.ref_dict_encode_i32(v.ctypes.data, n, w3.ctypes.data, d3.ctypes.data, ctypes.byref(nd3))
+                assert (nw3, nd3.value) == (nw.value, nd.value)
+                assert np.array_equal(w[:nw3], w3[:nw3]) and np.array_equal(d[:nd3.value], d3[:nd3.value])
+    finally:
+        L.gsnp_destroy(ctx)
