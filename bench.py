@@ -1,0 +1,180 @@
+#!/usr/bin/env python3
+"""Called sites per second of the SOAPsnp calling path on one MI355X.
+
+Workload: the 10 Mb plain-GPU configuration (seeded reference, 30x 100 bp
+pairs, 3% two-hit reads, 1% known-SNP prior file, refine mode on), generated
+by gsnp_amd's seeded generator.  A step is one calling pass over the whole
+chromosome: site matrix -> per-site sort -> FP64 likelihood -> posterior ->
+.cns rows formatted in device memory, with the reads already resident in HBM.
+The timed region excludes parsing, calibration and the host copy of the text.
+
+Prints one JSON line (metric, value, roofline of the dominant kernel from HIP
+events, and the CPU oracle timed on a bounded sample of the same workload).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, 8 TB/s
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--config", type=int, default=2)
+    ap.add_argument("--scale", type=float, default=1.0)
+    ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--mode", choices=["batched", "plain"], default="batched")
+    ap.add_argument("--per", type=int, default=0, help="windows per batched span (0: tuned default)")
+    ap.add_argument("--compare", action="store_true", help="also time the other mode")
+    ap.add_argument("--cpu-sites", type=int, default=2 * 262144, help="sites in the CPU oracle sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--workdir", default=None)
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (multi-GPU sharding is design-only)"}))
+        return 2
+
+    import gsnp_amd
+
+    if not gsnp_amd.LIB_PATH.exists():
+        import __graft_entry__
+
+        __graft_entry__.build()
+
+    work = Path(args.workdir) if args.workdir else Path(tempfile.mkdtemp(prefix="gsnp_bench_"))
+    work.mkdir(parents=True, exist_ok=True)
+    pre = work / f"c{args.config}"
+    t0 = time.time()
+    n_reads, rl = gsnp_amd.synth(args.config, args.seed, args.scale, pre)
+    fa, so = f"{pre}.fa", f"{pre}.soap"
+    snp = f"{pre}.snp" if Path(f"{pre}.snp").exists() else None
+    flags = dict(read_length=rl, refine_mode=1 if snp else 0)
+    print(f"[bench] generated config {args.config}: {n_reads} reads in {time.time() - t0:.1f}s", file=sys.stderr,
+          flush=True)
+    t0 = time.time()
+    job = gsnp_amd.Job(fa, so, dbsnp=snp, **flags)
+    name, chr_len, chr_reads = job.info(0)
+    job.select(0)
+    print(f"[bench] parsed + calibrated + uploaded {name} ({chr_len} bp, {chr_reads} reads) in "
+          f"{time.time() - t0:.1f}s", file=sys.stderr, flush=True)
+
+    per = args.per or int(os.environ.get("GSNP_BATCH_WINDOWS", "0")) or (chr_len // gsnp_amd.WIN + 1)
+
+    def timed(mode: str):
+        batched = per if mode == "batched" else 0
+        for _ in range(args.warmup):
+            job.call(batched=batched)
+        job.records(reset=True)
+        job.profile(enable=True)
+        t_start = time.perf_counter()
+        nbytes = 0
+        for _ in range(args.steps):
+            nbytes = job.call(batched=batched)
+        t_end = time.perf_counter()
+        prof = job.profile(enable=False)
+        recs = job.records(reset=True) // max(1, args.steps)
+        secs = (t_end - t_start) / args.steps
+        return secs, prof, recs, nbytes
+
+    secs, prof, recs, text_bytes = timed(args.mode)
+    sites_per_s = chr_len / secs
+    other = None
+    if args.compare:
+        o_mode = "plain" if args.mode == "batched" else "batched"
+        o_secs, _, _, _ = timed(o_mode)
+        other = {"mode": o_mode, "value": chr_len / o_secs, "ms_per_step": o_secs * 1e3}
+
+    # dominant stage and its algorithmic bytes per launch
+    steps = args.steps
+    stage_ms = {k: v[0] / steps for k, v in prof.items() if v[1]}
+    launches = {k: v[1] / steps for k, v in prof.items() if v[1]}
+    dom = max(stage_ms, key=stage_ms.get)
+    sites = chr_len
+    packed = chr_reads * rl  # one packed byte per read base
+    algo = {
+        # reads (packed bases + 8 B position/metadata) in, records + 60 B of counters per site out
+        "site_build": packed + 8 * chr_reads + 4 * recs + 60 * sites,
+        # records read and written back, record counts read
+        "sort": 8 * recs + 4 * sites,
+        # records and count in, 10 FP64 log-likelihoods out (4n + 84 per site)
+        "likelihood": 4 * recs + 84 * sites,
+        # counters, reference code and likelihoods in; calls out
+        "posterior": 121 * sites + 15 * sites,
+        "rank": 0,
+        # call columns and counters in, .cns text out
+        "format": 72 * sites + text_bytes,
+        "recycle": 0,
+        "fused": 4 * recs + 80 * sites,
+    }
+    a_bytes = algo.get(dom, 0) / max(1e-9, launches[dom])
+    t_launch = stage_ms[dom] / 1e3 / max(1e-9, launches[dom])
+    achieved = a_bytes / t_launch / 1e9
+    roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "bytes_per_launch": int(a_bytes), "ms_per_launch": round(t_launch * 1e3, 4)}
+    # the calling kernels together at B_site = 4n + 80 per called site
+    call_ms = sum(stage_ms.get(k, 0) for k in ("sort", "likelihood", "posterior", "rank"))
+    call_gbs = (4 * recs + 80 * sites) / (call_ms / 1e3) / 1e9 if call_ms else None
+
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, str(ROOT / "tests"))
+        import oracle_lib
+
+        oracle_lib.ensure_built()
+        out = work / "cpu.cns"
+        cmd = [str(oracle_lib.ORACLE_CLI), "-i", so, "-d", fa, "-o", str(out), "-L", str(rl), "-X", str(args.cpu_sites),
+               "-R"]
+        if snp:
+            cmd += ["-s", snp, "-2"]
+        res = subprocess.run(cmd, capture_output=True, text=True, check=True)
+        st = json.loads(res.stdout.strip().splitlines()[-1])
+        cpu = {"value": round(st["sites"] / st["seconds_call"], 1), "unit": "sites/s", "cores": 1, "kind": "port",
+               "sample": f"first {st['sites']} sites ({st['windows']} windows) of the same workload, "
+                         f"single-threaded CPU oracle, call stage only"}
+
+    line = {
+        "metric": "called sites/sec at bit-exact .cns",
+        "value": round(sites_per_s, 1),
+        "unit": "sites/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(secs * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded generator, config %d)" % args.config,
+        "config": {"workload": "10 Mb reference, 30x 100 bp pairs, 3% 2-hit, 1% known SNPs, refine on"
+                   if args.config == 2 else f"config {args.config} x{args.scale}",
+                   "sites": chr_len, "reads": chr_reads, "records": recs, "mode": args.mode,
+                   "windows_per_span": per if args.mode == "batched" else 1, "text_bytes": text_bytes},
+        "roofline": roofline,
+        "call_kernels_gbs": round(call_gbs, 2) if call_gbs else None,
+        "stage_ms_per_step": {k: round(v, 4) for k, v in stage_ms.items()},
+        "cpu_baseline": cpu,
+    }
+    if other:
+        line["other_mode"] = other
+    print(json.dumps(line), flush=True)
+    job.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

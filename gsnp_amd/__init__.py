@@ -117,6 +117,17 @@ SIGNATURES = {
                                               ctypes.POINTER(ctypes.c_uint64)]),
     "gsnp_sort_likelihood": (ctypes.c_int, [_P, _P, ctypes.c_uint32, _P, ctypes.c_uint32, _P]),
     "gsnp_format_g": (ctypes.c_int, [ctypes.c_double, ctypes.c_char_p]),
+    "gsnp_profile": (ctypes.c_int, [_P, ctypes.c_int, _P, _P, ctypes.c_int]),
+    "gsnp_stats": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_uint64), ctypes.c_int]),
+    "gsnp_job_open": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p,
+                                     ctypes.POINTER(Params), ctypes.POINTER(_P)]),
+    "gsnp_job_count": (ctypes.c_int, [_P]),
+    "gsnp_job_info": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_char_p, ctypes.c_uint64,
+                                     ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint64)]),
+    "gsnp_job_select": (ctypes.c_int, [_P, ctypes.c_int]),
+    "gsnp_job_call": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_uint64)]),
+    "gsnp_job_context": (_P, [_P]),
+    "gsnp_job_close": (None, [_P]),
     "gsnp_synth_write": (ctypes.c_longlong, [ctypes.c_int, ctypes.c_ulonglong, ctypes.c_double, ctypes.c_char_p,
                                              ctypes.POINTER(ctypes.c_int)]),
 }
@@ -185,3 +196,64 @@ def format_g(v: float) -> str:
     buf = ctypes.create_string_buffer(64)
     n = lib().gsnp_format_g(v, buf)
     return buf.raw[:n].decode()
+
+
+STAGES = ["site_build", "sort", "likelihood", "posterior", "rank", "format", "recycle", "fused"]
+
+
+class Job:
+    """A calling session over input files (gsnp_job_*): parse and calibrate
+    once, then repeat calling passes on device-resident inputs."""
+
+    def __init__(self, fasta, soap, dbsnp=None, matrix_in=None, **kw):
+        L = lib()
+        check(L.gsnp_device_init(0))
+        self._p = params(**kw)
+        h = ctypes.c_void_p()
+        check(L.gsnp_job_open(_b(fasta), _b(soap), _b(dbsnp), _b(matrix_in), ctypes.byref(self._p), ctypes.byref(h)))
+        self.h = h
+
+    def __len__(self) -> int:
+        return lib().gsnp_job_count(self.h)
+
+    def info(self, i: int) -> tuple[str, int, int]:
+        name = ctypes.create_string_buffer(1024)
+        n = ctypes.c_uint32()
+        r = ctypes.c_uint64()
+        check(lib().gsnp_job_info(self.h, i, name, 1024, ctypes.byref(n), ctypes.byref(r)))
+        return name.value.decode(), n.value, r.value
+
+    def select(self, i: int) -> None:
+        check(lib().gsnp_job_select(self.h, i))
+
+    def call(self, batched: int = 0, emit_text: bool = True) -> int:
+        nb = ctypes.c_uint64()
+        check(lib().gsnp_job_call(self.h, int(batched), int(bool(emit_text)), ctypes.byref(nb)))
+        return nb.value
+
+    @property
+    def ctx(self):
+        return lib().gsnp_job_context(self.h)
+
+    def profile(self, enable: bool) -> dict:
+        ms = (ctypes.c_double * len(STAGES))()
+        cnt = (ctypes.c_uint64 * len(STAGES))()
+        check(lib().gsnp_profile(self.ctx, int(enable), ms, cnt, len(STAGES)))
+        return {s: (ms[i], cnt[i]) for i, s in enumerate(STAGES)}
+
+    def records(self, reset: bool = True) -> int:
+        v = ctypes.c_uint64()
+        check(lib().gsnp_stats(self.ctx, ctypes.byref(v), int(reset)))
+        return v.value
+
+    def close(self) -> None:
+        if self.h:
+            lib().gsnp_job_close(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+

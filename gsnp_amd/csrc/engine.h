@@ -54,9 +54,37 @@ struct SiteBufs {
     }
 };
 
+// Stages timed with HIP events on the context stream when profiling is on.
+enum Stage : int {
+    kStSiteBuild = 0,
+    kStSort,
+    kStLikelihood,
+    kStPosterior,
+    kStRank,
+    kStFormat,
+    kStRecycle,
+    kStFused,
+    kNumStages
+};
+
+struct StageTimer {
+    bool on = false;
+    std::vector<hipEvent_t> pool;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> pending[kNumStages];
+    hipEvent_t open[kNumStages] = {};
+    double ms[kNumStages] = {};
+    uint64_t count[kNumStages] = {};
+    hipEvent_t take();
+    void begin(int st, hipStream_t s);
+    void end(int st, hipStream_t s);
+    int collect();
+    ~StageTimer();
+};
+
 struct Ctx {
     gsnp_params p{};
     hipStream_t stream = nullptr;
+    StageTimer timer;
     // tables
     DevBuf<double> score, lprior, prank;
     DevBuf<uint8_t> qadj;
@@ -99,6 +127,7 @@ struct Ctx {
     DevBuf<char> text;
     uint64_t text_cap = 0, text_len = 0;
     DevBuf<uint32_t> err;
+    DevBuf<unsigned long long> stat;  // [0] records called
     DevBuf<int4> head_qs;  // q_sum of window-boundary sites before their own reads (batched span)
     // device dictionary encoder scratch
     DevBuf<int32_t> dict_mm, dict_vals;

@@ -33,6 +33,59 @@ int fail(int code, const std::string& msg)
         if (e_ != hipSuccess) return fail(GSNP_E_DEVICE, std::string(#x) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
+hipEvent_t StageTimer::take()
+{
+    if (!pool.empty()) {
+        hipEvent_t e = pool.back();
+        pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+void StageTimer::begin(int st, hipStream_t s)
+{
+    if (!on) return;
+    open[st] = take();
+    if (open[st]) (void)hipEventRecord(open[st], s);
+}
+
+void StageTimer::end(int st, hipStream_t s)
+{
+    if (!on || !open[st]) return;
+    hipEvent_t e = take();
+    if (!e) return;
+    (void)hipEventRecord(e, s);
+    pending[st].push_back({open[st], e});
+    open[st] = nullptr;
+}
+
+int StageTimer::collect()
+{
+    for (int st = 0; st < kNumStages; st++) {
+        for (auto& pr : pending[st]) {
+            float t = 0.f;
+            HIPCHK(hipEventSynchronize(pr.second));
+            HIPCHK(hipEventElapsedTime(&t, pr.first, pr.second));
+            ms[st] += t;
+            count[st] += 1;
+            pool.push_back(pr.first);
+            pool.push_back(pr.second);
+        }
+        pending[st].clear();
+    }
+    return GSNP_OK;
+}
+
+StageTimer::~StageTimer()
+{
+    for (int st = 0; st < kNumStages; st++)
+        for (auto& pr : pending[st]) pool.push_back(pr.first), pool.push_back(pr.second);
+    for (hipEvent_t e : pool) (void)hipEventDestroy(e);
+}
+
 int Ctx::alloc_window(uint32_t n_sites)
 {
     if (n_sites <= win_cap) return GSNP_OK;
@@ -277,6 +330,8 @@ int gsnp_create(const gsnp_params* p, const double* p_matrix, gsnp_ctx** out)
     HIPCHK(c->job_count.alloc(1));
     HIPCHK(c->fix_count.alloc(1));
     HIPCHK(c->total.alloc(1));
+    HIPCHK(c->stat.alloc(4));
+    HIPCHK(hipMemsetAsync(c->stat.p, 0, 32, c->stream));
     HIPCHK(hipMemcpyAsync(c->score.p, sc.data(), sc.size() * 8, hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipMemcpyAsync(c->qadj.p, qt.v.data(), qt.v.size(), hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipMemcpyAsync(c->lprior.p, lp.data(), lp.size() * 8, hipMemcpyHostToDevice, c->stream));
@@ -411,8 +466,10 @@ int gsnp_window_update(gsnp_ctx* c, uint64_t r0, uint64_t r1, uint32_t* max_reco
     if (!c || !c->win_cap) return fail(GSNP_E_ARG, "window not initialised");
     if (r0 > r1 || r1 > c->n_reads) return fail(GSNP_E_ARG, "read range outside the loaded batch");
     const uint32_t L = (uint32_t)c->p.read_length;
+    c->timer.begin(kStSiteBuild, c->stream);
     launch_site_build(c->stream, c->read_view(), r0, r1, c->ws, GSNP_WIN + L, (int64_t)c->ws,
                       (int64_t)c->ws + GSNP_WIN - 1, L, c->site_view(0), nullptr, c->err.p);
+    c->timer.end(kStSiteBuild, c->stream);
     HIPCHK(hipGetLastError());
     c->built = true;
     if (max_records) {
@@ -433,9 +490,13 @@ int gsnp_window_likelihood(gsnp_ctx* c, uint32_t C)
         if (rc) return rc;
     }
     SiteArrays a = c->site_view(0);
+    c->timer.begin(kStSort, c->stream);
     launch_bucket(c->stream, a.nrec, C, c->lists.p, c->win_cap, c->counts.p);
     launch_sort(c->stream, a.rec, a.nrec, c->lists.p, c->win_cap, c->counts.p, C);
-    launch_likelihood(c->stream, c->tables(), a, C, c->lk.p, c->err.p);
+    c->timer.end(kStSort, c->stream);
+    c->timer.begin(kStLikelihood, c->stream);
+    launch_likelihood(c->stream, c->tables(), a, C, c->lk.p, c->err.p, c->stat.p);
+    c->timer.end(kStLikelihood, c->stream);
     HIPCHK(hipGetLastError());
     return GSNP_OK;
 }
@@ -448,9 +509,13 @@ int gsnp_window_posterior(gsnp_ctx* c, uint32_t C)
     Results res = c->result_view(0);
     HIPCHK(hipMemsetAsync(c->job_count.p, 0, 4, c->stream));
     HIPCHK(hipMemsetAsync(c->fix_count.p, 0, 4, c->stream));
+    c->timer.begin(kStPosterior, c->stream);
     launch_posterior(c->stream, c->tables(), c->ref_view(), c->ws, a, C, nullptr, res, c->jobs.p, c->job_count.p);
+    c->timer.end(kStPosterior, c->stream);
     if (c->p.rank_sum_mode) {
+        c->timer.begin(kStRank, c->stream);
         launch_rank(c->stream, c->tables(), a, c->jobs.p, c->job_count.p, C, res, c->fixes.p, c->fix_count.p);
+        c->timer.end(kStRank, c->stream);
         HIPCHK(hipGetLastError());
         c->fix_offset = 0;
         int rc = c->host_fixups();
@@ -503,7 +568,9 @@ int gsnp_window_recycle(gsnp_ctx* c)
         int rc = gsnp_window_update(c, 0, 0, nullptr);
         if (rc) return rc;
     }
+    c->timer.begin(kStRecycle, c->stream);
     launch_recycle(c->stream, c->site_view(0), GSNP_WIN, (uint32_t)c->p.read_length);
+    c->timer.end(kStRecycle, c->stream);
     HIPCHK(hipGetLastError());
     c->ws += GSNP_WIN;
     c->built = false;
@@ -523,16 +590,26 @@ int gsnp_batched_call(gsnp_ctx* c, uint32_t start, uint32_t n, int emit_text, ui
     SiteArrays a = c->site_view(0);
     Results res = c->result_view(0);
     // every read that covers [start, start+n], no window batches and no carry
+    c->timer.begin(kStSiteBuild, c->stream);
     launch_site_build(c->stream, c->read_view(), 0, c->n_reads, start, n + 1, INT64_MIN, (int64_t)start + n, 0, a,
                       c->head_qs.p, c->err.p);
+    c->timer.end(kStSiteBuild, c->stream);
+    c->timer.begin(kStSort, c->stream);
     launch_bucket(c->stream, a.nrec, n, c->lists.p, c->win_cap, c->counts.p);
     launch_sort(c->stream, a.rec, a.nrec, c->lists.p, c->win_cap, c->counts.p, n);
-    launch_likelihood(c->stream, c->tables(), a, n, c->lk.p, c->err.p);
+    c->timer.end(kStSort, c->stream);
+    c->timer.begin(kStLikelihood, c->stream);
+    launch_likelihood(c->stream, c->tables(), a, n, c->lk.p, c->err.p, c->stat.p);
+    c->timer.end(kStLikelihood, c->stream);
     HIPCHK(hipMemsetAsync(c->job_count.p, 0, 4, c->stream));
     HIPCHK(hipMemsetAsync(c->fix_count.p, 0, 4, c->stream));
+    c->timer.begin(kStPosterior, c->stream);
     launch_posterior(c->stream, c->tables(), c->ref_view(), start, a, n, c->head_qs.p, res, c->jobs.p, c->job_count.p);
+    c->timer.end(kStPosterior, c->stream);
     if (c->p.rank_sum_mode) {
+        c->timer.begin(kStRank, c->stream);
         launch_rank(c->stream, c->tables(), a, c->jobs.p, c->job_count.p, n, res, c->fixes.p, c->fix_count.p);
+        c->timer.end(kStRank, c->stream);
         HIPCHK(hipGetLastError());
         c->fix_offset = 0;
         rc = c->host_fixups();
@@ -567,7 +644,7 @@ int gsnp_sort_likelihood(gsnp_ctx* c, uint32_t* records, uint32_t pitch, const u
     SiteArrays a = c->site_view(0);
     launch_bucket(c->stream, a.nrec, n_sites, c->lists.p, c->win_cap, c->counts.p);
     launch_sort(c->stream, a.rec, a.nrec, c->lists.p, c->win_cap, c->counts.p, n_sites);
-    launch_likelihood(c->stream, c->tables(), a, n_sites, c->lk.p, c->err.p);
+    launch_likelihood(c->stream, c->tables(), a, n_sites, c->lk.p, c->err.p, nullptr);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(rows.data(), c->sites.rec.p, rows.size() * 4, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipMemcpyAsync(lk10, c->lk.p, 80ull * n_sites, hipMemcpyDeviceToHost, c->stream));
@@ -576,6 +653,31 @@ int gsnp_sort_likelihood(gsnp_ctx* c, uint32_t* records, uint32_t pitch, const u
         std::memcpy(records + (size_t)i * pitch, &rows[(size_t)i * kSlots], 4ull * counts[i]);
     c->built = false;
     return c->check_err();
+}
+
+int gsnp_profile(gsnp_ctx* c, int enable, double* stage_ms, uint64_t* stage_count, int n_stages)
+{
+    if (!c) return fail(GSNP_E_ARG, "null context");
+    int rc = c->timer.collect();
+    if (rc) return rc;
+    for (int i = 0; i < n_stages && i < kNumStages; i++) {
+        if (stage_ms) stage_ms[i] = c->timer.ms[i];
+        if (stage_count) stage_count[i] = c->timer.count[i];
+    }
+    for (int i = 0; i < kNumStages; i++) c->timer.ms[i] = 0, c->timer.count[i] = 0;
+    c->timer.on = enable != 0;
+    return GSNP_OK;
+}
+
+int gsnp_stats(gsnp_ctx* c, uint64_t* n_records, int reset)
+{
+    if (!c) return fail(GSNP_E_ARG, "null context");
+    unsigned long long v = 0;
+    HIPCHK(hipMemcpyAsync(&v, c->stat.p, 8, hipMemcpyDeviceToHost, c->stream));
+    if (reset) HIPCHK(hipMemsetAsync(c->stat.p, 0, 8, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (n_records) *n_records = v;
+    return GSNP_OK;
 }
 
 int gsnp_format_g(double v, char* out)
@@ -604,9 +706,11 @@ int Ctx::format_text(uint32_t offset, uint32_t ws_abs, uint32_t n_sites, uint32_
     SiteArrays a = site_view(offset);
     Results res = result_view(offset);
     const uint32_t nb = (n_sites + kRowBlock - 1) / kRowBlock;
+    timer.begin(kStFormat, stream);
     launch_rowlen(stream, ref_view(), ws_abs, a, res, n_sites, rc, rowlen.p, blocksum.p);
     launch_scan_blocks(stream, blocksum.p, nb, total.p);
     launch_format(stream, ref_view(), ws_abs, a, res, n_sites, rc, rowlen.p, blocksum.p, text.p);
+    timer.end(kStFormat, stream);
     HIPCHK(hipGetLastError());
     uint64_t t = 0;
     HIPCHK(hipMemcpyAsync(&t, total.p, 8, hipMemcpyDeviceToHost, stream));

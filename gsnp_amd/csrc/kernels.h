@@ -89,7 +89,8 @@ void launch_bucket(hipStream_t s, const int32_t* nrec, uint32_t n_sites, uint32_
                    uint32_t* counts);
 void launch_sort(hipStream_t s, uint32_t* rec, const int32_t* nrec, const uint32_t* lists, uint32_t list_pitch,
                  const uint32_t* counts, uint32_t max_sites);
-void launch_likelihood(hipStream_t s, const Tables& t, const SiteArrays& a, uint32_t n_sites, double* lk, uint32_t* err);
+void launch_likelihood(hipStream_t s, const Tables& t, const SiteArrays& a, uint32_t n_sites, double* lk, uint32_t* err,
+                       unsigned long long* n_records);
 void launch_posterior(hipStream_t s, const Tables& t, const RefView& ref, uint32_t ws, const SiteArrays& a,
                       uint32_t n_sites, const int4* head_qs, Results& out, RankJob* jobs, uint32_t* job_count);
 void launch_rank(hipStream_t s, const Tables& t, const SiteArrays& a, const RankJob* jobs, const uint32_t* job_count,

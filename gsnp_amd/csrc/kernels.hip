@@ -344,13 +344,18 @@ void launch_sort(hipStream_t st, uint32_t* rec, const int32_t* nrec, const uint3
 // set marks the (strand, cycle) keys seen in the segment; only a repeated key
 // rescans the segment to count its earlier occurrences.
 __global__ __launch_bounds__(256) void k_likelihood(Tables t, SiteArrays a, uint32_t n_sites, double* lk,
-                                                    uint32_t* err)
+                                                    uint32_t* err, unsigned long long* n_records)
 {
     __shared__ uint32_t seen_all[256 * 17];
     uint32_t* seen = seen_all + threadIdx.x * 17;
     const uint32_t s = blockIdx.x * 256u + threadIdx.x;
+    const uint32_t n = s < n_sites ? (uint32_t)a.nrec[s] : 0u;
+    if (n_records) {  // records called, for throughput accounting
+        unsigned long long w = n;
+        for (int off = 32; off > 0; off >>= 1) w += __shfl_xor(w, off);
+        if (lane_id() == 0 && w) atomicAdd(n_records, w);
+    }
     if (s >= n_sites) return;
-    const uint32_t n = (uint32_t)a.nrec[s];
     const uint32_t* row = a.rec + (size_t)s * kSlots;
     double acc[10];
 #pragma unroll
@@ -406,10 +411,11 @@ __global__ __launch_bounds__(256) void k_likelihood(Tables t, SiteArrays a, uint
     if (bad) atomicOr(err, bad);
 }
 
-void launch_likelihood(hipStream_t st, const Tables& t, const SiteArrays& a, uint32_t n_sites, double* lk, uint32_t* err)
+void launch_likelihood(hipStream_t st, const Tables& t, const SiteArrays& a, uint32_t n_sites, double* lk, uint32_t* err,
+                       unsigned long long* n_records)
 {
     const uint32_t blocks = (n_sites + 255) / 256;
-    if (blocks) hipLaunchKernelGGL(k_likelihood, dim3(blocks), dim3(256), 0, st, t, a, n_sites, lk, err);
+    if (blocks) hipLaunchKernelGGL(k_likelihood, dim3(blocks), dim3(256), 0, st, t, a, n_sites, lk, err, n_records);
 }
 
 // ---------------------------------------------------------------------------

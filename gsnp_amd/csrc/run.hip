@@ -62,110 +62,175 @@ struct CtxDeleter {
 
 }  // namespace
 
+// One calling pass over a chromosome already loaded into the context.
+// batched = 0 runs the per-window path (init, update, likelihood, posterior,
+// recycle per window, cpp/call_genotype.cc:474-644); batched = k calls spans
+// of k windows straight from the device read store.  After each window
+// (per-window path) or span (batched path) the results are handed to
+// on_results(offset, span_start, span_sites) and, with emit_text, the span's
+// rows are formatted in device memory and their size given to on_text.
+template <class OnResults, class OnText>
+int call_chromosome(gsnp_ctx* ctx, const std::vector<int32_t>& pos, uint32_t N, int batched, bool emit_text,
+                    OnResults on_results, OnText on_text)
+{
+    int rc;
+    const uint32_t nwin = N / GSNP_WIN + 1;
+    if (batched) {
+        const uint32_t per = (uint32_t)batched;
+        for (uint32_t w0 = 0; w0 < nwin; w0 += per) {
+            const uint32_t ws = w0 * GSNP_WIN;
+            const uint32_t we = (uint32_t)std::min<uint64_t>((uint64_t)(w0 + per) * GSNP_WIN, N);
+            const uint32_t n = we > ws ? we - ws : 0;
+            uint64_t nb = 0;
+            rc = gsnp_batched_call(ctx, ws, n, emit_text, &nb);
+            if (rc) return rc;
+            rc = on_results(0u, ws, n);
+            if (rc) return rc;
+            // a chromosome whose length is a multiple of the window ends with an
+            // empty window (cpp/call_genotype.cc:636-644); report it on its own
+            const uint32_t last = N / GSNP_WIN;
+            if (N % GSNP_WIN == 0 && n > 0 && last >= w0 && last < w0 + per) {
+                rc = on_results(0u, N, 0u);
+                if (rc) return rc;
+            }
+            if (emit_text) {
+                rc = on_text(nb);
+                if (rc) return rc;
+            }
+        }
+        return GSNP_OK;
+    }
+    rc = gsnp_window_init(ctx, 0);
+    if (rc) return rc;
+    uint64_t r = 0;
+    for (uint32_t w = 0; w < nwin; w++) {
+        const uint32_t ws = w * GSNP_WIN;
+        const uint32_t C = (w < N / GSNP_WIN) ? GSNP_WIN : N % GSNP_WIN;
+        const uint64_t r1 = (uint64_t)(std::lower_bound(pos.begin(), pos.end(), (int64_t)ws + GSNP_WIN) - pos.begin());
+        rc = gsnp_window_update(ctx, r, r1, nullptr);
+        if (rc) return rc;
+        r = r1;
+        rc = gsnp_window_likelihood(ctx, C);
+        if (!rc) rc = gsnp_window_posterior(ctx, C);
+        if (rc) return rc;
+        rc = on_results(0u, ws, C);
+        if (rc) return rc;
+        if (emit_text) {
+            uint64_t nb = 0;
+            rc = gsnp_window_text(ctx, C, &nb);
+            if (!rc) rc = on_text(nb);
+            if (rc) return rc;
+        }
+        rc = gsnp_window_recycle(ctx);
+        if (rc) return rc;
+    }
+    return GSNP_OK;
+}
+
+}  // namespace gsnp
+
+struct gsnp_job {
+    gsnp_params p{};
+    gsnp::Genome g;
+    std::vector<gsnp::ReadBlock> blocks;
+    std::vector<double> pm;
+    gsnp_ctx* ctx = nullptr;
+    int cur = -1;
+    ~gsnp_job()
+    {
+        if (ctx) gsnp_destroy(ctx);
+    }
+};
+
+namespace gsnp {
+
+int job_open(const char* fasta, const char* soap, const char* dbsnp, const char* matrix_in, const char* matrix_out,
+             const gsnp_params& p, std::unique_ptr<gsnp_job>& job)
+{
+    std::string err;
+    job.reset(new gsnp_job());
+    job->p = p;
+    int rc = load_genome(fasta, dbsnp, job->g, err);
+    if (rc) return fail(rc, err);
+    rc = load_alignments(soap, p, job->g, job->blocks, err);
+    if (rc) return fail(rc, err);
+    job->pm.assign((size_t)256 * 256 * 16, 1.0);
+    if (matrix_in && matrix_in[0]) {
+        rc = matrix_read(matrix_in, job->pm.data(), err);
+        if (rc) return fail(rc, err);
+    } else {
+        matrix_gen(job->blocks, job->g, p, job->pm.data());
+    }
+    if (matrix_out && matrix_out[0]) {
+        rc = matrix_write(matrix_out, job->pm.data(), p, err);
+        if (rc) return fail(rc, err);
+    }
+    return gsnp_create(&p, job->pm.data(), &job->ctx);
+}
+
+int job_select(gsnp_job* job, int index)
+{
+    if (index < 0 || index >= (int)job->blocks.size()) return fail(GSNP_E_ARG, "chromosome index out of range");
+    const ReadBlock& b = job->blocks[index];
+    const Chrom& chr = job->g.chr[b.chr];
+    int rc = gsnp_set_chromosome(job->ctx, chr.name.c_str(), chr.code.data(), (uint32_t)chr.code.size(),
+                                 chr.snp_pos.data(), chr.snp.data(), (uint32_t)chr.snp.size());
+    if (rc) return rc;
+    gsnp_read_batch rb;
+    rb.n = b.size();
+    rb.pitch = b.pitch;
+    rb.pos = b.pos.data();
+    rb.len_hit = b.len_hit.data();
+    rb.strand = b.strand.data();
+    rb.seq = b.seq.data();
+    rb.qual = b.qual.data();
+    rc = gsnp_load_reads(job->ctx, &rb);
+    if (rc) return rc;
+    job->cur = index;
+    return GSNP_OK;
+}
+
 int run_pipeline(const char* fasta, const char* soap, const char* dbsnp, const char* matrix_in, const char* matrix_out,
                  const gsnp_params& p, const char* out, int compress, int batched)
 {
-    std::string err;
-    Genome g;
-    int rc = load_genome(fasta, dbsnp, g, err);
-    if (rc) return fail(rc, err);
-    std::vector<ReadBlock> blocks;
-    rc = load_alignments(soap, p, g, blocks, err);
-    if (rc) return fail(rc, err);
-    std::vector<double> pm((size_t)256 * 256 * 16);
-    if (matrix_in && matrix_in[0]) {
-        rc = matrix_read(matrix_in, pm.data(), err);
-        if (rc) return fail(rc, err);
-    } else {
-        matrix_gen(blocks, g, p, pm.data());
-    }
-    if (matrix_out && matrix_out[0]) {
-        rc = matrix_write(matrix_out, pm.data(), p, err);
-        if (rc) return fail(rc, err);
-    }
-    gsnp_ctx* raw = nullptr;
-    rc = gsnp_create(&p, pm.data(), &raw);
+    std::unique_ptr<gsnp_job> job;
+    int rc = job_open(fasta, soap, dbsnp, matrix_in, matrix_out, p, job);
     if (rc) return rc;
-    std::unique_ptr<gsnp_ctx, CtxDeleter> ctx(raw);
+    std::string err;
     std::unique_ptr<FILE, FileCloser> fo(std::fopen(out, "w"));
     if (!fo) return fail(GSNP_E_IO, std::string("cannot create ") + out);
     CWriter cw;
     if (compress && cw.open(out, err)) return fail(GSNP_E_IO, err);
     std::vector<char> host_text;
     WindowCols cols;
-    for (const ReadBlock& b : blocks) {
-        const Chrom& chr = g.chr[b.chr];
+    gsnp_ctx* ctx = job->ctx;
+    for (int bi = 0; bi < (int)job->blocks.size(); bi++) {
+        rc = job_select(job.get(), bi);
+        if (rc) return rc;
+        const ReadBlock& b = job->blocks[bi];
+        const Chrom& chr = job->g.chr[b.chr];
         const uint32_t N = (uint32_t)chr.code.size();
-        rc = gsnp_set_chromosome(ctx.get(), chr.name.c_str(), chr.code.data(), N, chr.snp_pos.data(), chr.snp.data(),
-                                 (uint32_t)chr.snp.size());
-        if (rc) return rc;
-        gsnp_read_batch rb;
-        rb.n = b.size();
-        rb.pitch = b.pitch;
-        rb.pos = b.pos.data();
-        rb.len_hit = b.len_hit.data();
-        rb.strand = b.strand.data();
-        rb.seq = b.seq.data();
-        rb.qual = b.qual.data();
-        rc = gsnp_load_reads(ctx.get(), &rb);
-        if (rc) return rc;
-        const uint32_t nwin = N / GSNP_WIN + 1;
-        if (batched) {
-            // whole chromosome through the fused path, chunk by chunk of windows
-            const uint32_t per = batched > 1 ? (uint32_t)batched : 8u;
-            for (uint32_t w0 = 0; w0 < nwin; w0 += per) {
-                const uint32_t ws = w0 * GSNP_WIN;
-                const uint32_t we = std::min<uint64_t>((uint64_t)(w0 + per) * GSNP_WIN, N);
-                const uint32_t n = we > ws ? we - ws : 0;
-                uint64_t nb = 0;
-                rc = gsnp_batched_call(ctx.get(), ws, n, !compress, &nb);
-                if (rc) return rc;
-                if (compress) {
-                    for (uint32_t w = w0; w < std::min(nwin, w0 + per); w++) {
-                        const uint32_t s = w * GSNP_WIN;
-                        const uint32_t C = (w < N / GSNP_WIN) ? GSNP_WIN : N % GSNP_WIN;
-                        rc = fetch_cols(*ctx, s - ws, s, C, chr.code.data(), cols);
-                        if (rc) return rc;
-                        cw.write_window(chr.name, s, cols);
-                    }
-                } else {
-                    host_text.resize(nb);
-                    rc = gsnp_fetch_text(ctx.get(), host_text.data(), nb, &nb);
-                    if (rc) return rc;
-                    std::fwrite(host_text.data(), 1, nb, fo.get());
-                }
+        auto on_results = [&](uint32_t offset, uint32_t ws, uint32_t n) -> int {
+            if (!compress) return GSNP_OK;
+            // -C chunks are per reference window of the span
+            for (uint32_t s = ws; s < ws + n || (n == 0 && s == ws); s += GSNP_WIN) {
+                const uint32_t C = std::min<uint32_t>(GSNP_WIN, ws + n - s);
+                int r2 = fetch_cols(*ctx, offset + (s - ws), s, C, chr.code.data(), cols);
+                if (r2) return r2;
+                cw.write_window(chr.name, s, cols);
+                if (n == 0) break;
             }
-            continue;
-        }
-        rc = gsnp_window_init(ctx.get(), 0);
+            return GSNP_OK;
+        };
+        auto on_text = [&](uint64_t nb) -> int {
+            host_text.resize(nb);
+            int r2 = gsnp_fetch_text(ctx, host_text.data(), nb, &nb);
+            if (r2) return r2;
+            std::fwrite(host_text.data(), 1, nb, fo.get());
+            return GSNP_OK;
+        };
+        rc = call_chromosome(ctx, b.pos, N, batched, !compress, on_results, on_text);
         if (rc) return rc;
-        uint64_t r = 0;
-        for (uint32_t w = 0; w < nwin; w++) {
-            const uint32_t ws = w * GSNP_WIN;
-            const uint32_t C = (w < N / GSNP_WIN) ? GSNP_WIN : N % GSNP_WIN;
-            const uint64_t r1 = (uint64_t)(std::lower_bound(b.pos.begin(), b.pos.end(), (int64_t)ws + GSNP_WIN) - b.pos.begin());
-            rc = gsnp_window_update(ctx.get(), r, r1, nullptr);
-            if (rc) return rc;
-            r = r1;
-            rc = gsnp_window_likelihood(ctx.get(), C);
-            if (!rc) rc = gsnp_window_posterior(ctx.get(), C);
-            if (rc) return rc;
-            if (compress) {
-                rc = fetch_cols(*ctx, 0, ws, C, chr.code.data(), cols);
-                if (rc) return rc;
-                cw.write_window(chr.name, ws, cols);
-            } else {
-                uint64_t nb = 0;
-                rc = gsnp_window_text(ctx.get(), C, &nb);
-                if (rc) return rc;
-                host_text.resize(nb);
-                rc = gsnp_fetch_text(ctx.get(), host_text.data(), nb, &nb);
-                if (rc) return rc;
-                std::fwrite(host_text.data(), 1, nb, fo.get());
-            }
-            rc = gsnp_window_recycle(ctx.get());
-            if (rc) return rc;
-        }
     }
     if (compress && cw.close(err)) return fail(GSNP_E_IO, err);
     return GSNP_OK;
@@ -181,6 +246,58 @@ int gsnp_run(const char* fasta, const char* soap, const char* dbsnp, const char*
     if (!fasta || !soap || !p || !out) return gsnp::fail(GSNP_E_ARG, "null argument");
     return gsnp::run_pipeline(fasta, soap, dbsnp, matrix_in, matrix_out, *p, out, compress, batched);
 }
+
+int gsnp_job_open(const char* fasta, const char* soap, const char* dbsnp, const char* matrix_in, const gsnp_params* p,
+                  gsnp_job** out)
+{
+    if (!fasta || !soap || !p || !out) return gsnp::fail(GSNP_E_ARG, "null argument");
+    std::unique_ptr<gsnp_job> job;
+    int rc = gsnp::job_open(fasta, soap, dbsnp, matrix_in, nullptr, *p, job);
+    if (rc) return rc;
+    *out = job.release();
+    return GSNP_OK;
+}
+
+int gsnp_job_count(gsnp_job* job) { return job ? (int)job->blocks.size() : 0; }
+
+int gsnp_job_info(gsnp_job* job, int index, char* name, uint64_t cap, uint32_t* length, uint64_t* n_reads)
+{
+    if (!job || index < 0 || index >= (int)job->blocks.size()) return gsnp::fail(GSNP_E_ARG, "bad chromosome index");
+    const gsnp::ReadBlock& b = job->blocks[index];
+    const gsnp::Chrom& c = job->g.chr[b.chr];
+    if (name && cap) {
+        std::snprintf(name, cap, "%s", c.name.c_str());
+    }
+    if (length) *length = (uint32_t)c.code.size();
+    if (n_reads) *n_reads = b.size();
+    return GSNP_OK;
+}
+
+int gsnp_job_select(gsnp_job* job, int index)
+{
+    if (!job) return gsnp::fail(GSNP_E_ARG, "null job");
+    return gsnp::job_select(job, index);
+}
+
+int gsnp_job_call(gsnp_job* job, int batched, int emit_text, uint64_t* nbytes)
+{
+    if (!job || job->cur < 0) return gsnp::fail(GSNP_E_ARG, "no chromosome selected");
+    const gsnp::ReadBlock& b = job->blocks[job->cur];
+    const uint32_t N = (uint32_t)job->g.chr[b.chr].code.size();
+    uint64_t total = 0;
+    auto on_results = [](uint32_t, uint32_t, uint32_t) -> int { return GSNP_OK; };
+    auto on_text = [&](uint64_t nb) -> int {
+        total += nb;
+        return GSNP_OK;
+    };
+    int rc = gsnp::call_chromosome(job->ctx, b.pos, N, batched, emit_text != 0, on_results, on_text);
+    if (nbytes) *nbytes = total;
+    return rc;
+}
+
+gsnp_ctx* gsnp_job_context(gsnp_job* job) { return job ? job->ctx : nullptr; }
+
+void gsnp_job_close(gsnp_job* job) { delete job; }
 
 int gsnp_decompress(const char* prefix, const char* out)
 {

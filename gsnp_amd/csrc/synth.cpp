@@ -166,14 +166,26 @@ void emit_read(Xoshiro256ss& g, const Cfg& c, const Contig& ct, int hap, int64_t
     const int L = c.read_len;
     r.seq.resize(L);
     r.qual.resize(L);
+    // error probability 10^(-(q - delta)/10) per (cycle, q), delta = 0 unless miscalibrated
+    static thread_local std::vector<double> perr;
+    static thread_local int perr_key = -1;
+    const int key = L * 2 + (c.miscal ? 1 : 0);
+    if (perr_key != key) {
+        perr.assign((size_t)L * 64, 0.0);
+        for (int cy = 0; cy < L; cy++)
+            for (int q = 0; q < 64; q++) {
+                double qe = q;
+                if (c.miscal) qe = q - (3.0 * (q > 30) + 2.0 * cy / 99.0);
+                perr[(size_t)cy * 64 + q] = std::pow(10.0, -qe / 10.0);
+            }
+        perr_key = key;
+    }
     for (int k = 0; k < L; k++) {
         int cycle = strand ? L - 1 - k : k;
         int q = draw_q(g, c, cycle);
-        double qe = q;
-        if (c.miscal) qe = q - (3.0 * (q > 30) + 2.0 * cycle / 99.0);
         uint8_t b = ct.hap[hap][pos + k];
         if (b == 7) b = (uint8_t)g.below(4);  // reads over reference N carry real bases
-        if (g.uniform() < std::pow(10.0, -qe / 10.0)) b = (uint8_t)((b + 1 + g.below(3)) & 3);
+        if (g.uniform() < perr[(size_t)cycle * 64 + q]) b = (uint8_t)((b + 1 + g.below(3)) & 3);
         r.seq[k] = kBase[b];
         r.qual[k] = (char)(64 + q);
         if (c.stress) {

@@ -183,6 +183,38 @@ int gsnp_dict_compress_i32(gsnp_ctx* ctx, const int32_t* in, uint64_t n, uint32_
 int gsnp_sort_likelihood(gsnp_ctx* ctx, uint32_t* records, uint32_t pitch, const uint32_t* counts,
                          uint32_t n_sites, double* lk10);
 
+/* Per-stage device time from HIP events recorded on the context's stream:
+ * returns (and resets) the accumulated milliseconds and launch counts of the
+ * stages site build, sort, likelihood, posterior, rank-sum, format, recycle,
+ * fused (8 entries), then turns recording on or off.  Replaces the cutil
+ * timers of the reference (kernel/cuda_header.h:37-50). */
+#define GSNP_N_STAGES 8
+int gsnp_profile(gsnp_ctx* ctx, int enable, double* stage_ms, uint64_t* stage_count, int n_stages);
+
+/* Observation records the likelihood stage has consumed since the last
+ * reset (throughput accounting). */
+int gsnp_stats(gsnp_ctx* ctx, uint64_t* n_records, int reset);
+
+/* Session over input files: parses FASTA, SOAP and known SNPs, trains (or
+ * reads) the calibration matrix and creates the context, so that calling
+ * passes can be repeated on device-resident inputs.  Replaces the pass-1
+ * set-up of main, cpp/main.cc:363-403. */
+typedef struct gsnp_job gsnp_job;
+int gsnp_job_open(const char* fasta, const char* soap, const char* dbsnp, const char* matrix_in,
+                  const gsnp_params* p, gsnp_job** out);
+/* number of chromosomes with alignments, in call order */
+int gsnp_job_count(gsnp_job* job);
+int gsnp_job_info(gsnp_job* job, int index, char* name, uint64_t cap, uint32_t* length, uint64_t* n_reads);
+/* upload a chromosome's reference and reads to the device */
+int gsnp_job_select(gsnp_job* job, int index);
+/* one full calling pass over the selected chromosome (batched = windows per
+ * fused span, 0 for the per-window path); rows stay in device memory and
+ * *nbytes sums their size.  Replaces the window loop of new_soap2cns,
+ * cpp/call_genotype.cc:474-644. */
+int gsnp_job_call(gsnp_job* job, int batched, int emit_text, uint64_t* nbytes);
+gsnp_ctx* gsnp_job_context(gsnp_job* job);
+void gsnp_job_close(gsnp_job* job);
+
 /* %#g text of one double as written in .cns rows (host build of the device
  * formatter). Returns the length. */
 int gsnp_format_g(double v, char* out);
