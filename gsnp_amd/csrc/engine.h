@@ -124,6 +124,8 @@ struct Ctx {
     // text
     DevBuf<uint32_t> rowlen;
     DevBuf<uint64_t> blocksum, total;
+    DevBuf<uint64_t> ranges;  // read interval per 256-site block (site build)
+    bool keep_lk = false;     // batched form: also store the per-site log-likelihoods
     DevBuf<char> text;
     uint64_t text_cap = 0, text_len = 0;
     DevBuf<uint32_t> err;

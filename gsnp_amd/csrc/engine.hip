@@ -111,6 +111,7 @@ int Ctx::alloc_window(uint32_t n_sites)
     HIPCHK(fixes.alloc(n));
     HIPCHK(rowlen.alloc(n));
     HIPCHK(blocksum.alloc(n / kRowBlock + 2));
+    HIPCHK(ranges.alloc(2 * (n / 256 + 2)));
     HIPCHK(hipMemsetAsync(sites.nrec.p, 0, n * 4, stream));
     HIPCHK(hipMemsetAsync(sites.depth.p, 0, n * 4, stream));
     HIPCHK(hipMemsetAsync(sites.repeat.p, 0, n * 4, stream));
@@ -181,7 +182,7 @@ ReadView Ctx::read_view() const
     v.pos = rpos.p;
     v.meta = rmeta.p;
     v.seq = rseq.p;
-    v.pitch = pitch;
+    v.pitch = pitch ? pitch : 1u;
     return v;
 }
 
@@ -468,7 +469,7 @@ int gsnp_window_update(gsnp_ctx* c, uint64_t r0, uint64_t r1, uint32_t* max_reco
     const uint32_t L = (uint32_t)c->p.read_length;
     c->timer.begin(kStSiteBuild, c->stream);
     launch_site_build(c->stream, c->read_view(), r0, r1, c->ws, GSNP_WIN + L, (int64_t)c->ws,
-                      (int64_t)c->ws + GSNP_WIN - 1, L, c->site_view(0), nullptr, c->err.p);
+                      (int64_t)c->ws + GSNP_WIN - 1, L, c->site_view(0), nullptr, c->ranges.p);
     c->timer.end(kStSiteBuild, c->stream);
     HIPCHK(hipGetLastError());
     c->built = true;
@@ -491,11 +492,11 @@ int gsnp_window_likelihood(gsnp_ctx* c, uint32_t C)
     }
     SiteArrays a = c->site_view(0);
     c->timer.begin(kStSort, c->stream);
-    launch_bucket(c->stream, a.nrec, C, c->lists.p, c->win_cap, c->counts.p);
+    launch_bucket(c->stream, a.nrec, C, c->lists.p, c->win_cap, c->counts.p, 0, c->stat.p);
     launch_sort(c->stream, a.rec, a.nrec, c->lists.p, c->win_cap, c->counts.p, C);
     c->timer.end(kStSort, c->stream);
     c->timer.begin(kStLikelihood, c->stream);
-    launch_likelihood(c->stream, c->tables(), a, C, c->lk.p, c->err.p, c->stat.p);
+    launch_likelihood(c->stream, c->tables(), a, C, c->lk.p, c->err.p);
     c->timer.end(kStLikelihood, c->stream);
     HIPCHK(hipGetLastError());
     return GSNP_OK;
@@ -592,20 +593,27 @@ int gsnp_batched_call(gsnp_ctx* c, uint32_t start, uint32_t n, int emit_text, ui
     // every read that covers [start, start+n], no window batches and no carry
     c->timer.begin(kStSiteBuild, c->stream);
     launch_site_build(c->stream, c->read_view(), 0, c->n_reads, start, n + 1, INT64_MIN, (int64_t)start + n, 0, a,
-                      c->head_qs.p, c->err.p);
+                      c->head_qs.p, c->ranges.p);
     c->timer.end(kStSiteBuild, c->stream);
-    c->timer.begin(kStSort, c->stream);
-    launch_bucket(c->stream, a.nrec, n, c->lists.p, c->win_cap, c->counts.p);
-    launch_sort(c->stream, a.rec, a.nrec, c->lists.p, c->win_cap, c->counts.p, n);
-    c->timer.end(kStSort, c->stream);
-    c->timer.begin(kStLikelihood, c->stream);
-    launch_likelihood(c->stream, c->tables(), a, n, c->lk.p, c->err.p, c->stat.p);
-    c->timer.end(kStLikelihood, c->stream);
     HIPCHK(hipMemsetAsync(c->job_count.p, 0, 4, c->stream));
     HIPCHK(hipMemsetAsync(c->fix_count.p, 0, 4, c->stream));
-    c->timer.begin(kStPosterior, c->stream);
-    launch_posterior(c->stream, c->tables(), c->ref_view(), start, a, n, c->head_qs.p, res, c->jobs.p, c->job_count.p);
-    c->timer.end(kStPosterior, c->stream);
+    // every site into a depth class, then one fused sort+likelihood+posterior launch per class
+    c->timer.begin(kStFused, c->stream);
+    launch_bucket(c->stream, a.nrec, n, c->lists.p, c->win_cap, c->counts.p, 1, c->stat.p);
+    FusedArgs f;
+    f.t = c->tables();
+    f.ref = c->ref_view();
+    f.ws = start;
+    f.a = a;
+    f.head_qs = c->head_qs.p;
+    f.out = res;
+    f.jobs = c->jobs.p;
+    f.job_count = c->job_count.p;
+    f.err = c->err.p;
+    f.write_lk = c->keep_lk ? 1 : 0;
+    f.sw = ((uint32_t)(c->p.read_length + 7) / 8) | 1u;
+    launch_fused(c->stream, f, c->lists.p, c->win_cap, c->counts.p, n);
+    c->timer.end(kStFused, c->stream);
     if (c->p.rank_sum_mode) {
         c->timer.begin(kStRank, c->stream);
         launch_rank(c->stream, c->tables(), a, c->jobs.p, c->job_count.p, n, res, c->fixes.p, c->fix_count.p);
@@ -642,9 +650,9 @@ int gsnp_sort_likelihood(gsnp_ctx* c, uint32_t* records, uint32_t pitch, const u
     HIPCHK(hipMemcpyAsync(c->sites.rec.p, rows.data(), rows.size() * 4, hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipMemcpyAsync(c->sites.nrec.p, n.data(), 4ull * n_sites, hipMemcpyHostToDevice, c->stream));
     SiteArrays a = c->site_view(0);
-    launch_bucket(c->stream, a.nrec, n_sites, c->lists.p, c->win_cap, c->counts.p);
+    launch_bucket(c->stream, a.nrec, n_sites, c->lists.p, c->win_cap, c->counts.p, 0, nullptr);
     launch_sort(c->stream, a.rec, a.nrec, c->lists.p, c->win_cap, c->counts.p, n_sites);
-    launch_likelihood(c->stream, c->tables(), a, n_sites, c->lk.p, c->err.p, nullptr);
+    launch_likelihood(c->stream, c->tables(), a, n_sites, c->lk.p, c->err.p);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(rows.data(), c->sites.rec.p, rows.size() * 4, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipMemcpyAsync(lk10, c->lk.p, 80ull * n_sites, hipMemcpyDeviceToHost, c->stream));

@@ -82,17 +82,34 @@ struct RowCtx {
 // error flags written by kernels
 enum : uint32_t { kErrQadjRange = 1u, kErrRecordOverflow = 2u };
 
+// Arguments of the fused sort + likelihood + posterior launch (batched form).
+struct FusedArgs {
+    Tables t;
+    RefView ref;
+    uint32_t ws = 0;
+    SiteArrays a;
+    const int4* head_qs = nullptr;
+    Results out;
+    RankJob* jobs = nullptr;
+    uint32_t* job_count = nullptr;
+    uint32_t* err = nullptr;
+    int write_lk = 0;  // also store the ten log-likelihoods of each site
+    uint32_t sw = 1;   // words per lane of the dependency-count table, odd, >= ceil(L/8)
+};
+
+// range: 2 u64 per 256-site block (read interval of the block)
 void launch_site_build(hipStream_t s, const ReadView& r, uint64_t r0, uint64_t r1, uint32_t ws, uint32_t n_sites,
                        int64_t lo_min, int64_t hi_max, uint32_t carry, const SiteArrays& a, int4* head_qs,
-                       uint32_t* err);
+                       uint64_t* range);
 void launch_bucket(hipStream_t s, const int32_t* nrec, uint32_t n_sites, uint32_t* lists, uint32_t list_pitch,
-                   uint32_t* counts);
+                   uint32_t* counts, int all, unsigned long long* n_records);
 void launch_sort(hipStream_t s, uint32_t* rec, const int32_t* nrec, const uint32_t* lists, uint32_t list_pitch,
                  const uint32_t* counts, uint32_t max_sites);
-void launch_likelihood(hipStream_t s, const Tables& t, const SiteArrays& a, uint32_t n_sites, double* lk, uint32_t* err,
-                       unsigned long long* n_records);
+void launch_likelihood(hipStream_t s, const Tables& t, const SiteArrays& a, uint32_t n_sites, double* lk, uint32_t* err);
 void launch_posterior(hipStream_t s, const Tables& t, const RefView& ref, uint32_t ws, const SiteArrays& a,
                       uint32_t n_sites, const int4* head_qs, Results& out, RankJob* jobs, uint32_t* job_count);
+void launch_fused(hipStream_t s, const FusedArgs& f, const uint32_t* lists, uint32_t list_pitch, const uint32_t* counts,
+                  uint32_t max_sites);
 void launch_rank(hipStream_t s, const Tables& t, const SiteArrays& a, const RankJob* jobs, const uint32_t* job_count,
                  uint32_t max_jobs, Results& out, FixJob* fixes, uint32_t* fix_count);
 void launch_apply_fix(hipStream_t s, const FixJob* fixes, const double* rank, const int32_t* qcns, uint32_t n,

@@ -32,45 +32,69 @@ __device__ uint64_t lower_pos(const int32_t* pos, uint64_t lo, uint64_t hi, int6
 // ---------------------------------------------------------------------------
 // (a) site matrix.  One thread per site, 256 consecutive sites per block.  The
 // block walks, in input order, every read of [r0, r1) that can cover one of
-// its sites; read position and metadata are block-uniform (scalar loads) and
-// the lanes of a covering read fetch consecutive bytes of it (coalesced).
+// its sites.  The reads' positions, metadata and packed bytes are first
+// staged into LDS with wide coalesced loads (in chunks when a pileup is
+// deeper than the stage), so the walk itself runs at LDS latency; read fields
+// are block-uniform broadcasts and the lanes of a covering read take
+// consecutive bytes of it.
 // Per-site semantics follow kernel/update_kernel.cu:233-326: depth and
 // repeat for every covering read; for an observed base (ACGT, quality >=
 // q_min) while dep_uni < 255: unique hits append a record and bump
-// count_uni/q_sum/dep_uni; every hit bumps count_all.
+// count_uni/q_sum/dep_uni; every hit bumps count_all.  Records are buffered
+// four at a time in registers and stored as 16-byte words.
 // Sites below `carry` continue from the values already in memory (the
 // recycled read-length tail of the previous window); the others start at 0.
-// Only reads starting in [lo_min, hi_max] take part.  With head_qs set, a site
-// on a window boundary (absolute position a multiple of the window) also
-// records its q_sum before the reads that start on it, which is what the
-// per-window path holds in its first tail site when the window is called.
-__global__ __launch_bounds__(256) void k_site_build(ReadView r, uint64_t r0, uint64_t r1, uint32_t ws, uint32_t n_sites,
-                                                    int64_t lo_min, int64_t hi_max, uint32_t carry, SiteArrays a,
-                                                    int4* head_qs, uint32_t* err)
+// Only reads starting in [lo_min, hi_max] take part; `range` holds each
+// block's read interval (k_read_ranges).  With head_qs set, a site on a
+// window boundary (absolute position a multiple of the window) also records
+// its q_sum before the reads that start on it, which is what the per-window
+// path holds in its first tail site when the window is called.
+
+constexpr uint32_t kStageBytes = 32768;  // packed read bytes staged per chunk
+constexpr uint32_t kStageReads = 512;
+
+// read interval [b, e) of each 256-site block: reads starting in
+// [ws + s0 - (L-1), ws + s0 + 255] clipped to [lo_min, hi_max]
+__global__ __launch_bounds__(256) void k_read_ranges(const int32_t* pos, uint64_t r0, uint64_t r1, uint32_t ws,
+                                                     uint32_t n_blocks, uint32_t L, int64_t lo_min, int64_t hi_max,
+                                                     uint64_t* range)
 {
-    __shared__ uint64_t range[2];
-    const uint32_t s0 = blockIdx.x * 256u;
-    const uint32_t s = s0 + threadIdx.x;
-    const int L = (int)r.pitch;
-    if (threadIdx.x == 0) {
-        int64_t lo = (int64_t)ws + (int64_t)s0 - (L - 1);
-        if (lo < lo_min) lo = lo_min;
-        int64_t hi = (int64_t)ws + (int64_t)s0 + 255;
-        if (hi > hi_max) hi = hi_max;
-        range[0] = lower_pos(r.pos, r0, r1, lo);
-        range[1] = lower_pos(r.pos, r0, r1, hi + 1);
-    }
-    __syncthreads();
+    const uint32_t blk = blockIdx.x * 256u + threadIdx.x;
+    if (blk >= n_blocks) return;
+    const uint32_t s0 = blk * 256u;
+    int64_t lo = (int64_t)ws + (int64_t)s0 - ((int64_t)L - 1);
+    if (lo < lo_min) lo = lo_min;
+    int64_t hi = (int64_t)ws + (int64_t)s0 + 255;
+    if (hi > hi_max) hi = hi_max;
+    const uint64_t b = lower_pos(pos, r0, r1, lo);
+    uint64_t e = lower_pos(pos, b, r1, hi + 1);
+    if (e < b) e = b;
+    range[2 * blk] = b;
+    range[2 * blk + 1] = e;
+}
+
+__global__ __launch_bounds__(256) void k_site_build(ReadView r, const uint64_t* range, uint32_t ws, uint32_t n_sites,
+                                                    uint32_t carry, SiteArrays a, int4* head_qs)
+{
+    __shared__ __align__(16) uint8_t sseq[kStageBytes + 32];
+    __shared__ int32_t spos[kStageReads];
+    __shared__ uint32_t smeta[kStageReads];
+    const uint32_t s = blockIdx.x * 256u + threadIdx.x;
+    const uint32_t pitch = r.pitch;
+    const uint32_t kr = kStageBytes / pitch < kStageReads ? kStageBytes / pitch : kStageReads;
     // block-uniform bounds: keep the read loop on scalar registers
-    const uint64_t b = ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(range[0] >> 32)) << 32) |
-                       (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)range[0]);
-    const uint64_t e = ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(range[1] >> 32)) << 32) |
-                       (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)range[1]);
+    const uint64_t rb = range[2 * blockIdx.x], re = range[2 * blockIdx.x + 1];
+    const uint64_t b = ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(rb >> 32)) << 32) |
+                       (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)rb);
+    const uint64_t e = ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(re >> 32)) << 32) |
+                       (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)re);
     const bool live = s < n_sites;
     int depth = 0, repeat = 0, nrec = 0;
     uint32_t cu = 0;   // 4 x 8-bit count_uni
     uint64_t qs = 0;   // 4 x 16-bit q_sum
     int ca0 = 0, ca1 = 0, ca2 = 0, ca3 = 0;
+    uint32_t* row = a.rec + (size_t)s * kSlots;
+    uint32_t pb0 = 0, pb1 = 0, pb2 = 0, pb3 = 0;  // records of slots [nrec & ~3, nrec)
     if (live && s < carry) {
         depth = a.depth[s];
         repeat = a.repeat[s];
@@ -80,43 +104,71 @@ __global__ __launch_bounds__(256) void k_site_build(ReadView r, uint64_t r0, uin
         qs = (uint64_t)(uint32_t)q.x | ((uint64_t)(uint32_t)q.y << 16) | ((uint64_t)(uint32_t)q.z << 32) |
              ((uint64_t)(uint32_t)q.w << 48);
         ca0 = al.x, ca1 = al.y, ca2 = al.z, ca3 = al.w;
+        if (nrec & 3) {
+            const uint4 v = *reinterpret_cast<const uint4*>(row + (nrec & ~3));
+            pb0 = v.x, pb1 = v.y, pb2 = v.z, pb3 = v.w;
+        }
     }
     const int64_t p = (int64_t)ws + s;
-    uint32_t* row = a.rec + (size_t)s * kSlots;
     const bool head = head_qs && live && (p % kWin) == 0;
     bool snapped = false;
-    for (uint64_t i = b; i < e; i++) {
-        const int64_t start = r.pos[i];
-        const uint32_t m = r.meta[i];
-        if (head && !snapped && start >= p) {
-            head_qs[p / kWin - ws / kWin] = make_int4((int)(qs & 0xffff), (int)((qs >> 16) & 0xffff),
-                                                      (int)((qs >> 32) & 0xffff), (int)(qs >> 48));
-            snapped = true;
+    for (uint64_t c0 = b; c0 < e; c0 += kr) {
+        const uint32_t cn = (uint32_t)(e - c0 < kr ? e - c0 : kr);
+        __syncthreads();  // the previous chunk has been consumed
+        for (uint32_t i = threadIdx.x; i < cn; i += 256) {
+            spos[i] = r.pos[c0 + i];
+            smeta[i] = r.meta[c0 + i];
         }
-        const int64_t coord = p - start;
-        const uint32_t len = rm_len(m);
-        if (!live || coord < 0 || coord >= (int64_t)len) continue;
-        const uint32_t hit = rm_hit(m);
-        depth += 1;
-        repeat += (int)hit;
-        const uint32_t byte = r.seq[i * r.pitch + (uint64_t)coord];
-        if (byte == 0 || nrec >= (int)kMaxDepUni) continue;
-        const uint32_t o = pb_base(byte);
-        if (hit == 1) {
-            const uint32_t q = pb_q(byte);
-            const uint32_t strand = rm_strand(m);
-            const uint32_t cyc = strand ? (uint32_t)(len - 1 - coord) : (uint32_t)coord;
-            row[nrec] = rec_encode(cyc, strand, o, q, 1);
-            nrec += 1;
-            cu += 1u << (8 * o);
-            qs += (uint64_t)q << (16 * o);
+        // bytes [c0*pitch, (c0+cn)*pitch) from the 16-byte boundary below;
+        // the read store is padded so the last vector stays in the allocation
+        const uint64_t g0 = c0 * pitch;
+        const uint64_t a0 = g0 & ~15ull;
+        const uint32_t delta = (uint32_t)(g0 - a0);
+        const uint32_t nvec = (uint32_t)((g0 + (uint64_t)cn * pitch - a0 + 15) >> 4);
+        const uint4* src = reinterpret_cast<const uint4*>(r.seq + a0);
+        uint4* dst = reinterpret_cast<uint4*>(sseq);
+        for (uint32_t v = threadIdx.x; v < nvec; v += 256) dst[v] = src[v];
+        __syncthreads();
+        for (uint32_t j = 0; j < cn; j++) {
+            const int64_t start = spos[j];
+            const uint32_t m = smeta[j];
+            if (head && !snapped && start >= p) {
+                head_qs[p / kWin - ws / kWin] = make_int4((int)(qs & 0xffff), (int)((qs >> 16) & 0xffff),
+                                                          (int)((qs >> 32) & 0xffff), (int)(qs >> 48));
+                snapped = true;
+            }
+            const int64_t coord = p - start;
+            const uint32_t len = rm_len(m);
+            if (!live || coord < 0 || coord >= (int64_t)len) continue;
+            const uint32_t hit = rm_hit(m);
+            depth += 1;
+            repeat += (int)hit;
+            const uint32_t byte = sseq[delta + j * pitch + (uint32_t)coord];
+            if (byte == 0 || nrec >= (int)kMaxDepUni) continue;
+            const uint32_t o = pb_base(byte);
+            if (hit == 1) {
+                const uint32_t q = pb_q(byte);
+                const uint32_t strand = rm_strand(m);
+                const uint32_t cyc = strand ? (uint32_t)(len - 1 - coord) : (uint32_t)coord;
+                const uint32_t w = rec_encode(cyc, strand, o, q, 1);
+                const int k = nrec & 3;
+                pb0 = k == 0 ? w : pb0;
+                pb1 = k == 1 ? w : pb1;
+                pb2 = k == 2 ? w : pb2;
+                pb3 = k == 3 ? w : pb3;
+                nrec += 1;
+                if ((nrec & 3) == 0) *reinterpret_cast<uint4*>(row + (nrec - 4)) = make_uint4(pb0, pb1, pb2, pb3);
+                cu += 1u << (8 * o);
+                qs += (uint64_t)q << (16 * o);
+            }
+            ca0 += o == 0;
+            ca1 += o == 1;
+            ca2 += o == 2;
+            ca3 += o == 3;
         }
-        ca0 += o == 0;
-        ca1 += o == 1;
-        ca2 += o == 2;
-        ca3 += o == 3;
     }
     if (!live) return;
+    if (nrec & 3) *reinterpret_cast<uint4*>(row + (nrec & ~3)) = make_uint4(pb0, pb1, pb2, pb3);
     if (head && !snapped)
         head_qs[p / kWin - ws / kWin] =
             make_int4((int)(qs & 0xffff), (int)((qs >> 16) & 0xffff), (int)((qs >> 32) & 0xffff), (int)(qs >> 48));
@@ -126,17 +178,17 @@ __global__ __launch_bounds__(256) void k_site_build(ReadView r, uint64_t r0, uin
     a.cu[s] = make_int4((int)(cu & 0xff), (int)((cu >> 8) & 0xff), (int)((cu >> 16) & 0xff), (int)(cu >> 24));
     a.qs[s] = make_int4((int)(qs & 0xffff), (int)((qs >> 16) & 0xffff), (int)((qs >> 32) & 0xffff), (int)(qs >> 48));
     a.ca[s] = make_int4(ca0, ca1, ca2, ca3);
-    (void)err;
 }
 
 void launch_site_build(hipStream_t st, const ReadView& r, uint64_t r0, uint64_t r1, uint32_t ws, uint32_t n_sites,
                        int64_t lo_min, int64_t hi_max, uint32_t carry, const SiteArrays& a, int4* head_qs,
-                       uint32_t* err)
+                       uint64_t* range)
 {
     const uint32_t blocks = (n_sites + 255) / 256;
-    if (blocks)
-        hipLaunchKernelGGL(k_site_build, dim3(blocks), dim3(256), 0, st, r, r0, r1, ws, n_sites, lo_min, hi_max, carry,
-                           a, head_qs, err);
+    if (!blocks) return;
+    hipLaunchKernelGGL(k_read_ranges, dim3((blocks + 255) / 256), dim3(256), 0, st, r.pos, r0, r1, ws, blocks, r.pitch,
+                       lo_min, hi_max, range);
+    hipLaunchKernelGGL(k_site_build, dim3(blocks), dim3(256), 0, st, r, range, ws, n_sites, carry, a, head_qs);
 }
 
 // ---------------------------------------------------------------------------
@@ -149,34 +201,72 @@ void launch_site_build(hipStream_t st, const ReadView& r, uint64_t r0, uint64_t 
 // order (base, quality descending, cycle, strand) comes out in the first n
 // slots, as kernel/bitonic_kernel.cu:119-176 produces.
 
+// Bucketing: each block classifies 4096 sites (16 per thread) with
+// wave-aggregated LDS counters, reserves its slice of every list with one
+// global atomic per class, then scatters the site indices.  With `all` set,
+// sites of 0 or 1 record go to the first class too (the fused path calls
+// every site); the records of all sites are summed into n_records.
+constexpr uint32_t kBucketPer = 16;
+
 __global__ __launch_bounds__(256) void k_bucket(const int32_t* nrec, uint32_t n_sites, uint32_t* lists,
-                                                uint32_t pitch, uint32_t* counts)
+                                                uint32_t pitch, uint32_t* counts, int all,
+                                                unsigned long long* n_records)
 {
-    const uint32_t s = blockIdx.x * 256u + threadIdx.x;
-    int b = -1;
-    if (s < n_sites) {
-        const int n = nrec[s];
-        b = n <= 1 ? -1 : n <= 8 ? 0 : n <= 16 ? 1 : n <= 32 ? 2 : n <= 64 ? 3 : n <= 128 ? 4 : 5;
-    }
+    __shared__ uint32_t cnt[6], base[6];
+    __shared__ unsigned long long rsum[4];
+    if (threadIdx.x < 6) cnt[threadIdx.x] = 0;
+    __syncthreads();
     const uint32_t lane = lane_id();
     const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    for (int k = 0; k < 6; k++) {
-        const uint64_t mask = __ballot(b == k);
-        if (!mask) continue;
-        const uint32_t leader = (uint32_t)__ffsll((long long)mask) - 1;
-        uint32_t base = 0;
-        if (lane == leader) base = atomicAdd(&counts[k], (uint32_t)__popcll(mask));
-        base = __shfl(base, (int)leader);
-        if (b == k) lists[(size_t)k * pitch + base + (uint32_t)__popcll(mask & below)] = s;
+    const uint32_t s0 = blockIdx.x * 256u * kBucketPer;
+    int cls[kBucketPer];
+    uint32_t loc[kBucketPer];
+    unsigned long long recs = 0;
+#pragma unroll
+    for (uint32_t it = 0; it < kBucketPer; it++) {
+        const uint32_t s = s0 + it * 256u + threadIdx.x;
+        int b = -1;
+        if (s < n_sites) {
+            const int n = nrec[s];
+            recs += (unsigned long long)n;
+            b = n <= 1 ? (all ? 0 : -1) : n <= 8 ? 0 : n <= 16 ? 1 : n <= 32 ? 2 : n <= 64 ? 3 : n <= 128 ? 4 : 5;
+        }
+        cls[it] = b;
+        loc[it] = 0;
+        for (int k = 0; k < 6; k++) {
+            const uint64_t mask = __ballot(b == k);
+            if (!mask) continue;
+            const uint32_t leader = (uint32_t)__ffsll((long long)mask) - 1;
+            uint32_t off = 0;
+            if (lane == leader) off = atomicAdd(&cnt[k], (uint32_t)__popcll(mask));
+            off = __shfl(off, (int)leader);
+            if (b == k) loc[it] = off + (uint32_t)__popcll(mask & below);
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) recs += __shfl_xor(recs, off);
+    if (lane == 0) rsum[threadIdx.x >> 6] = recs;
+    __syncthreads();
+    if (threadIdx.x < 6) base[threadIdx.x] = cnt[threadIdx.x] ? atomicAdd(&counts[threadIdx.x], cnt[threadIdx.x]) : 0u;
+    if (threadIdx.x == 0 && n_records) {
+        const unsigned long long t = rsum[0] + rsum[1] + rsum[2] + rsum[3];
+        if (t) atomicAdd(n_records, t);
+    }
+    __syncthreads();
+#pragma unroll
+    for (uint32_t it = 0; it < kBucketPer; it++) {
+        const int b = cls[it];
+        if (b >= 0) lists[(size_t)b * pitch + base[b] + loc[it]] = s0 + it * 256u + threadIdx.x;
     }
 }
 
 void launch_bucket(hipStream_t st, const int32_t* nrec, uint32_t n_sites, uint32_t* lists, uint32_t pitch,
-                   uint32_t* counts)
+                   uint32_t* counts, int all, unsigned long long* n_records)
 {
     hipMemsetAsync(counts, 0, 6 * sizeof(uint32_t), st);
-    const uint32_t blocks = (n_sites + 255) / 256;
-    if (blocks) hipLaunchKernelGGL(k_bucket, dim3(blocks), dim3(256), 0, st, nrec, n_sites, lists, pitch, counts);
+    const uint32_t blocks = (n_sites + 256 * kBucketPer - 1) / (256 * kBucketPer);
+    if (blocks)
+        hipLaunchKernelGGL(k_bucket, dim3(blocks), dim3(256), 0, st, nrec, n_sites, lists, pitch, counts, all,
+                           n_records);
 }
 
 template <int J>
@@ -344,17 +434,12 @@ void launch_sort(hipStream_t st, uint32_t* rec, const int32_t* nrec, const uint3
 // set marks the (strand, cycle) keys seen in the segment; only a repeated key
 // rescans the segment to count its earlier occurrences.
 __global__ __launch_bounds__(256) void k_likelihood(Tables t, SiteArrays a, uint32_t n_sites, double* lk,
-                                                    uint32_t* err, unsigned long long* n_records)
+                                                    uint32_t* err)
 {
     __shared__ uint32_t seen_all[256 * 17];
     uint32_t* seen = seen_all + threadIdx.x * 17;
     const uint32_t s = blockIdx.x * 256u + threadIdx.x;
     const uint32_t n = s < n_sites ? (uint32_t)a.nrec[s] : 0u;
-    if (n_records) {  // records called, for throughput accounting
-        unsigned long long w = n;
-        for (int off = 32; off > 0; off >>= 1) w += __shfl_xor(w, off);
-        if (lane_id() == 0 && w) atomicAdd(n_records, w);
-    }
     if (s >= n_sites) return;
     const uint32_t* row = a.rec + (size_t)s * kSlots;
     double acc[10];
@@ -411,11 +496,10 @@ __global__ __launch_bounds__(256) void k_likelihood(Tables t, SiteArrays a, uint
     if (bad) atomicOr(err, bad);
 }
 
-void launch_likelihood(hipStream_t st, const Tables& t, const SiteArrays& a, uint32_t n_sites, double* lk, uint32_t* err,
-                       unsigned long long* n_records)
+void launch_likelihood(hipStream_t st, const Tables& t, const SiteArrays& a, uint32_t n_sites, double* lk, uint32_t* err)
 {
     const uint32_t blocks = (n_sites + 255) / 256;
-    if (blocks) hipLaunchKernelGGL(k_likelihood, dim3(blocks), dim3(256), 0, st, t, a, n_sites, lk, err, n_records);
+    if (blocks) hipLaunchKernelGGL(k_likelihood, dim3(blocks), dim3(256), 0, st, t, a, n_sites, lk, err);
 }
 
 // ---------------------------------------------------------------------------
@@ -429,12 +513,12 @@ __device__ __forceinline__ int get4(const int4& v, uint32_t i)
     return i == 0 ? v.x : i == 1 ? v.y : i == 2 ? v.z : v.w;
 }
 
-__global__ __launch_bounds__(256) void k_posterior(Tables t, RefView ref, uint32_t ws, SiteArrays a, uint32_t n_sites,
-                                                   const int4* head_qs, Results out, RankJob* jobs,
-                                                   uint32_t* job_count)
+// Calls site s from its ten log-likelihoods lkv(0..9).
+template <class LK>
+__device__ __forceinline__ void posterior_site(const Tables& t, const RefView& ref, uint32_t ws, const SiteArrays& a,
+                                               uint32_t s, LK lkv, const int4* head_qs, const Results& out,
+                                               RankJob* jobs, uint32_t* job_count)
 {
-    const uint32_t s = blockIdx.x * 256u + threadIdx.x;
-    if (s >= n_sites) return;
     const uint32_t ori = ref.code[ws + s] & 0xF;
     const int depth = a.depth[s];
     if ((ori & 4) && depth == 0) return;  // N row, not called (nflag)
@@ -486,14 +570,13 @@ __global__ __launch_bounds__(256) void k_posterior(Tables t, RefView ref, uint32
         }
         if (lo < ref.n_snp && ref.snp_pos[lo] == p) lp = ref.snp_lp + (size_t)lo * 10;
     }
-    const double* lk = out.lk + (size_t)s * 10;
     uint32_t type1 = 16, type2 = 16;
     double tp1 = 0.0, tp2 = 0.0;
 #pragma unroll
     for (int g = 0; g < 10; g++) {
         const uint32_t gt = geno_code(g);
         if (t.monoploid && (gt >> 2) != (gt & 3)) continue;
-        const double v = lk[g] + lp[g];
+        const double v = lkv(g) + lp[g];
         if (v >= tp1 || type1 == 16) {
             type2 = type1;
             tp2 = tp1;
@@ -509,16 +592,17 @@ __global__ __launch_bounds__(256) void k_posterior(Tables t, RefView ref, uint32
     // HET cap inputs; q_sum is flat over sites, so base codes 4..7 read the next
     // site -- for the last site of a window, as it stands before the reads
     // that start on the window boundary (head_qs in the batched span)
-    auto qsum_flat = [&](uint32_t b) -> int {
-        if (b < 4) return get4(qs, b);
-        const uint32_t nx = ws + s + 1;
-        const int4 nq = (head_qs && nx % kWin == 0) ? head_qs[nx / kWin - ws / kWin] : a.qs[s + 1];
-        return get4(nq, b - 4);
-    };
     bool het_ok = false;
     if (!hom) {
         const uint32_t het = base1 < base2 ? (base1 << 2 | base2) : (base2 << 2 | base1);
-        het_ok = qsum_flat(base1) > 0 && qsum_flat(base2) > 0 && type1 == het;
+        int4 nq = qs;
+        if (base1 >= 4 || base2 >= 4) {
+            const uint32_t nx = ws + s + 1;
+            nq = (head_qs && nx % kWin == 0) ? head_qs[nx / kWin - ws / kWin] : a.qs[s + 1];
+        }
+        const int q_b1 = base1 < 4 ? get4(qs, base1) : get4(nq, base1 & 3);
+        const int q_b2 = base2 < 4 ? get4(qs, base2) : get4(nq, base2 & 3);
+        het_ok = q_b1 > 0 && q_b2 > 0 && type1 == het;
     }
     double rank = 1.0;
     int q_cns;
@@ -528,14 +612,12 @@ __global__ __launch_bounds__(256) void k_posterior(Tables t, RefView ref, uint32
             rank = 0.0;
         } else {
             const uint32_t j = atomicAdd(job_count, 1u);
-            RankJob jb;
-            jb.site = s;
-            jb.type1 = (uint8_t)type1;
-            jb.het_ok = het_ok;
-            jb.pad0 = jb.pad1 = 0;
-            jb.cap = qual2 - qual3;
-            jb.dtp = dtp;
-            jobs[j] = jb;
+            // written as three words (a struct copy with byte fields would go through scratch)
+            uint32_t* jw = reinterpret_cast<uint32_t*>(jobs + j);
+            jw[0] = s;
+            jw[1] = (type1 & 0xffu) | ((het_ok ? 1u : 0u) << 8);
+            jw[2] = (uint32_t)(qual2 - qual3);
+            jobs[j].dtp = dtp;
             out.base1[s] = (uint8_t)base1;
             out.base2[s] = (uint8_t)base2;
             out.type1[s] = (uint8_t)type1;
@@ -567,6 +649,16 @@ __global__ __launch_bounds__(256) void k_posterior(Tables t, RefView ref, uint32
     out.rank[s] = rank;
 }
 
+__global__ __launch_bounds__(256) void k_posterior(Tables t, RefView ref, uint32_t ws, SiteArrays a, uint32_t n_sites,
+                                                   const int4* head_qs, Results out, RankJob* jobs,
+                                                   uint32_t* job_count)
+{
+    const uint32_t s = blockIdx.x * 256u + threadIdx.x;
+    if (s >= n_sites) return;
+    const double* lk = out.lk + (size_t)s * 10;
+    posterior_site(t, ref, ws, a, s, [&](int g) { return lk[g]; }, head_qs, out, jobs, job_count);
+}
+
 void launch_posterior(hipStream_t st, const Tables& t, const RefView& ref, uint32_t ws, const SiteArrays& a,
                       uint32_t n_sites, const int4* head_qs, Results& out, RankJob* jobs, uint32_t* job_count)
 {
@@ -574,6 +666,244 @@ void launch_posterior(hipStream_t st, const Tables& t, const RefView& ref, uint3
     if (blocks)
         hipLaunchKernelGGL(k_posterior, dim3(blocks), dim3(256), 0, st, t, ref, ws, a, n_sites, head_qs, out, jobs,
                            job_count);
+}
+
+// ---------------------------------------------------------------------------
+// Batched form: sort + likelihood + posterior fused per depth class.  A wave
+// takes a group of G sites of one class (record count <= S) from the bucket
+// lists; lane i owns site i of the group.
+//  1. Sort: for S <= 64 every record slot of the group is loaded first (S
+//     loads per lane in flight together), then lane groups of S sort 64/S
+//     sites per pass with the k_sort_lanes networks; for S = 128/256 one
+//     site per pass with S/64 records per lane.  Sorted rows are parked in
+//     LDS as [G][S+1] words (conflict-free both ways).
+//  2a. One lane per site walks its sorted row: the dependency counts t and g
+//     of each record (t from a per-lane table of 2-bit saturating counters
+//     over the segment's (strand, cycle) keys; from the third occurrence on,
+//     an exact recount) give the adjusted-quality table index, which replaces
+//     the record in LDS together with its cycle and base.
+//  2b. The lane sums the ten score rows in record order, four records per
+//     step so that their table loads are in flight together; the additions
+//     stay in record order, as in k_likelihood.
+//  3. posterior_site calls the site from the sums held in registers.
+// Records come from k_site_build, so every occurrence field is 1.
+
+constexpr uint32_t kRecKeyMask = 0x0181FF00u;  // base, cycle, strand fields of a record
+constexpr uint32_t kRecQMask = 0x007E0000u;    // 64-q field
+
+template <int S>
+constexpr uint32_t fused_group()
+{
+    return S <= 64 ? 64u : 4096u / S;
+}
+
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <int S>
+__global__ __launch_bounds__(256) void k_fused(FusedArgs f, const uint32_t* list, const uint32_t* count_ptr)
+{
+    constexpr uint32_t G = fused_group<S>();
+    constexpr uint32_t RS = S + 1;  // LDS row stride of a sorted site
+    extern __shared__ uint32_t lds[];
+    const uint32_t lane = lane_id();
+    const uint32_t wib = threadIdx.x >> 6;
+    const uint32_t per_wave = G * RS + 64u * f.sw;
+    uint32_t* srt = lds + wib * per_wave;
+    uint32_t* seen = srt + G * RS + lane * f.sw;
+    const uint32_t count = *count_ptr;
+    const uint32_t ngroups = (count + G - 1) / G;
+    const uint32_t wave = blockIdx.x * (blockDim.x >> 6) + wib;
+    const uint32_t nwave = gridDim.x * (blockDim.x >> 6);
+    const uint32_t L = (uint32_t)f.t.read_len;
+    const uint32_t td = (uint32_t)f.t.td, gd = (uint32_t)f.t.gd;
+    for (uint32_t grp = wave; grp < ngroups; grp += nwave) {
+        const uint32_t g0 = grp * G;
+        const uint32_t gcount = count - g0 < G ? count - g0 : G;
+        // lane gi owns site gi of the group
+        const bool act = lane < gcount;
+        const uint32_t site = act ? list[g0 + lane] : 0u;
+        const uint32_t n = act ? (uint32_t)f.a.nrec[site] : 0u;
+        wave_lds_sync();  // the previous group's rows have been read
+        // phase 1: sort into LDS
+        if constexpr (S <= 64) {
+            // all G*S record slots of the group are loaded first (S per lane),
+            // so the loads are in flight together; then the P = 64/S sites of
+            // each pass are sorted by lane groups of S
+            constexpr uint32_t P = 64 / S, NP = G / P;
+            const uint32_t sub = lane / S, e = lane % S;
+            uint32_t v[NP];
+#pragma unroll
+            for (uint32_t k = 0; k < NP; k++) {
+                const uint32_t gi = k * P + sub;
+                const uint32_t st = (uint32_t)__shfl((int)site, (int)gi);
+                const uint32_t nn = (uint32_t)__shfl((int)n, (int)gi);
+                v[k] = e < nn ? f.a.rec[(size_t)st * kSlots + e] : 0xFFFFFFFFu;
+            }
+#pragma unroll
+            for (uint32_t k = 0; k < NP; k++) {
+                const uint32_t gi = k * P + sub;
+                const uint32_t nn = (uint32_t)__shfl((int)n, (int)gi);
+                bitonic_all<2, S>(v[k], e);
+                if (e < nn) srt[gi * RS + e] = v[k];
+            }
+        } else {
+            constexpr int R = S / 64;
+            for (uint32_t gi = 0; gi < gcount; gi++) {
+                const uint32_t st = (uint32_t)__shfl((int)site, (int)gi);
+                const uint32_t nn = (uint32_t)__shfl((int)n, (int)gi);
+                const uint32_t* row = f.a.rec + (size_t)st * kSlots;
+                uint32_t v[R];
+#pragma unroll
+                for (int r = 0; r < R; r++) {
+                    const uint32_t e = (uint32_t)r * 64u + lane;
+                    v[r] = e < nn ? row[e] : 0xFFFFFFFFu;
+                }
+#pragma unroll
+                for (int r = 0; r < R; r++) bitonic_all<2, 64>(v[r], (uint32_t)(r * 64) + lane);
+                regs_merge<R, 128>(v, lane);
+                if constexpr (S == 256) regs_merge<R, 256>(v, lane);
+#pragma unroll
+                for (int r = 0; r < R; r++) {
+                    const uint32_t e = (uint32_t)r * 64u + lane;
+                    if (e < nn) srt[gi * RS + e] = v[r];
+                }
+            }
+        }
+        wave_lds_sync();
+        // phase 2a, one lane per site: dependency counts t, g of each record
+        // in sorted order; the record is replaced by its adjusted-quality
+        // table index and score-row fields, (qidx << 10) | (cycle << 2) | o
+        uint32_t* row = srt + lane * RS;
+        for (uint32_t k = 0; k < f.sw; k++) seen[k] = 0u;
+        int last_o = -1, gdc = -1;
+        uint32_t bad = 0, prev_w = 0xFFFFFFFFu, run_start = 0;
+        for (uint32_t i = 0; i < n; i++) {
+            const uint32_t w = row[i];
+            const uint32_t o = rec_base(w);
+            uint32_t q = rec_q(w);
+            const uint32_t cyc = rec_cycle(w);
+            if (w != prev_w) {
+                run_start = i;
+                prev_w = w;
+            }
+            if ((int)o != last_o) {
+                if (last_o >= 0)
+                    for (uint32_t k = 0; k < f.sw; k++) seen[k] = 0u;
+                gdc = -1;
+                last_o = (int)o;
+            }
+            uint32_t key = rec_strand(w) * L + cyc;
+            if (cyc >= L) {  // cannot come from k_site_build (cycle < read length)
+                bad |= kErrRecordOverflow;
+                key = 0;
+            }
+            const uint32_t wi = key >> 4, sh = (key & 15u) * 2u;
+            const uint32_t word = seen[wi];
+            const uint32_t stt = (word >> sh) & 3u;
+            uint32_t prior = stt;
+            if (stt >= 2u) {
+                // third occurrence or later.  The earlier copies precede it in
+                // sorted order: the same key with a higher quality, or the same
+                // word earlier in its run.  The LDS row already holds rewritten
+                // entries, so the higher-quality copies are counted in the
+                // unsorted row in device memory.
+                prior = i - run_start;
+                const uint32_t* hrow = f.a.rec + (size_t)site * kSlots;
+                const uint32_t kv = w & kRecKeyMask, qf = w & kRecQMask;
+                for (uint32_t j = 0; j < n; j++) {
+                    const uint32_t x = hrow[j];
+                    prior += ((x & kRecKeyMask) == kv && (x & kRecQMask) < qf) ? 1u : 0u;
+                }
+            } else {
+                seen[wi] = word + (1u << sh);
+            }
+            if (prior == 0) gdc += 1;
+            if (q > 63) {
+                bad |= kErrQadjRange;
+                q = 63;
+            }
+            const uint32_t tm1 = prior < td ? prior : td - 1;
+            const uint32_t gi = (uint32_t)gdc < gd ? (uint32_t)gdc : gd - 1;
+            const uint32_t qidx = (q * td + tm1) * gd + gi;
+            row[i] = (qidx << 10) | (cyc << 2) | o;
+        }
+        // phase 2b: adjusted qualities and score rows, summed in record order;
+        // four records per step so their loads are in flight together
+        double acc[10];
+#pragma unroll
+        for (int g = 0; g < 10; g++) acc[g] = 0.0;
+        for (uint32_t i = 0; i < n; i += 4) {
+            uint32_t x[4], qa[4];
+#pragma unroll
+            for (int j = 0; j < 4; j++) x[j] = i + j < n ? row[i + j] : 0u;
+#pragma unroll
+            for (int j = 0; j < 4; j++) qa[j] = f.t.qadj[x[j] >> 10];
+            double2 v[4][5];
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                uint32_t a = qa[j];
+                if (i + j < n && a == 0) {
+                    bad |= kErrQadjRange;
+                    a = 1;
+                }
+                const double2* sr =
+                    reinterpret_cast<const double2*>(f.t.score + (size_t)((a << 10) | (x[j] & 1023u)) * 10);
+#pragma unroll
+                for (int h = 0; h < 5; h++) v[j][h] = sr[h];
+            }
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                if (i + j < n) {
+#pragma unroll
+                    for (int h = 0; h < 5; h++) {
+                        acc[2 * h] += v[j][h].x;
+                        acc[2 * h + 1] += v[j][h].y;
+                    }
+                }
+            }
+        }
+        if (bad) atomicOr(f.err, bad);
+        if (!act) continue;
+        if (f.write_lk) {
+            double2* out = reinterpret_cast<double2*>(f.out.lk + (size_t)site * 10);
+#pragma unroll
+            for (int h = 0; h < 5; h++) out[h] = make_double2(acc[2 * h], acc[2 * h + 1]);
+        }
+        // phase 3: posterior from the sums in registers
+        posterior_site(f.t, f.ref, f.ws, f.a, site, [&](int g) { return acc[g]; }, f.head_qs, f.out, f.jobs,
+                       f.job_count);
+    }
+}
+
+template <int S>
+static void launch_fused_class(hipStream_t st, const FusedArgs& f, const uint32_t* list, const uint32_t* count,
+                               uint32_t max_sites)
+{
+    constexpr uint32_t G = fused_group<S>();
+    const uint32_t per_wave = (G * (S + 1) + 64u * f.sw) * 4u;
+    const uint32_t wpb = 4u * per_wave <= 65536u ? 4u : 2u * per_wave <= 65536u ? 2u : 1u;
+    const uint32_t groups = (max_sites + G - 1) / G;
+    uint32_t blocks = (groups + wpb - 1) / wpb;
+    if (blocks > 8192u) blocks = 8192u;
+    if (!blocks) blocks = 1;
+    hipLaunchKernelGGL(k_fused<S>, dim3(blocks), dim3(64 * wpb), wpb * per_wave, st, f, list, count);
+}
+
+void launch_fused(hipStream_t st, const FusedArgs& f, const uint32_t* lists, uint32_t pitch, const uint32_t* counts,
+                  uint32_t max_sites)
+{
+    if (!max_sites) return;
+    launch_fused_class<256>(st, f, lists + 5 * (size_t)pitch, counts + 5, max_sites);
+    launch_fused_class<128>(st, f, lists + 4 * (size_t)pitch, counts + 4, max_sites);
+    launch_fused_class<64>(st, f, lists + 3 * (size_t)pitch, counts + 3, max_sites);
+    launch_fused_class<32>(st, f, lists + 2 * (size_t)pitch, counts + 2, max_sites);
+    launch_fused_class<16>(st, f, lists + 1 * (size_t)pitch, counts + 1, max_sites);
+    launch_fused_class<8>(st, f, lists + 0 * (size_t)pitch, counts + 0, max_sites);
 }
 
 // ---------------------------------------------------------------------------
