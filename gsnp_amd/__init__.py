@@ -130,6 +130,13 @@ SIGNATURES = {
     "gsnp_job_close": (None, [_P]),
     "gsnp_synth_write": (ctypes.c_longlong, [ctypes.c_int, ctypes.c_ulonglong, ctypes.c_double, ctypes.c_char_p,
                                              ctypes.POINTER(ctypes.c_int)]),
+    "gsnp_adjoint_prepare": (ctypes.c_int, [_P, ctypes.c_uint32, ctypes.c_uint32]),
+    "gsnp_adjoint_eval": (ctypes.c_int, [_P, _P, ctypes.c_double, ctypes.POINTER(ctypes.c_double), _P,
+                                         ctypes.POINTER(ctypes.c_double), _P]),
+    "gsnp_adjoint_info": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint64),
+                                         ctypes.POINTER(ctypes.c_uint32)]),
+    "gsnp_adjoint_export": (ctypes.c_int, [_P, _P, _P, _P]),
+    "gsnp_adjoint_priors": (ctypes.c_int, [_P, ctypes.c_double, _P, _P, ctypes.c_uint32]),
 }
 
 
@@ -192,13 +199,24 @@ def synth(config: int, seed: int, scale: float, prefix) -> tuple[int, int]:
     return int(n), rl.value
 
 
+def matrix_gen(fasta, soap, dbsnp=None, **kw):
+    """Counting-trained calibration matrix (gsnp_matrix_gen), 256*256*16 doubles
+    indexed q<<12 | cycle<<4 | ref<<2 | obs, as a numpy array."""
+    import numpy as np
+
+    pm = np.zeros(256 * 256 * 16, dtype=np.float64)
+    p = params(**kw)
+    check(lib().gsnp_matrix_gen(_b(fasta), _b(soap), _b(dbsnp), ctypes.byref(p), pm.ctypes.data))
+    return pm
+
+
 def format_g(v: float) -> str:
     buf = ctypes.create_string_buffer(64)
     n = lib().gsnp_format_g(v, buf)
     return buf.raw[:n].decode()
 
 
-STAGES = ["site_build", "sort", "likelihood", "posterior", "rank", "format", "recycle", "fused"]
+STAGES = ["site_build", "sort", "likelihood", "posterior", "rank", "format", "recycle", "fused", "adjoint"]
 
 
 class Job:

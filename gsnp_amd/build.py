@@ -21,7 +21,8 @@ OUT = PKG / "_build"
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("GSNP_OFFLOAD_ARCH", "gfx950")
 
-LIB_SOURCES = ["tables.cpp", "inputs.cpp", "cwriter.cpp", "synth.cpp", "kernels.hip", "dict.hip", "engine.hip", "run.hip"]
+LIB_SOURCES = ["tables.cpp", "inputs.cpp", "cwriter.cpp", "synth.cpp", "kernels.hip", "dict.hip", "engine.hip", "run.hip",
+               "adjoint.hip"]
 TOOLS = {"gsnp": "cli_main.cpp", "gsnp-decompress": "decompress_main.cpp"}
 
 # -ffp-contract=off everywhere: FP64 sums and comparisons must round exactly as
@@ -39,11 +40,11 @@ def _hash(paths: list[Path], flags: list[str]) -> str:
     return h.hexdigest()[:16]
 
 
-def _compile(src: Path, obj: Path, flags: list[str]) -> None:
+def _compile(src: Path, obj: Path, flags: list[str], force: bool = False) -> None:
     headers = sorted(CSRC.glob("*.h")) + [ROOT / "include" / "gsnp_abi.h"]
     digest = _hash([src] + headers, flags)
     stamp = obj.with_suffix(".hash")
-    if obj.exists() and stamp.exists() and stamp.read_text() == digest:
+    if not force and obj.exists() and stamp.exists() and stamp.read_text() == digest:
         return
     cmd = [HIPCC] + flags + ["-c", str(src), "-o", str(obj)]
     res = subprocess.run(cmd, capture_output=True, text=True)
@@ -52,7 +53,8 @@ def _compile(src: Path, obj: Path, flags: list[str]) -> None:
     stamp.write_text(digest)
 
 
-def build(verbose: bool = True) -> Path:
+def build(verbose: bool = True, force: bool = False) -> Path:
+    """Compiles the library and tools; unchanged objects are reused unless force."""
     OUT.mkdir(exist_ok=True)
     jobs = []
     for name in LIB_SOURCES:
@@ -63,7 +65,7 @@ def build(verbose: bool = True) -> Path:
         src = CSRC / name
         jobs.append((src, OUT / (src.stem + ".o"), COMMON))
     with ThreadPoolExecutor(max_workers=min(8, len(jobs))) as ex:
-        list(ex.map(lambda j: _compile(*j), jobs))
+        list(ex.map(lambda j: _compile(*j, force=force), jobs))
     lib = OUT / "libgsnp.so"
     objs = [str(OUT / (Path(n).stem + ".o")) for n in LIB_SOURCES]
     cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-o", str(lib)] + objs
@@ -84,7 +86,7 @@ def build(verbose: bool = True) -> Path:
 
 if __name__ == "__main__":
     try:
-        build()
+        build(force="--force" in sys.argv)
     except RuntimeError as e:
         print(e, file=sys.stderr)
         sys.exit(1)

@@ -64,6 +64,7 @@ enum Stage : int {
     kStFormat,
     kStRecycle,
     kStFused,
+    kStAdjoint,
     kNumStages
 };
 
@@ -131,6 +132,17 @@ struct Ctx {
     DevBuf<uint32_t> err;
     DevBuf<unsigned long long> stat;  // [0] records called
     DevBuf<int4> head_qs;  // q_sum of window-boundary sites before their own reads (batched span)
+    // known SNPs of the chromosome (host copies, for the adjoint's prior rows)
+    std::vector<gsnp_snp> h_snps;
+    std::vector<uint8_t> h_snp_ref;
+    // adjoint: records of a prepared span in site order, their score-row keys,
+    // their place in key order and the reduction pieces
+    uint32_t adj_start = 0, adj_n = 0, adj_npieces = 0;
+    uint64_t adj_nrec = 0;
+    DevBuf<uint32_t> adj_cnt, adj_prow, adj_key, adj_skey, adj_perm, adj_rank, adj_tmp, adj_pstart, adj_kpiece;
+    DevBuf<uint32_t> adj_hist, adj_mask;
+    DevBuf<uint64_t> adj_off, adj_pidx, adj_hoff, adj_bsum;
+    DevBuf<double> adj_T, adj_w, adj_ll, adj_dh, adj_c, adj_psum, adj_part, adj_out, adj_pm, adj_grad, adj_lp;
     // device dictionary encoder scratch
     DevBuf<int32_t> dict_mm, dict_vals;
     DevBuf<uint32_t> dict_bits, dict_wpre, dict_out;

@@ -19,6 +19,8 @@
 
 namespace gsnp {
 
+static_assert(kScoreStride == kScoreRowStride, "host and device score-row strides differ");
+
 thread_local std::string g_err;
 
 int fail(int code, const std::string& msg)
@@ -362,7 +364,11 @@ int gsnp_set_chromosome(gsnp_ctx* c, const char* name, const uint8_t* codes, uin
     HIPCHK(c->ref.alloc((size_t)len + GSNP_WIN + 512));
     HIPCHK(hipMemsetAsync(c->ref.p, 4, (size_t)len + GSNP_WIN + 512, c->stream));  // beyond the end reads as N
     if (len) HIPCHK(hipMemcpyAsync(c->ref.p, codes, len, hipMemcpyHostToDevice, c->stream));
+    if (n_snp && (!snp_pos || !snps)) return fail(GSNP_E_ARG, "null known-SNP arrays");
     c->n_snp = n_snp;
+    c->h_snps.assign(snps, snps + n_snp);
+    c->h_snp_ref.resize(n_snp);
+    for (uint32_t i = 0; i < n_snp; i++) c->h_snp_ref[i] = snp_pos[i] < len ? codes[snp_pos[i]] : 0;
     if (n_snp) {
         std::vector<double> lp((size_t)n_snp * 10);
         for (uint32_t i = 0; i < n_snp; i++) {

@@ -15,6 +15,7 @@ constexpr uint32_t kWin = 262144;     // sites per window (32*8192)
 constexpr uint32_t kSlots = 256;      // record slots per site (MAX_NUM_NON_ZERO)
 constexpr uint32_t kMaxDepUni = 255;  // dep_uni cap (kernel/update_kernel.cu:288)
 constexpr uint32_t kScoreRows = 256 * 256 * 4;  // (q_adj, cycle, o_base)
+constexpr uint32_t kScoreRowStride = 16;  // doubles per score row: 10 used, padded to one 128-byte line
 constexpr uint32_t kPrank = 64 * 64 * 2048;
 
 // Observation record (kernel/likelihood_kernel.cu:24-42):

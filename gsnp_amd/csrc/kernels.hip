@@ -7,6 +7,7 @@
 
 #include "gsnp_common.h"
 #include "kernels.h"
+#include "walk.h"
 
 namespace gsnp {
 
@@ -426,70 +427,31 @@ void launch_sort(hipStream_t st, uint32_t* rec, const int32_t* nrec, const uint3
 
 // ---------------------------------------------------------------------------
 // (c) likelihood.  One thread per site over its sorted records
-// (kernel/likelihood_kernel.cu:168-256).  Within a base segment, t is the
-// occurrence count of the record's (strand, cycle) and g the number of
-// distinct (strand, cycle) seen so far minus one; the adjusted quality comes
-// from the host-built table and the 10 genotype scores (80 contiguous bytes
-// of the score table) are added in record order.  A per-thread 512-bit LDS
-// set marks the (strand, cycle) keys seen in the segment; only a repeated key
-// rescans the segment to count its earlier occurrences.
+// (kernel/likelihood_kernel.cu:168-256): walk_adjusted gives each record's
+// dependency-adjusted quality in record order and the 10 genotype scores of
+// its row (one 128-byte line of the score table) are added in that order.
 __global__ __launch_bounds__(256) void k_likelihood(Tables t, SiteArrays a, uint32_t n_sites, double* lk,
                                                     uint32_t* err)
 {
     __shared__ uint32_t seen_all[256 * 17];
     uint32_t* seen = seen_all + threadIdx.x * 17;
     const uint32_t s = blockIdx.x * 256u + threadIdx.x;
-    const uint32_t n = s < n_sites ? (uint32_t)a.nrec[s] : 0u;
     if (s >= n_sites) return;
+    const uint32_t n = (uint32_t)a.nrec[s];
     const uint32_t* row = a.rec + (size_t)s * kSlots;
     double acc[10];
 #pragma unroll
     for (int g = 0; g < 10; g++) acc[g] = 0.0;
-    int last_o = -1, gdc = -1;
-    uint32_t seg = 0;
-    uint32_t bad = 0;
-    for (uint32_t i = 0; i < n; i++) {
-        const uint32_t w = row[i];
-        const uint32_t o = rec_base(w);
-        uint32_t q = rec_q(w);
-        const uint32_t cyc = rec_cycle(w), occ = rec_occ(w), key = rec_key(w);
-        if ((int)o != last_o) {
+    const uint32_t bad = walk_adjusted(row, n, t, seen, [&](uint32_t qa, uint32_t cyc, uint32_t o) {
+        const double2* sr =
+            reinterpret_cast<const double2*>(t.score + (size_t)((qa << 10) | (cyc << 2) | o) * kScoreRowStride);
 #pragma unroll
-            for (int k = 0; k < 16; k++) seen[k] = 0;
-            gdc = -1;
-            seg = i;
-            last_o = (int)o;
+        for (int h = 0; h < 5; h++) {
+            const double2 v = sr[h];
+            acc[2 * h] += v.x;
+            acc[2 * h + 1] += v.y;
         }
-        const uint32_t word = seen[key >> 5], bit = 1u << (key & 31);
-        uint32_t prior = 0;
-        if (word & bit) {
-            for (uint32_t j = seg; j < i; j++)
-                if (rec_key(row[j]) == key) prior += rec_occ(row[j]);
-        } else {
-            seen[key >> 5] = word | bit;
-        }
-        if (q > 63) {
-            bad |= kErrQadjRange;
-            q = 63;
-        }
-        for (uint32_t k = 0; k < occ; k++) {
-            if (prior + k == 0) gdc += 1;
-            const uint32_t tm1 = prior + k < (uint32_t)t.td ? prior + k : (uint32_t)t.td - 1;
-            const uint32_t gi = (uint32_t)gdc < (uint32_t)t.gd ? (uint32_t)gdc : (uint32_t)t.gd - 1;
-            uint32_t qa = t.qadj[((size_t)q * t.td + tm1) * t.gd + gi];
-            if (qa == 0) {
-                bad |= kErrQadjRange;
-                qa = 1;
-            }
-            const double2* sr = reinterpret_cast<const double2*>(t.score + (size_t)((qa << 10) | (cyc << 2) | o) * 10);
-#pragma unroll
-            for (int h = 0; h < 5; h++) {
-                const double2 v = sr[h];
-                acc[2 * h] += v.x;
-                acc[2 * h + 1] += v.y;
-            }
-        }
-    }
+    });
     double2* out = reinterpret_cast<double2*>(lk + (size_t)s * 10);
 #pragma unroll
     for (int h = 0; h < 5; h++) out[h] = make_double2(acc[2 * h], acc[2 * h + 1]);
@@ -697,6 +659,14 @@ constexpr uint32_t fused_group()
     return S <= 64 ? 64u : 4096u / S;
 }
 
+// LDS words per site row: odd (conflict-free row walks) and at least 20 (the
+// row later holds the site's ten FP64 sums)
+template <int S>
+constexpr uint32_t fused_row_stride()
+{
+    return S + 1 < 21 ? 21u : (uint32_t)S + 1u;
+}
+
 __device__ __forceinline__ void wave_lds_sync()
 {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -708,7 +678,7 @@ template <int S>
 __global__ __launch_bounds__(256) void k_fused(FusedArgs f, const uint32_t* list, const uint32_t* count_ptr)
 {
     constexpr uint32_t G = fused_group<S>();
-    constexpr uint32_t RS = S + 1;  // LDS row stride of a sorted site
+    constexpr uint32_t RS = fused_row_stride<S>();
     extern __shared__ uint32_t lds[];
     const uint32_t lane = lane_id();
     const uint32_t wib = threadIdx.x >> 6;
@@ -832,49 +802,96 @@ __global__ __launch_bounds__(256) void k_fused(FusedArgs f, const uint32_t* list
             const uint32_t qidx = (q * td + tm1) * gd + gi;
             row[i] = (qidx << 10) | (cyc << 2) | o;
         }
-        // phase 2b: adjusted qualities and score rows, summed in record order;
-        // four records per step so their loads are in flight together
-        double acc[10];
+        // phase 2b: adjusted qualities and score rows summed in record order.
+        // Five lanes share a site, lane h of the five loading and summing
+        // genotypes 2h, 2h+1 (16 bytes of the site's 128-byte score row), so a
+        // load instruction touches one line per record instead of one per
+        // lane.  The G*5 (site, h) slots are spread over the wave, five per
+        // lane, and two records per slot are in flight per step.
+        wave_lds_sync();
+        constexpr uint32_t NSLOT = (G * 5 + 63) / 64;
+        uint32_t sl_row[NSLOT], sl_n[NSLOT], sl_h[NSLOT];
+        double s0[NSLOT], s1[NSLOT];
+        uint32_t maxn = 0;
 #pragma unroll
-        for (int g = 0; g < 10; g++) acc[g] = 0.0;
-        for (uint32_t i = 0; i < n; i += 4) {
-            uint32_t x[4], qa[4];
+        for (uint32_t k = 0; k < NSLOT; k++) {
+            const uint32_t v = lane + 64u * k;
+            const uint32_t si = v / 5u;
+            const uint32_t nn = (uint32_t)__shfl((int)n, (int)(si < 64u ? si : 0u));
+            const bool ok = si < gcount;
+            sl_row[k] = (ok ? si : 0u) * RS;
+            sl_n[k] = ok ? nn : 0u;
+            sl_h[k] = v % 5u;
+            s0[k] = 0.0;
+            s1[k] = 0.0;
+            maxn = sl_n[k] > maxn ? sl_n[k] : maxn;
+        }
+        for (uint32_t i = 0; i < maxn; i += 2) {
+            uint32_t x[NSLOT][2];
 #pragma unroll
-            for (int j = 0; j < 4; j++) x[j] = i + j < n ? row[i + j] : 0u;
+            for (uint32_t k = 0; k < NSLOT; k++)
 #pragma unroll
-            for (int j = 0; j < 4; j++) qa[j] = f.t.qadj[x[j] >> 10];
-            double2 v[4][5];
+                for (uint32_t u = 0; u < 2; u++) x[k][u] = i + u < sl_n[k] ? srt[sl_row[k] + i + u] : 0u;
+            uint32_t qa[NSLOT][2];
 #pragma unroll
-            for (int j = 0; j < 4; j++) {
-                uint32_t a = qa[j];
-                if (i + j < n && a == 0) {
-                    bad |= kErrQadjRange;
-                    a = 1;
-                }
-                const double2* sr =
-                    reinterpret_cast<const double2*>(f.t.score + (size_t)((a << 10) | (x[j] & 1023u)) * 10);
+            for (uint32_t k = 0; k < NSLOT; k++)
 #pragma unroll
-                for (int h = 0; h < 5; h++) v[j][h] = sr[h];
-            }
+                for (uint32_t u = 0; u < 2; u++) qa[k][u] = i + u < sl_n[k] ? f.t.qadj[x[k][u] >> 10] : 1u;
+            double2 val[NSLOT][2];
 #pragma unroll
-            for (int j = 0; j < 4; j++) {
-                if (i + j < n) {
+            for (uint32_t k = 0; k < NSLOT; k++)
 #pragma unroll
-                    for (int h = 0; h < 5; h++) {
-                        acc[2 * h] += v[j][h].x;
-                        acc[2 * h + 1] += v[j][h].y;
+                for (uint32_t u = 0; u < 2; u++) {
+                    uint32_t a = qa[k][u];
+                    if (a == 0) {
+                        bad |= kErrQadjRange;
+                        a = 1;
                     }
+                    val[k][u] = i + u < sl_n[k]
+                                    ? *reinterpret_cast<const double2*>(
+                                          f.t.score + (size_t)((a << 10) | (x[k][u] & 1023u)) * kScoreRowStride +
+                                          2u * sl_h[k])
+                                    : make_double2(0.0, 0.0);
                 }
-            }
+#pragma unroll
+            for (uint32_t k = 0; k < NSLOT; k++)
+#pragma unroll
+                for (uint32_t u = 0; u < 2; u++)
+                    if (i + u < sl_n[k]) {
+                        s0[k] += val[k][u].x;
+                        s1[k] += val[k][u].y;
+                    }
         }
         if (bad) atomicOr(f.err, bad);
+        // the row area of each site now takes its ten sums (20 words)
+        wave_lds_sync();
+#pragma unroll
+        for (uint32_t k = 0; k < NSLOT; k++) {
+            const uint32_t v = lane + 64u * k;
+            if (v / 5u < gcount) {
+                uint32_t* d = srt + (v / 5u) * RS + 4u * sl_h[k];
+                const unsigned long long b0 = (unsigned long long)__double_as_longlong(s0[k]);
+                const unsigned long long b1 = (unsigned long long)__double_as_longlong(s1[k]);
+                d[0] = (uint32_t)b0;
+                d[1] = (uint32_t)(b0 >> 32);
+                d[2] = (uint32_t)b1;
+                d[3] = (uint32_t)(b1 >> 32);
+            }
+        }
+        wave_lds_sync();
         if (!act) continue;
+        double acc[10];
+#pragma unroll
+        for (int g = 0; g < 10; g++) {
+            const uint32_t* d = srt + lane * RS + 2u * (uint32_t)g;
+            acc[g] = __longlong_as_double((long long)((unsigned long long)d[0] | ((unsigned long long)d[1] << 32)));
+        }
         if (f.write_lk) {
             double2* out = reinterpret_cast<double2*>(f.out.lk + (size_t)site * 10);
 #pragma unroll
             for (int h = 0; h < 5; h++) out[h] = make_double2(acc[2 * h], acc[2 * h + 1]);
         }
-        // phase 3: posterior from the sums in registers
+        // phase 3: posterior from the sums
         posterior_site(f.t, f.ref, f.ws, f.a, site, [&](int g) { return acc[g]; }, f.head_qs, f.out, f.jobs,
                        f.job_count);
     }
@@ -885,7 +902,7 @@ static void launch_fused_class(hipStream_t st, const FusedArgs& f, const uint32_
                                uint32_t max_sites)
 {
     constexpr uint32_t G = fused_group<S>();
-    const uint32_t per_wave = (G * (S + 1) + 64u * f.sw) * 4u;
+    const uint32_t per_wave = (G * fused_row_stride<S>() + 64u * f.sw) * 4u;
     const uint32_t wpb = 4u * per_wave <= 65536u ? 4u : 2u * per_wave <= 65536u ? 2u : 1u;
     const uint32_t groups = (max_sites + G - 1) / G;
     uint32_t blocks = (groups + wpb - 1) / wpb;

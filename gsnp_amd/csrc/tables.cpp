@@ -11,12 +11,12 @@ namespace gsnp {
 
 std::vector<double> build_score_table(const double* pm)
 {
-    std::vector<double> s((size_t)256 * 256 * 4 * 10);
+    std::vector<double> s((size_t)256 * 256 * 4 * kScoreStride, 0.0);
     for (uint32_t q = 0; q < 256; q++) {
         for (uint32_t c = 0; c < 256; c++) {
             const double* row = pm + ((size_t)q << 12) + ((size_t)c << 4);
             for (uint32_t o = 0; o < 4; o++) {
-                double* out = &s[((size_t)((q << 10) | (c << 2) | o)) * 10];
+                double* out = &s[((size_t)((q << 10) | (c << 2) | o)) * kScoreStride];
                 int g = 0;
                 for (uint32_t a1 = 0; a1 < 4; a1++)
                     for (uint32_t a2 = a1; a2 < 4; a2++) {

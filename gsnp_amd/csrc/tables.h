@@ -10,7 +10,9 @@
 namespace gsnp {
 
 // log10(0.5 p[q,c,a1,o] + 0.5 p[q,c,a2,o]) for every (q_adj, cycle, o) row and
-// the 10 genotypes (kernel/likelihood_kernel.cu:137-161).  256*256*4*10 doubles.
+// the 10 genotypes (kernel/likelihood_kernel.cu:137-161).  256*256*4 rows of
+// kScoreStride doubles (10 used), so that every row is one 128-byte line.
+constexpr uint32_t kScoreStride = 16;
 std::vector<double> build_score_table(const double* p_matrix);
 
 // Adjusted quality max(1, (int)(10^(log10 q + (t-1) pcr + g glob) + 0.5))

@@ -186,9 +186,9 @@ int gsnp_sort_likelihood(gsnp_ctx* ctx, uint32_t* records, uint32_t pitch, const
 /* Per-stage device time from HIP events recorded on the context's stream:
  * returns (and resets) the accumulated milliseconds and launch counts of the
  * stages site build, sort, likelihood, posterior, rank-sum, format, recycle,
- * fused (8 entries), then turns recording on or off.  Replaces the cutil
+ * fused, adjoint (9 entries), then turns recording on or off.  Replaces the cutil
  * timers of the reference (kernel/cuda_header.h:37-50). */
-#define GSNP_N_STAGES 8
+#define GSNP_N_STAGES 9
 int gsnp_profile(gsnp_ctx* ctx, int enable, double* stage_ms, uint64_t* stage_count, int n_stages);
 
 /* Observation records the likelihood stage has consumed since the last
@@ -214,6 +214,33 @@ int gsnp_job_select(gsnp_job* job, int index);
 int gsnp_job_call(gsnp_job* job, int batched, int emit_text, uint64_t* nbytes);
 gsnp_ctx* gsnp_job_context(gsnp_job* job);
 void gsnp_job_close(gsnp_job* job);
+
+/* Adjoint of the calling likelihood (no reference counterpart; the reference
+ * computes no gradient).  For the sites [start, start+n_sites) of the selected
+ * chromosome, with lk_s(g) the site's genotype log10-likelihoods and prior_s
+ * the calling priors (cpp/prior.cc), the objective is
+ *   l = sum over called sites of ln sum_g prior_s(g) 10^lk_s(g).
+ * gsnp_adjoint_prepare builds the span's records, their dependency-adjusted
+ * qualities and the key order used for the deterministic reduction (once per
+ * span).  gsnp_adjoint_eval takes a p_matrix (256*256*16 doubles, layout
+ * q<<12|cycle<<4|ref<<2|obs) and ln het_rate, and returns l, dl/d ln p for
+ * q < 64 (64*256*16 doubles, same layout; may be NULL), dl/d ln het and
+ * optionally the per-site terms (n_sites doubles).  The score rows are
+ * evaluated on the device from p_matrix; sums are in a fixed order, so
+ * repeated calls return identical bits. */
+int gsnp_adjoint_prepare(gsnp_ctx* ctx, uint32_t start, uint32_t n_sites);
+int gsnp_adjoint_eval(gsnp_ctx* ctx, const double* p_matrix, double log_het, double* loglik, double* grad_log_p,
+                      double* grad_log_het, double* site_loglik);
+int gsnp_adjoint_info(gsnp_ctx* ctx, uint32_t* n_sites, uint64_t* n_records, uint32_t* n_prior_rows);
+/* The prepared span's inputs to the objective: keys[r] = q<<10|cycle<<2|o of
+ * each record (site order), offsets[n_sites+1] of each site's records, and
+ * prior_row[s] (0..7 reference-code rows, 8+i known SNP i, 0xFFFFFFFF for an
+ * uncalled N site).  Any pointer may be NULL. */
+int gsnp_adjoint_export(gsnp_ctx* ctx, uint32_t* keys, uint64_t* offsets, uint32_t* prior_row);
+/* log10 prior rows at ln het = log_het (10 per row, genotype order AA AC AT
+ * AG CC CT CG TT TG GG) and per row a 10-bit mask of the genotypes whose
+ * prior is proportional to het. */
+int gsnp_adjoint_priors(gsnp_ctx* ctx, double log_het, double* lp10, uint32_t* het_mask, uint32_t cap_rows);
 
 /* %#g text of one double as written in .cns rows (host build of the device
  * formatter). Returns the length. */
