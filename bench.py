@@ -41,6 +41,11 @@ def main() -> int:
     ap.add_argument("--compare", action="store_true", help="also time the other mode")
     ap.add_argument("--cpu-sites", type=int, default=2 * 262144, help="sites in the CPU oracle sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="keep the default fused launch shape")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint leg (2 Mb miscalibrated workload)")
+    ap.add_argument("--adjoint-evals", type=int, default=10)
+    ap.add_argument("--fd-entries", type=int, default=256)
+    ap.add_argument("--refit-steps", type=int, default=50)
     ap.add_argument("--workdir", default=None)
     args = ap.parse_args()
 
@@ -73,6 +78,11 @@ def main() -> int:
           f"{time.time() - t0:.1f}s", file=sys.stderr, flush=True)
 
     per = args.per or int(os.environ.get("GSNP_BATCH_WINDOWS", "0")) or (chr_len // gsnp_amd.WIN + 1)
+    tuned = None
+    if args.mode == "batched" and not args.no_tune:
+        t0 = time.time()
+        tuned = job.autotune(0, min(chr_len, per * gsnp_amd.WIN))
+        print(f"[bench] autotuned fused launch in {time.time() - t0:.1f}s: {tuned}", file=sys.stderr, flush=True)
 
     def timed(mode: str):
         batched = per if mode == "batched" else 0
@@ -147,6 +157,8 @@ def main() -> int:
                "sample": f"first {st['sites']} sites ({st['windows']} windows) of the same workload, "
                          f"single-threaded CPU oracle, call stage only"}
 
+    adjoint = None if args.no_adjoint else adjoint_leg(args, work)
+
     line = {
         "metric": "called sites/sec at bit-exact .cns",
         "value": round(sites_per_s, 1),
@@ -163,7 +175,8 @@ def main() -> int:
         "config": {"workload": "10 Mb reference, 30x 100 bp pairs, 3% 2-hit, 1% known SNPs, refine on"
                    if args.config == 2 else f"config {args.config} x{args.scale}",
                    "sites": chr_len, "reads": chr_reads, "records": recs, "mode": args.mode,
-                   "windows_per_span": per if args.mode == "batched" else 1, "text_bytes": text_bytes},
+                   "windows_per_span": per if args.mode == "batched" else 1, "text_bytes": text_bytes,
+                   "autotuned": tuned},
         "roofline": roofline,
         "call_kernels_gbs": round(call_gbs, 2) if call_gbs else None,
         "stage_ms_per_step": {k: round(v, 4) for k, v in stage_ms.items()},
@@ -171,9 +184,77 @@ def main() -> int:
     }
     if other:
         line["other_mode"] = other
+    if adjoint:
+        line["adjoint"] = adjoint
     print(json.dumps(line), flush=True)
     job.close()
     return 0
+
+
+def adjoint_leg(args, work: Path) -> dict:
+    """Gradient evaluations per second of the adjoint on the 2 Mb miscalibrated
+    workload (30x, 100 bp), its central-difference check on random table
+    entries, and the refit judged by the plain calling path."""
+    import math
+
+    import numpy as np
+
+    import gsnp_amd
+    from gsnp_amd.adjoint import Adjoint, calls_changed, central_differences, refit, write_matrix
+
+    pre = work / "c4"
+    t0 = time.time()
+    _, rl = gsnp_amd.synth(4, args.seed, 1.0, pre)
+    fa, so = f"{pre}.fa", f"{pre}.soap"
+    p0 = gsnp_amd.matrix_gen(fa, so, read_length=rl)
+    job = gsnp_amd.Job(fa, so, read_length=rl)
+    try:
+        adj = Adjoint(job)
+        lh = math.log(job._p.het_novel_r)
+        print(f"[bench] adjoint workload ready ({adj.n_sites} sites, {adj.n_records} records) in "
+              f"{time.time() - t0:.1f}s", file=sys.stderr, flush=True)
+        for _ in range(3):
+            adj.eval(p0, lh)
+        job.profile(enable=True)
+        t_start = time.perf_counter()
+        for _ in range(args.adjoint_evals):
+            ll, grad, dh = adj.eval(p0, lh)
+        t_eval = (time.perf_counter() - t_start) / args.adjoint_evals
+        prof = job.profile(enable=False)
+        k_ms = prof["adjoint"][0] / max(1, prof["adjoint"][1])
+        # central differences (h = 1e-6) on random entries that records use
+        flat = grad.reshape(-1)
+        nz = np.flatnonzero(flat != 0)
+        rng = np.random.default_rng(args.seed)
+        pick = rng.choice(nz, size=min(args.fd_entries, nz.size), replace=False)
+        fd, fd_het = central_differences(adj, p0, lh, pick, h=1e-6)
+        rel = np.abs(fd - flat[pick]) / np.maximum(np.abs(flat[pick]), 1e-300)
+        print(f"[bench] adjoint: {1 / t_eval:.1f} evals/s, finite differences on {len(pick)} entries done",
+              file=sys.stderr, flush=True)
+        fit = None
+        if args.refit_steps > 0:
+            p1, trace = refit(adj, p0, lh, steps=args.refit_steps)
+            m0, m1 = work / "c4.p0.matrix", work / "c4.p1.matrix"
+            write_matrix(m0, p0, rl)
+            write_matrix(m1, p1, rl)
+            c0, c1 = work / "c4.plain0.cns", work / "c4.plain1.cns"
+            gsnp_amd.run(fa, so, c0, matrix_in=m0, read_length=rl)
+            gsnp_amd.run(fa, so, c1, matrix_in=m1, read_length=rl)
+            fit = {"steps": args.refit_steps, "objective_start": trace[0], "objective_end": trace[-1],
+                   "calls_changed_vs_plain": calls_changed(c0, c1)}
+        recs, sites = adj.n_records, adj.n_sites
+    finally:
+        job.close()
+    # algorithmic bytes per evaluation: record keys read twice, four keyed FP64
+    # contributions written and read back plus a rank, 80 B per site, 2 MB gradient
+    algo = 2 * 4 * recs + 48 * recs + 80 * sites + 64 * 256 * 16 * 8
+    return {"metric": "adjoint gradient evaluations/sec", "value": round(1 / t_eval, 2), "unit": "evals/s",
+            "ms_per_eval": round(t_eval * 1e3, 3), "kernel_ms_per_eval": round(k_ms, 3),
+            "workload": "2 Mb reference, 30x 100 bp, miscalibrated qualities (delta 3 above q30, cycle offset to 2)",
+            "sites": sites, "records": recs, "loglik": ll, "dloglik_dlog_het": dh,
+            "hbm_gbs_kernels": round(algo / (k_ms / 1e3) / 1e9, 2) if k_ms else None,
+            "fd_entries": int(len(pick)), "fd_h": 1e-6, "fd_max_rel_err": float(rel.max()),
+            "fd_het_rel_err": abs(fd_het - dh) / abs(dh) if dh else None, "refit": fit}
 
 
 if __name__ == "__main__":

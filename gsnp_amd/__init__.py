@@ -130,6 +130,7 @@ SIGNATURES = {
     "gsnp_job_close": (None, [_P]),
     "gsnp_synth_write": (ctypes.c_longlong, [ctypes.c_int, ctypes.c_ulonglong, ctypes.c_double, ctypes.c_char_p,
                                              ctypes.POINTER(ctypes.c_int)]),
+    "gsnp_autotune": (ctypes.c_int, [_P, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, _P]),
     "gsnp_adjoint_prepare": (ctypes.c_int, [_P, ctypes.c_uint32, ctypes.c_uint32]),
     "gsnp_adjoint_eval": (ctypes.c_int, [_P, _P, ctypes.c_double, ctypes.POINTER(ctypes.c_double), _P,
                                          ctypes.POINTER(ctypes.c_double), _P]),
@@ -243,6 +244,7 @@ class Job:
 
     def select(self, i: int) -> None:
         check(lib().gsnp_job_select(self.h, i))
+        self._length = self.info(i)[1]
 
     def call(self, batched: int = 0, emit_text: bool = True) -> int:
         nb = ctypes.c_uint64()
@@ -258,6 +260,15 @@ class Job:
         cnt = (ctypes.c_uint64 * len(STAGES))()
         check(lib().gsnp_profile(self.ctx, int(enable), ms, cnt, len(STAGES)))
         return {s: (ms[i], cnt[i]) for i, s in enumerate(STAGES)}
+
+    def autotune(self, start: int = 0, n_sites: int | None = None, reps: int = 2) -> dict:
+        """Tune the fused launch of the batched form on the selected chromosome
+        (gsnp_autotune); returns the chosen shape."""
+        if n_sites is None:
+            n_sites = self._length - start
+        cfg = (ctypes.c_uint32 * 13)()
+        check(lib().gsnp_autotune(self.ctx, start, n_sites, reps, cfg))
+        return {"smallest_network": 8 << cfg[0], "waves_per_block": list(cfg[1:7]), "grid_cap": list(cfg[7:13])}
 
     def records(self, reset: bool = True) -> int:
         v = ctypes.c_uint64()

@@ -127,6 +127,7 @@ struct Ctx {
     DevBuf<uint64_t> blocksum, total;
     DevBuf<uint64_t> ranges;  // read interval per 256-site block (site build)
     bool keep_lk = false;     // batched form: also store the per-site log-likelihoods
+    FusedTune tune;           // launch shape of the fused stage (gsnp_autotune)
     DevBuf<char> text;
     uint64_t text_cap = 0, text_len = 0;
     DevBuf<uint32_t> err;
@@ -157,6 +158,8 @@ struct Ctx {
     int host_fixups();
     int check_err();
     int format_text(uint32_t offset, uint32_t ws_abs, uint32_t n_sites, uint32_t pos_base, uint64_t* nbytes);
+    int span_build(uint32_t start, uint32_t n);
+    FusedArgs fused_args(uint32_t start);
 };
 
 }  // namespace gsnp

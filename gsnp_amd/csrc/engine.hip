@@ -587,38 +587,16 @@ int gsnp_window_recycle(gsnp_ctx* c)
 int gsnp_batched_call(gsnp_ctx* c, uint32_t start, uint32_t n, int emit_text, uint64_t* nbytes)
 {
     if (!c) return fail(GSNP_E_ARG, "null context");
-    if ((uint64_t)start + n > c->chr_len) return fail(GSNP_E_ARG, "span outside the chromosome");
-    if (start % GSNP_WIN) return fail(GSNP_E_ARG, "span must start on a window boundary");
-    int rc = c->alloc_window(n + 2);
+    int rc = c->span_build(start, n);
     if (rc) return rc;
-    const uint32_t nheads = n / GSNP_WIN + 2;
-    if (c->head_qs.n < nheads) HIPCHK(c->head_qs.alloc(nheads));
-    HIPCHK(hipMemsetAsync(c->head_qs.p, 0, sizeof(int4) * nheads, c->stream));
     SiteArrays a = c->site_view(0);
     Results res = c->result_view(0);
-    // every read that covers [start, start+n], no window batches and no carry
-    c->timer.begin(kStSiteBuild, c->stream);
-    launch_site_build(c->stream, c->read_view(), 0, c->n_reads, start, n + 1, INT64_MIN, (int64_t)start + n, 0, a,
-                      c->head_qs.p, c->ranges.p);
-    c->timer.end(kStSiteBuild, c->stream);
     HIPCHK(hipMemsetAsync(c->job_count.p, 0, 4, c->stream));
     HIPCHK(hipMemsetAsync(c->fix_count.p, 0, 4, c->stream));
     // every site into a depth class, then one fused sort+likelihood+posterior launch per class
     c->timer.begin(kStFused, c->stream);
-    launch_bucket(c->stream, a.nrec, n, c->lists.p, c->win_cap, c->counts.p, 1, c->stat.p);
-    FusedArgs f;
-    f.t = c->tables();
-    f.ref = c->ref_view();
-    f.ws = start;
-    f.a = a;
-    f.head_qs = c->head_qs.p;
-    f.out = res;
-    f.jobs = c->jobs.p;
-    f.job_count = c->job_count.p;
-    f.err = c->err.p;
-    f.write_lk = c->keep_lk ? 1 : 0;
-    f.sw = ((uint32_t)(c->p.read_length + 7) / 8) | 1u;
-    launch_fused(c->stream, f, c->lists.p, c->win_cap, c->counts.p, n);
+    launch_bucket(c->stream, a.nrec, n, c->lists.p, c->win_cap, c->counts.p, 1, c->stat.p, (int)c->tune.min_class);
+    launch_fused(c->stream, c->fused_args(start), c->lists.p, c->win_cap, c->counts.p, n, c->tune);
     c->timer.end(kStFused, c->stream);
     if (c->p.rank_sum_mode) {
         c->timer.begin(kStRank, c->stream);
@@ -636,6 +614,71 @@ int gsnp_batched_call(gsnp_ctx* c, uint32_t start, uint32_t n, int emit_text, ui
     c->built = false;
     if (emit_text) return c->format_text(0, start, n, start + 1, nbytes);
     if (nbytes) *nbytes = 0;
+    return GSNP_OK;
+}
+
+int gsnp_autotune(gsnp_ctx* c, uint32_t start, uint32_t n, int reps, uint32_t* config)
+{
+    if (!c) return fail(GSNP_E_ARG, "null context");
+    if (reps < 1) reps = 1;
+    int rc = c->span_build(start, n);
+    if (rc) return rc;
+    SiteArrays a = c->site_view(0);
+    const FusedArgs f = c->fused_args(start);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    auto time_ms = [&](auto&& launch) -> float {
+        float best = 3.4e38f;
+        for (int r = 0; r < reps; r++) {
+            (void)hipMemsetAsync(c->job_count.p, 0, 4, c->stream);
+            (void)hipEventRecord(e0, c->stream);
+            launch();
+            (void)hipEventRecord(e1, c->stream);
+            float t = 0.f;
+            if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&t, e0, e1) != hipSuccess) return 3.4e38f;
+            best = t < best ? t : best;
+        }
+        return best;
+    };
+    static const uint32_t kWpb[3] = {1, 2, 4};
+    static const uint32_t kCaps[5] = {512, 1024, 2048, 4096, 8192};
+    FusedTune best_tune;
+    float best_total = 3.4e38f;
+    for (uint32_t mc = 0; mc <= 2; mc++) {  // smallest network: 8, 16 or 32 lanes
+        launch_bucket(c->stream, a.nrec, n, c->lists.p, c->win_cap, c->counts.p, 1, nullptr, (int)mc);
+        FusedTune t;
+        t.min_class = mc;
+        float total = 0.f;
+        for (uint32_t cls = mc; cls < 6; cls++) {
+            float bt = 3.4e38f;
+            for (uint32_t w : kWpb)
+                for (uint32_t cap : kCaps) {
+                    const float ms = time_ms(
+                        [&] { launch_fused_one(c->stream, f, c->lists.p, c->win_cap, c->counts.p, n, cls, w, cap); });
+                    if (ms < bt) {
+                        bt = ms;
+                        t.wpb[cls] = w;
+                        t.max_blocks[cls] = cap;
+                    }
+                }
+            total += bt;
+        }
+        if (total < best_total) {
+            best_total = total;
+            best_tune = t;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    HIPCHK(hipGetLastError());
+    rc = c->check_err();
+    if (rc) return rc;
+    c->tune = best_tune;
+    if (config) {
+        config[0] = best_tune.min_class;
+        for (int i = 0; i < 6; i++) config[1 + i] = best_tune.wpb[i], config[7 + i] = best_tune.max_blocks[i];
+    }
     return GSNP_OK;
 }
 
@@ -704,6 +747,43 @@ int gsnp_format_g(double v, char* out)
 }  // extern "C"
 
 namespace gsnp {
+
+// Site matrix of the span [start, start+n) straight from the device read
+// store: every read that covers it, no window batches and no carry, with the
+// q_sum of window-boundary sites before their own reads kept in head_qs.
+int Ctx::span_build(uint32_t start, uint32_t n)
+{
+    if ((uint64_t)start + n > chr_len) return fail(GSNP_E_ARG, "span outside the chromosome");
+    if (start % GSNP_WIN) return fail(GSNP_E_ARG, "span must start on a window boundary");
+    int rc = alloc_window(n + 2);
+    if (rc) return rc;
+    const uint32_t nheads = n / GSNP_WIN + 2;
+    if (head_qs.n < nheads) HIPCHK(head_qs.alloc(nheads));
+    HIPCHK(hipMemsetAsync(head_qs.p, 0, sizeof(int4) * nheads, stream));
+    timer.begin(kStSiteBuild, stream);
+    launch_site_build(stream, read_view(), 0, n_reads, start, n + 1, INT64_MIN, (int64_t)start + n, 0, site_view(0),
+                      head_qs.p, ranges.p);
+    timer.end(kStSiteBuild, stream);
+    HIPCHK(hipGetLastError());
+    return GSNP_OK;
+}
+
+FusedArgs Ctx::fused_args(uint32_t start)
+{
+    FusedArgs f;
+    f.t = tables();
+    f.ref = ref_view();
+    f.ws = start;
+    f.a = site_view(0);
+    f.head_qs = head_qs.p;
+    f.out = result_view(0);
+    f.jobs = jobs.p;
+    f.job_count = job_count.p;
+    f.err = err.p;
+    f.write_lk = keep_lk ? 1 : 0;
+    f.sw = ((uint32_t)(p.read_length + 7) / 8) | 1u;
+    return f;
+}
 
 int Ctx::format_text(uint32_t offset, uint32_t ws_abs, uint32_t n_sites, uint32_t pos_base, uint64_t* nbytes)
 {

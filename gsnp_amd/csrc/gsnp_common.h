@@ -207,6 +207,12 @@ GSNP_HD uint64_t round_scaled(uint64_t m, int e, int k, bool trunc)
 
 GSNP_HD int fmt_g(double v, char* o)
 {
+    // the two values most .cns rows carry (rank-sum 1, copy number 1 or 255)
+    if (v == 1.0 || v == 255.0) {
+        const char* s = v == 1.0 ? "1.00000" : "255.000";
+        for (int i = 0; i < 7; i++) o[i] = s[i];
+        return 7;
+    }
     int n = 0;
     uint64_t bits;
     __builtin_memcpy(&bits, &v, 8);

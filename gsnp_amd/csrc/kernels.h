@@ -97,19 +97,30 @@ struct FusedArgs {
     uint32_t sw = 1;   // words per lane of the dependency-count table, odd, >= ceil(L/8)
 };
 
+// Launch shape of the fused stage: bucket classes below min_class run on
+// min_class's network (the smallest bucket edge), and each class gets its
+// waves per block (0: the most that fit 64 KB of LDS) and a grid cap.
+struct FusedTune {
+    uint32_t min_class = 0;
+    uint32_t wpb[6] = {0, 0, 0, 0, 0, 0};
+    uint32_t max_blocks[6] = {8192, 8192, 8192, 8192, 8192, 8192};
+};
+
 // range: 2 u64 per 256-site block (read interval of the block)
 void launch_site_build(hipStream_t s, const ReadView& r, uint64_t r0, uint64_t r1, uint32_t ws, uint32_t n_sites,
                        int64_t lo_min, int64_t hi_max, uint32_t carry, const SiteArrays& a, int4* head_qs,
                        uint64_t* range);
 void launch_bucket(hipStream_t s, const int32_t* nrec, uint32_t n_sites, uint32_t* lists, uint32_t list_pitch,
-                   uint32_t* counts, int all, unsigned long long* n_records);
+                   uint32_t* counts, int all, unsigned long long* n_records, int min_class = 0);
 void launch_sort(hipStream_t s, uint32_t* rec, const int32_t* nrec, const uint32_t* lists, uint32_t list_pitch,
                  const uint32_t* counts, uint32_t max_sites);
 void launch_likelihood(hipStream_t s, const Tables& t, const SiteArrays& a, uint32_t n_sites, double* lk, uint32_t* err);
 void launch_posterior(hipStream_t s, const Tables& t, const RefView& ref, uint32_t ws, const SiteArrays& a,
                       uint32_t n_sites, const int4* head_qs, Results& out, RankJob* jobs, uint32_t* job_count);
 void launch_fused(hipStream_t s, const FusedArgs& f, const uint32_t* lists, uint32_t list_pitch, const uint32_t* counts,
-                  uint32_t max_sites);
+                  uint32_t max_sites, const FusedTune& tune);
+void launch_fused_one(hipStream_t s, const FusedArgs& f, const uint32_t* lists, uint32_t list_pitch,
+                      const uint32_t* counts, uint32_t max_sites, uint32_t cls, uint32_t wpb, uint32_t max_blocks);
 void launch_rank(hipStream_t s, const Tables& t, const SiteArrays& a, const RankJob* jobs, const uint32_t* job_count,
                  uint32_t max_jobs, Results& out, FixJob* fixes, uint32_t* fix_count);
 void launch_apply_fix(hipStream_t s, const FixJob* fixes, const double* rank, const int32_t* qcns, uint32_t n,

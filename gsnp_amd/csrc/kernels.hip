@@ -210,7 +210,7 @@ void launch_site_build(hipStream_t st, const ReadView& r, uint64_t r0, uint64_t 
 constexpr uint32_t kBucketPer = 16;
 
 __global__ __launch_bounds__(256) void k_bucket(const int32_t* nrec, uint32_t n_sites, uint32_t* lists,
-                                                uint32_t pitch, uint32_t* counts, int all,
+                                                uint32_t pitch, uint32_t* counts, int all, int min_class,
                                                 unsigned long long* n_records)
 {
     __shared__ uint32_t cnt[6], base[6];
@@ -231,6 +231,7 @@ __global__ __launch_bounds__(256) void k_bucket(const int32_t* nrec, uint32_t n_
             const int n = nrec[s];
             recs += (unsigned long long)n;
             b = n <= 1 ? (all ? 0 : -1) : n <= 8 ? 0 : n <= 16 ? 1 : n <= 32 ? 2 : n <= 64 ? 3 : n <= 128 ? 4 : 5;
+            if (b >= 0 && b < min_class) b = min_class;
         }
         cls[it] = b;
         loc[it] = 0;
@@ -261,13 +262,13 @@ __global__ __launch_bounds__(256) void k_bucket(const int32_t* nrec, uint32_t n_
 }
 
 void launch_bucket(hipStream_t st, const int32_t* nrec, uint32_t n_sites, uint32_t* lists, uint32_t pitch,
-                   uint32_t* counts, int all, unsigned long long* n_records)
+                   uint32_t* counts, int all, unsigned long long* n_records, int min_class)
 {
     hipMemsetAsync(counts, 0, 6 * sizeof(uint32_t), st);
     const uint32_t blocks = (n_sites + 256 * kBucketPer - 1) / (256 * kBucketPer);
     if (blocks)
         hipLaunchKernelGGL(k_bucket, dim3(blocks), dim3(256), 0, st, nrec, n_sites, lists, pitch, counts, all,
-                           n_records);
+                           min_class, n_records);
 }
 
 template <int J>
@@ -899,28 +900,43 @@ __global__ __launch_bounds__(256) void k_fused(FusedArgs f, const uint32_t* list
 
 template <int S>
 static void launch_fused_class(hipStream_t st, const FusedArgs& f, const uint32_t* list, const uint32_t* count,
-                               uint32_t max_sites)
+                               uint32_t max_sites, uint32_t want_wpb, uint32_t max_blocks)
 {
     constexpr uint32_t G = fused_group<S>();
     const uint32_t per_wave = (G * fused_row_stride<S>() + 64u * f.sw) * 4u;
-    const uint32_t wpb = 4u * per_wave <= 65536u ? 4u : 2u * per_wave <= 65536u ? 2u : 1u;
+    uint32_t wpb = 4u * per_wave <= 65536u ? 4u : 2u * per_wave <= 65536u ? 2u : 1u;
+    if (want_wpb && want_wpb < wpb) wpb = want_wpb;
     const uint32_t groups = (max_sites + G - 1) / G;
     uint32_t blocks = (groups + wpb - 1) / wpb;
-    if (blocks > 8192u) blocks = 8192u;
+    if (blocks > max_blocks) blocks = max_blocks;
     if (!blocks) blocks = 1;
     hipLaunchKernelGGL(k_fused<S>, dim3(blocks), dim3(64 * wpb), wpb * per_wave, st, f, list, count);
 }
 
-void launch_fused(hipStream_t st, const FusedArgs& f, const uint32_t* lists, uint32_t pitch, const uint32_t* counts,
-                  uint32_t max_sites)
+void launch_fused_one(hipStream_t st, const FusedArgs& f, const uint32_t* lists, uint32_t pitch,
+                      const uint32_t* counts, uint32_t max_sites, uint32_t cls, uint32_t wpb, uint32_t max_blocks)
 {
     if (!max_sites) return;
-    launch_fused_class<256>(st, f, lists + 5 * (size_t)pitch, counts + 5, max_sites);
-    launch_fused_class<128>(st, f, lists + 4 * (size_t)pitch, counts + 4, max_sites);
-    launch_fused_class<64>(st, f, lists + 3 * (size_t)pitch, counts + 3, max_sites);
-    launch_fused_class<32>(st, f, lists + 2 * (size_t)pitch, counts + 2, max_sites);
-    launch_fused_class<16>(st, f, lists + 1 * (size_t)pitch, counts + 1, max_sites);
-    launch_fused_class<8>(st, f, lists + 0 * (size_t)pitch, counts + 0, max_sites);
+    const uint32_t* list = lists + (size_t)cls * pitch;
+    const uint32_t* count = counts + cls;
+    if (!max_blocks) max_blocks = 8192u;
+    switch (cls) {
+    case 0: launch_fused_class<8>(st, f, list, count, max_sites, wpb, max_blocks); break;
+    case 1: launch_fused_class<16>(st, f, list, count, max_sites, wpb, max_blocks); break;
+    case 2: launch_fused_class<32>(st, f, list, count, max_sites, wpb, max_blocks); break;
+    case 3: launch_fused_class<64>(st, f, list, count, max_sites, wpb, max_blocks); break;
+    case 4: launch_fused_class<128>(st, f, list, count, max_sites, wpb, max_blocks); break;
+    default: launch_fused_class<256>(st, f, list, count, max_sites, wpb, max_blocks); break;
+    }
+}
+
+void launch_fused(hipStream_t st, const FusedArgs& f, const uint32_t* lists, uint32_t pitch, const uint32_t* counts,
+                  uint32_t max_sites, const FusedTune& tune)
+{
+    if (!max_sites) return;
+    // deepest classes first: their groups are the longest
+    for (int cls = 5; cls >= (int)tune.min_class; cls--)
+        launch_fused_one(st, f, lists, pitch, counts, max_sites, (uint32_t)cls, tune.wpb[cls], tune.max_blocks[cls]);
 }
 
 // ---------------------------------------------------------------------------

@@ -154,6 +154,15 @@ int gsnp_window_recycle(gsnp_ctx* ctx);
  * Call_win::new_soap2cns, cpp/call_genotype.cc:474-644). */
 int gsnp_batched_call(gsnp_ctx* ctx, uint32_t start, uint32_t n_sites, int emit_text, uint64_t* nbytes);
 
+/* Autotuner of the batched form: builds the span's site matrix, then times
+ * the fused launch of every depth class on it for each smallest network width
+ * (8, 16, 32: the lowest bucket edge), waves per block (1, 2, 4) and grid cap
+ * (512..8192 blocks), and keeps the fastest shape in the context for later
+ * gsnp_batched_call / gsnp_job_call.  reps timings per candidate, best kept.
+ * config (13 entries, may be NULL) receives min class, waves per block of the
+ * six classes, grid caps of the six classes.  No reference counterpart. */
+int gsnp_autotune(gsnp_ctx* ctx, uint32_t start, uint32_t n_sites, int reps, uint32_t* config);
+
 /* Whole pipeline on files: FASTA + sorted SOAP alignments (+ known SNPs) to
  * .cns text or, with compress, the -C column files <out>.1_2 ... <out>.15_17.
  * matrix_in/matrix_out are the -I/-M calibration files (may be NULL).
