@@ -193,20 +193,21 @@ def refit(adj: Adjoint, p0: np.ndarray, log_het: float, steps: int = 50, lr: flo
     return out.reshape(-1), trace
 
 
-def calls_changed(cns_a, cns_b) -> int:
-    """Rows whose genotype call or consensus quality differ between two .cns files."""
-    n = 0
+def calls_changed(cns_a, cns_b) -> tuple[int, int]:
+    """(rows whose genotype call differs, rows whose consensus quality differs)
+    between two .cns files of the same sites."""
+    ng = nq = 0
     with open(cns_a) as fa, open(cns_b) as fb:
         for la, lb in zip(fa, fb):
             ta, tb = la.split("\t", 5), lb.split("\t", 5)
-            if ta[3] != tb[3] or ta[4] != tb[4]:
-                n += 1
-    return n
+            ng += ta[3] != tb[3]
+            nq += ta[4] != tb[4]
+    return ng, nq
 
 
 def refit_and_judge(fasta, soap, workdir, steps: int = 50, lr: float = 0.02, **kw):
     """Counting-trained table -> refit -> plain calling with both tables; returns
-    (calls changed, objective trace, n_sites)."""
+    ((genotype calls changed, consensus qualities changed), objective trace, n_sites)."""
     from . import Job, matrix_gen, run
 
     work = Path(workdir)

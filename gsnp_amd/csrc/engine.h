@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -82,6 +83,23 @@ struct StageTimer {
     ~StageTimer();
 };
 
+// An instantiated graph of a captured launch sequence and the key (shape and
+// buffer generation) it was captured for.
+struct GraphSlot {
+    hipGraphExec_t exec = nullptr;
+    std::vector<uint64_t> key;
+    GraphSlot() = default;
+    GraphSlot(const GraphSlot&) = delete;
+    GraphSlot& operator=(const GraphSlot&) = delete;
+    void reset()
+    {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        exec = nullptr;
+        key.clear();
+    }
+    ~GraphSlot() { reset(); }
+};
+
 struct Ctx {
     gsnp_params p{};
     hipStream_t stream = nullptr;
@@ -128,6 +146,14 @@ struct Ctx {
     DevBuf<uint64_t> ranges;  // read interval per 256-site block (site build)
     bool keep_lk = false;     // batched form: also store the per-site log-likelihoods
     FusedTune tune;           // launch shape of the fused stage (gsnp_autotune)
+    // batched span and its formatting replay as graphs while shape and buffers
+    // stay the same; buf_gen counts buffer (re)allocations.  Off: a run that
+    // replayed a captured span did not finish and the cause is not known, so
+    // the spans are launched eagerly until it is.
+    bool use_graphs = false;
+    uint64_t buf_gen = 0, graphs_gen = 0;
+    std::vector<std::unique_ptr<GraphSlot>> graphs;  // captured spans, at most kMaxGraphs
+    GraphSlot& graph_slot(const std::vector<uint64_t>& key);
     DevBuf<char> text;
     uint64_t text_cap = 0, text_len = 0;
     DevBuf<uint32_t> err;
@@ -157,8 +183,13 @@ struct Ctx {
     ReadView read_view() const;
     int host_fixups();
     int check_err();
-    int format_text(uint32_t offset, uint32_t ws_abs, uint32_t n_sites, uint32_t pos_base, uint64_t* nbytes);
+    int format_text(uint32_t offset, uint32_t ws_abs, uint32_t n_sites, uint32_t pos_base, uint64_t* nbytes,
+                    bool graph = false);
     int span_build(uint32_t start, uint32_t n);
+    int span_prepare(uint32_t n);
+    void span_enqueue(uint32_t start, uint32_t n);
+    template <class F>
+    int launch_captured(GraphSlot& slot, const std::vector<uint64_t>& key, F enqueue);
     FusedArgs fused_args(uint32_t start);
 };
 

@@ -92,6 +92,7 @@ int Ctx::alloc_window(uint32_t n_sites)
 {
     if (n_sites <= win_cap) return GSNP_OK;
     HIPCHK(hipStreamSynchronize(stream));  // nothing queued may still use the old buffers
+    buf_gen++;
     sites.reset();
     win_cap = 0;
     const size_t n = n_sites;
@@ -361,6 +362,7 @@ int gsnp_set_chromosome(gsnp_ctx* c, const char* name, const uint8_t* codes, uin
     HIPCHK(hipStreamSynchronize(c->stream));
     c->chr_name = name;
     c->chr_len = len;
+    c->buf_gen++;
     HIPCHK(c->ref.alloc((size_t)len + GSNP_WIN + 512));
     HIPCHK(hipMemsetAsync(c->ref.p, 4, (size_t)len + GSNP_WIN + 512, c->stream));  // beyond the end reads as N
     if (len) HIPCHK(hipMemcpyAsync(c->ref.p, codes, len, hipMemcpyHostToDevice, c->stream));
@@ -437,6 +439,7 @@ int gsnp_load_reads(gsnp_ctx* c, const gsnp_read_batch* b)
     HIPCHK(hipStreamSynchronize(c->stream));
     c->n_reads = b->n;
     c->pitch = L;
+    c->buf_gen++;
     c->h_rpos.assign(b->pos, b->pos + b->n);
     HIPCHK(c->rpos.alloc(b->n + 1));
     HIPCHK(c->rmeta.alloc(b->n + 1));
@@ -587,32 +590,26 @@ int gsnp_window_recycle(gsnp_ctx* c)
 int gsnp_batched_call(gsnp_ctx* c, uint32_t start, uint32_t n, int emit_text, uint64_t* nbytes)
 {
     if (!c) return fail(GSNP_E_ARG, "null context");
-    int rc = c->span_build(start, n);
+    if ((uint64_t)start + n > c->chr_len) return fail(GSNP_E_ARG, "span outside the chromosome");
+    if (start % GSNP_WIN) return fail(GSNP_E_ARG, "span must start on a window boundary");
+    int rc = c->span_prepare(n);
     if (rc) return rc;
-    SiteArrays a = c->site_view(0);
-    Results res = c->result_view(0);
-    HIPCHK(hipMemsetAsync(c->job_count.p, 0, 4, c->stream));
-    HIPCHK(hipMemsetAsync(c->fix_count.p, 0, 4, c->stream));
-    // every site into a depth class, then one fused sort+likelihood+posterior launch per class
-    c->timer.begin(kStFused, c->stream);
-    launch_bucket(c->stream, a.nrec, n, c->lists.p, c->win_cap, c->counts.p, 1, c->stat.p, (int)c->tune.min_class);
-    launch_fused(c->stream, c->fused_args(start), c->lists.p, c->win_cap, c->counts.p, n, c->tune);
-    c->timer.end(kStFused, c->stream);
+    const FusedTune& t = c->tune;
+    std::vector<uint64_t> key = {c->buf_gen, start, n, (uint64_t)c->p.rank_sum_mode, (uint64_t)c->keep_lk,
+                                 t.min_class};
+    for (int i = 0; i < 6; i++) key.push_back(((uint64_t)t.wpb[i] << 32) | t.max_blocks[i]);
+    rc = c->launch_captured(c->graph_slot(key), key, [&] { c->span_enqueue(start, n); });
+    if (rc) return rc;
     if (c->p.rank_sum_mode) {
-        c->timer.begin(kStRank, c->stream);
-        launch_rank(c->stream, c->tables(), a, c->jobs.p, c->job_count.p, n, res, c->fixes.p, c->fix_count.p);
-        c->timer.end(kStRank, c->stream);
-        HIPCHK(hipGetLastError());
         c->fix_offset = 0;
         rc = c->host_fixups();
         if (rc) return rc;
     }
-    HIPCHK(hipGetLastError());
     rc = c->check_err();
     if (rc) return rc;
     c->ws = start;
     c->built = false;
-    if (emit_text) return c->format_text(0, start, n, start + 1, nbytes);
+    if (emit_text) return c->format_text(0, start, n, start + 1, nbytes, true);
     if (nbytes) *nbytes = 0;
     return GSNP_OK;
 }
@@ -751,15 +748,101 @@ namespace gsnp {
 // Site matrix of the span [start, start+n) straight from the device read
 // store: every read that covers it, no window batches and no carry, with the
 // q_sum of window-boundary sites before their own reads kept in head_qs.
+int Ctx::span_prepare(uint32_t n)
+{
+    int rc = alloc_window(n + 2);
+    if (rc) return rc;
+    const uint32_t nheads = n / GSNP_WIN + 2;
+    if (head_qs.n < nheads) {
+        HIPCHK(hipStreamSynchronize(stream));
+        HIPCHK(head_qs.alloc(nheads));
+        buf_gen++;
+    }
+    return GSNP_OK;
+}
+
+// The device work of one batched span, up to the rank-sum jobs: site matrix,
+// depth classes, fused sort + likelihood + posterior, rank-sum.  Launches
+// only (no copies, no memsets, no host waits), so it can be captured.
+void Ctx::span_enqueue(uint32_t start, uint32_t n)
+{
+    launch_fill_u32(stream, head_qs.p, (uint32_t)(4 * (n / GSNP_WIN + 2)), 0u);
+    timer.begin(kStSiteBuild, stream);
+    launch_site_build(stream, read_view(), 0, n_reads, start, n + 1, INT64_MIN, (int64_t)start + n, 0, site_view(0),
+                      head_qs.p, ranges.p);
+    timer.end(kStSiteBuild, stream);
+    launch_fill_u32(stream, job_count.p, 1, 0u);
+    launch_fill_u32(stream, fix_count.p, 1, 0u);
+    SiteArrays a = site_view(0);
+    timer.begin(kStFused, stream);
+    launch_bucket(stream, a.nrec, n, lists.p, win_cap, counts.p, 1, stat.p, (int)tune.min_class);
+    launch_fused(stream, fused_args(start), lists.p, win_cap, counts.p, n, tune);
+    timer.end(kStFused, stream);
+    if (p.rank_sum_mode) {
+        Results res = result_view(0);
+        timer.begin(kStRank, stream);
+        launch_rank(stream, tables(), a, jobs.p, job_count.p, n, res, fixes.p, fix_count.p);
+        timer.end(kStRank, stream);
+    }
+}
+
+// Runs enqueue() through a graph captured for `key`, capturing it on first
+// use; eagerly when graphs are off or the stage timers are on (their events
+// belong to the eager stream).  A capture that fails turns graphs off.
+template <class F>
+int Ctx::launch_captured(GraphSlot& slot, const std::vector<uint64_t>& key, F enqueue)
+{
+    if (!use_graphs || timer.on) {
+        enqueue();
+        HIPCHK(hipGetLastError());
+        return GSNP_OK;
+    }
+    if (!slot.exec || slot.key != key) {
+        slot.reset();
+        hipGraph_t g = nullptr;
+        HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+        enqueue();
+        hipError_t e = hipStreamEndCapture(stream, &g);
+        if (e == hipSuccess && g) e = hipGraphInstantiate(&slot.exec, g, nullptr, nullptr, 0);
+        if (g) (void)hipGraphDestroy(g);
+        if (e != hipSuccess || !slot.exec) {
+            (void)hipGetLastError();
+            slot.exec = nullptr;
+            use_graphs = false;
+            enqueue();
+            HIPCHK(hipGetLastError());
+            return GSNP_OK;
+        }
+        slot.key = key;
+    }
+    HIPCHK(hipGraphLaunch(slot.exec, stream));
+    return GSNP_OK;
+}
+
+constexpr size_t kMaxGraphs = 64;
+
+// The cached graph for `key`, or an empty slot for it.  Graphs captured
+// before the last buffer (re)allocation are dropped: their pointers are gone.
+GraphSlot& Ctx::graph_slot(const std::vector<uint64_t>& key)
+{
+    if (graphs_gen != buf_gen) {
+        graphs.clear();
+        graphs_gen = buf_gen;
+    }
+    for (auto& g : graphs)
+        if (g->key == key) return *g;
+    if (graphs.size() >= kMaxGraphs) graphs.erase(graphs.begin());
+    graphs.push_back(std::make_unique<GraphSlot>());
+    return *graphs.back();
+}
+
 int Ctx::span_build(uint32_t start, uint32_t n)
 {
     if ((uint64_t)start + n > chr_len) return fail(GSNP_E_ARG, "span outside the chromosome");
     if (start % GSNP_WIN) return fail(GSNP_E_ARG, "span must start on a window boundary");
-    int rc = alloc_window(n + 2);
+    int rc = span_prepare(n);
     if (rc) return rc;
-    const uint32_t nheads = n / GSNP_WIN + 2;
-    if (head_qs.n < nheads) HIPCHK(head_qs.alloc(nheads));
-    HIPCHK(hipMemsetAsync(head_qs.p, 0, sizeof(int4) * nheads, stream));
+    launch_fill_u32(stream, head_qs.p, (uint32_t)(4 * (n / GSNP_WIN + 2)), 0u);
     timer.begin(kStSiteBuild, stream);
     launch_site_build(stream, read_view(), 0, n_reads, start, n + 1, INT64_MIN, (int64_t)start + n, 0, site_view(0),
                       head_qs.p, ranges.p);
@@ -785,13 +868,16 @@ FusedArgs Ctx::fused_args(uint32_t start)
     return f;
 }
 
-int Ctx::format_text(uint32_t offset, uint32_t ws_abs, uint32_t n_sites, uint32_t pos_base, uint64_t* nbytes)
+int Ctx::format_text(uint32_t offset, uint32_t ws_abs, uint32_t n_sites, uint32_t pos_base, uint64_t* nbytes,
+                     bool graph)
 {
     const size_t name_len = chr_name.size();
     const uint64_t need = (uint64_t)n_sites * (name_len + 160) + 64;
     if (need > text_cap) {
+        HIPCHK(hipStreamSynchronize(stream));
         HIPCHK(text.alloc(need));
         text_cap = need;
+        buf_gen++;
     }
     RowCtx rc;
     rc.name = d_name.p;
@@ -800,12 +886,21 @@ int Ctx::format_text(uint32_t offset, uint32_t ws_abs, uint32_t n_sites, uint32_
     SiteArrays a = site_view(offset);
     Results res = result_view(offset);
     const uint32_t nb = (n_sites + kRowBlock - 1) / kRowBlock;
-    timer.begin(kStFormat, stream);
-    launch_rowlen(stream, ref_view(), ws_abs, a, res, n_sites, rc, rowlen.p, blocksum.p);
-    launch_scan_blocks(stream, blocksum.p, nb, total.p);
-    launch_format(stream, ref_view(), ws_abs, a, res, n_sites, rc, rowlen.p, blocksum.p, text.p);
-    timer.end(kStFormat, stream);
-    HIPCHK(hipGetLastError());
+    auto enqueue = [&] {
+        timer.begin(kStFormat, stream);
+        launch_rowlen(stream, ref_view(), ws_abs, a, res, n_sites, rc, rowlen.p, blocksum.p);
+        launch_scan_blocks(stream, blocksum.p, nb, total.p);
+        launch_format(stream, ref_view(), ws_abs, a, res, n_sites, rc, rowlen.p, blocksum.p, text.p);
+        timer.end(kStFormat, stream);
+    };
+    if (graph) {
+        const std::vector<uint64_t> key = {buf_gen, 1u, offset, ws_abs, n_sites, pos_base};
+        int r = launch_captured(graph_slot(key), key, enqueue);
+        if (r) return r;
+    } else {
+        enqueue();
+        HIPCHK(hipGetLastError());
+    }
     uint64_t t = 0;
     HIPCHK(hipMemcpyAsync(&t, total.p, 8, hipMemcpyDeviceToHost, stream));
     HIPCHK(hipStreamSynchronize(stream));

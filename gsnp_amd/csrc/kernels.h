@@ -106,6 +106,8 @@ struct FusedTune {
     uint32_t max_blocks[6] = {8192, 8192, 8192, 8192, 8192, 8192};
 };
 
+void launch_fill_u32(hipStream_t s, void* p, uint32_t n_words, uint32_t v);
+
 // range: 2 u64 per 256-site block (read interval of the block)
 void launch_site_build(hipStream_t s, const ReadView& r, uint64_t r0, uint64_t r1, uint32_t ws, uint32_t n_sites,
                        int64_t lo_min, int64_t hi_max, uint32_t carry, const SiteArrays& a, int4* head_qs,

@@ -202,6 +202,21 @@ void launch_site_build(hipStream_t st, const ReadView& r, uint64_t r0, uint64_t 
 // order (base, quality descending, cycle, strand) comes out in the first n
 // slots, as kernel/bitonic_kernel.cu:119-176 produces.
 
+// word fill: clears counters and flags with a kernel, so that the sequence
+// can be captured into a graph (memset nodes are avoided there)
+__global__ __launch_bounds__(256) void k_fill_u32(uint32_t* p, uint32_t n, uint32_t v)
+{
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+void launch_fill_u32(hipStream_t st, void* p, uint32_t n_words, uint32_t v)
+{
+    if (n_words)
+        hipLaunchKernelGGL(k_fill_u32, dim3((n_words + 255) / 256), dim3(256), 0, st, static_cast<uint32_t*>(p),
+                           n_words, v);
+}
+
 // Bucketing: each block classifies 4096 sites (16 per thread) with
 // wave-aggregated LDS counters, reserves its slice of every list with one
 // global atomic per class, then scatters the site indices.  With `all` set,
@@ -264,7 +279,7 @@ __global__ __launch_bounds__(256) void k_bucket(const int32_t* nrec, uint32_t n_
 void launch_bucket(hipStream_t st, const int32_t* nrec, uint32_t n_sites, uint32_t* lists, uint32_t pitch,
                    uint32_t* counts, int all, unsigned long long* n_records, int min_class)
 {
-    hipMemsetAsync(counts, 0, 6 * sizeof(uint32_t), st);
+    launch_fill_u32(st, counts, 6, 0u);
     const uint32_t blocks = (n_sites + 256 * kBucketPer - 1) / (256 * kBucketPer);
     if (blocks)
         hipLaunchKernelGGL(k_bucket, dim3(blocks), dim3(256), 0, st, nrec, n_sites, lists, pitch, counts, all,
@@ -647,7 +662,8 @@ void launch_posterior(hipStream_t st, const Tables& t, const RefView& ref, uint3
 //     the record in LDS together with its cycle and base.
 //  2b. The lane sums the ten score rows in record order, four records per
 //     step so that their table loads are in flight together; the additions
-//     stay in record order, as in k_likelihood.
+//     stay in record order, as in k_likelihood.  (Spreading a row's loads
+//     over five lanes measured slower on MI355X.)
 //  3. posterior_site calls the site from the sums held in registers.
 // Records come from k_site_build, so every occurrence field is 1.
 
@@ -660,12 +676,12 @@ constexpr uint32_t fused_group()
     return S <= 64 ? 64u : 4096u / S;
 }
 
-// LDS words per site row: odd (conflict-free row walks) and at least 20 (the
-// row later holds the site's ten FP64 sums)
+// LDS words per site row: odd, so that row walks of the 64 lanes are
+// conflict-free
 template <int S>
 constexpr uint32_t fused_row_stride()
 {
-    return S + 1 < 21 ? 21u : (uint32_t)S + 1u;
+    return (uint32_t)S + 1u;
 }
 
 __device__ __forceinline__ void wave_lds_sync()
@@ -803,96 +819,50 @@ __global__ __launch_bounds__(256) void k_fused(FusedArgs f, const uint32_t* list
             const uint32_t qidx = (q * td + tm1) * gd + gi;
             row[i] = (qidx << 10) | (cyc << 2) | o;
         }
-        // phase 2b: adjusted qualities and score rows summed in record order.
-        // Five lanes share a site, lane h of the five loading and summing
-        // genotypes 2h, 2h+1 (16 bytes of the site's 128-byte score row), so a
-        // load instruction touches one line per record instead of one per
-        // lane.  The G*5 (site, h) slots are spread over the wave, five per
-        // lane, and two records per slot are in flight per step.
-        wave_lds_sync();
-        constexpr uint32_t NSLOT = (G * 5 + 63) / 64;
-        uint32_t sl_row[NSLOT], sl_n[NSLOT], sl_h[NSLOT];
-        double s0[NSLOT], s1[NSLOT];
-        uint32_t maxn = 0;
-#pragma unroll
-        for (uint32_t k = 0; k < NSLOT; k++) {
-            const uint32_t v = lane + 64u * k;
-            const uint32_t si = v / 5u;
-            const uint32_t nn = (uint32_t)__shfl((int)n, (int)(si < 64u ? si : 0u));
-            const bool ok = si < gcount;
-            sl_row[k] = (ok ? si : 0u) * RS;
-            sl_n[k] = ok ? nn : 0u;
-            sl_h[k] = v % 5u;
-            s0[k] = 0.0;
-            s1[k] = 0.0;
-            maxn = sl_n[k] > maxn ? sl_n[k] : maxn;
-        }
-        for (uint32_t i = 0; i < maxn; i += 2) {
-            uint32_t x[NSLOT][2];
-#pragma unroll
-            for (uint32_t k = 0; k < NSLOT; k++)
-#pragma unroll
-                for (uint32_t u = 0; u < 2; u++) x[k][u] = i + u < sl_n[k] ? srt[sl_row[k] + i + u] : 0u;
-            uint32_t qa[NSLOT][2];
-#pragma unroll
-            for (uint32_t k = 0; k < NSLOT; k++)
-#pragma unroll
-                for (uint32_t u = 0; u < 2; u++) qa[k][u] = i + u < sl_n[k] ? f.t.qadj[x[k][u] >> 10] : 1u;
-            double2 val[NSLOT][2];
-#pragma unroll
-            for (uint32_t k = 0; k < NSLOT; k++)
-#pragma unroll
-                for (uint32_t u = 0; u < 2; u++) {
-                    uint32_t a = qa[k][u];
-                    if (a == 0) {
-                        bad |= kErrQadjRange;
-                        a = 1;
-                    }
-                    val[k][u] = i + u < sl_n[k]
-                                    ? *reinterpret_cast<const double2*>(
-                                          f.t.score + (size_t)((a << 10) | (x[k][u] & 1023u)) * kScoreRowStride +
-                                          2u * sl_h[k])
-                                    : make_double2(0.0, 0.0);
-                }
-#pragma unroll
-            for (uint32_t k = 0; k < NSLOT; k++)
-#pragma unroll
-                for (uint32_t u = 0; u < 2; u++)
-                    if (i + u < sl_n[k]) {
-                        s0[k] += val[k][u].x;
-                        s1[k] += val[k][u].y;
-                    }
-        }
-        if (bad) atomicOr(f.err, bad);
-        // the row area of each site now takes its ten sums (20 words)
-        wave_lds_sync();
-#pragma unroll
-        for (uint32_t k = 0; k < NSLOT; k++) {
-            const uint32_t v = lane + 64u * k;
-            if (v / 5u < gcount) {
-                uint32_t* d = srt + (v / 5u) * RS + 4u * sl_h[k];
-                const unsigned long long b0 = (unsigned long long)__double_as_longlong(s0[k]);
-                const unsigned long long b1 = (unsigned long long)__double_as_longlong(s1[k]);
-                d[0] = (uint32_t)b0;
-                d[1] = (uint32_t)(b0 >> 32);
-                d[2] = (uint32_t)b1;
-                d[3] = (uint32_t)(b1 >> 32);
-            }
-        }
-        wave_lds_sync();
-        if (!act) continue;
+        // phase 2b: adjusted qualities and score rows (one 128-byte line each),
+        // summed in record order; four records per step so their table loads
+        // are in flight together
         double acc[10];
 #pragma unroll
-        for (int g = 0; g < 10; g++) {
-            const uint32_t* d = srt + lane * RS + 2u * (uint32_t)g;
-            acc[g] = __longlong_as_double((long long)((unsigned long long)d[0] | ((unsigned long long)d[1] << 32)));
+        for (int g = 0; g < 10; g++) acc[g] = 0.0;
+        for (uint32_t i = 0; i < n; i += 4) {
+            uint32_t x[4], qa[4];
+#pragma unroll
+            for (int j = 0; j < 4; j++) x[j] = i + j < n ? row[i + j] : 0u;
+#pragma unroll
+            for (int j = 0; j < 4; j++) qa[j] = i + j < n ? f.t.qadj[x[j] >> 10] : 1u;
+            double2 v[4][5];
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                uint32_t a = qa[j];
+                if (a == 0) {
+                    bad |= kErrQadjRange;
+                    a = 1;
+                }
+                const double2* sr =
+                    reinterpret_cast<const double2*>(f.t.score + (size_t)((a << 10) | (x[j] & 1023u)) * kScoreRowStride);
+#pragma unroll
+                for (int h = 0; h < 5; h++) v[j][h] = i + j < n ? sr[h] : make_double2(0.0, 0.0);
+            }
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                if (i + j < n) {
+#pragma unroll
+                    for (int h = 0; h < 5; h++) {
+                        acc[2 * h] += v[j][h].x;
+                        acc[2 * h + 1] += v[j][h].y;
+                    }
+                }
+            }
         }
+        if (bad) atomicOr(f.err, bad);
+        if (!act) continue;
         if (f.write_lk) {
             double2* out = reinterpret_cast<double2*>(f.out.lk + (size_t)site * 10);
 #pragma unroll
             for (int h = 0; h < 5; h++) out[h] = make_double2(acc[2 * h], acc[2 * h + 1]);
         }
-        // phase 3: posterior from the sums
+        // phase 3: posterior from the sums in registers
         posterior_site(f.t, f.ref, f.ws, f.a, site, [&](int g) { return acc[g]; }, f.head_qs, f.out, f.jobs,
                        f.job_count);
     }

