@@ -650,30 +650,33 @@ void launch_posterior(hipStream_t st, const Tables& t, const RefView& ref, uint3
 // Batched form: sort + likelihood + posterior fused per depth class.  A wave
 // takes a group of G sites of one class (record count <= S) from the bucket
 // lists; lane i owns site i of the group.
-//  1. Sort: for S <= 64 every record slot of the group is loaded first (S
-//     loads per lane in flight together), then lane groups of S sort 64/S
-//     sites per pass with the k_sort_lanes networks; for S = 128/256 one
-//     site per pass with S/64 records per lane.  Sorted rows are parked in
-//     LDS as [G][S+1] words (conflict-free both ways).
-//  2a. One lane per site walks its sorted row: the dependency counts t and g
-//     of each record (t from a per-lane table of 2-bit saturating counters
-//     over the segment's (strand, cycle) keys; from the third occurrence on,
-//     an exact recount) give the adjusted-quality table index, which replaces
-//     the record in LDS together with its cycle and base.
-//  2b. The lane sums the ten score rows in record order, four records per
-//     step so that their table loads are in flight together; the additions
-//     stay in record order, as in k_likelihood.  (Spreading a row's loads
-//     over five lanes measured slower on MI355X.)
+//  1. Sort: every record slot of the group is loaded first (the loads are in
+//     flight together), then for S <= 64 lane groups of S sort 64/S sites per
+//     pass with the k_sort_lanes networks, for S = 128/256 one site per pass
+//     with S/64 records per lane.  Each sorted record then learns its
+//     dependency count t - 1: how many records before it in sorted order
+//     share its (base, strand, cycle) key -- the same base means the same
+//     segment.  For S <= 64 one ballot per key bit gives each lane the set of
+//     lanes in its group holding its key, and t - 1 is the count of those
+//     below it; for the wide classes the lanes scan the parked row.  t - 1 (< 255)
+//     replaces the record's occurrence field (always 1 from k_site_build) and
+//     the sorted rows are parked in LDS as [G][S+1] words.
+//  2. One lane per site walks its row in record order: g counts the
+//     segment's first occurrences, (q, t, g) indexes the adjusted-quality
+//     table and the ten score values of (q_adj, cycle, base) -- one 128-byte
+//     line -- are added in that order, as in k_likelihood; four records per
+//     step so that their table loads are in flight together.  (Spreading a
+//     row's loads over five lanes measured slower on MI355X.)
 //  3. posterior_site calls the site from the sums held in registers.
-// Records come from k_site_build, so every occurrence field is 1.
 
 constexpr uint32_t kRecKeyMask = 0x0181FF00u;  // base, cycle, strand fields of a record
-constexpr uint32_t kRecQMask = 0x007E0000u;    // 64-q field
 
+// G: sites per wave group.  The wide classes hold few sites, so their groups
+// are small to spread them over more waves.
 template <int S>
 constexpr uint32_t fused_group()
 {
-    return S <= 64 ? 64u : 4096u / S;
+    return S <= 64 ? 64u : S == 128 ? 16u : 8u;
 }
 
 // LDS words per site row: odd, so that row walks of the 64 lanes are
@@ -699,14 +702,11 @@ __global__ __launch_bounds__(256) void k_fused(FusedArgs f, const uint32_t* list
     extern __shared__ uint32_t lds[];
     const uint32_t lane = lane_id();
     const uint32_t wib = threadIdx.x >> 6;
-    const uint32_t per_wave = G * RS + 64u * f.sw;
-    uint32_t* srt = lds + wib * per_wave;
-    uint32_t* seen = srt + G * RS + lane * f.sw;
+    uint32_t* srt = lds + wib * (G * RS);
     const uint32_t count = *count_ptr;
     const uint32_t ngroups = (count + G - 1) / G;
     const uint32_t wave = blockIdx.x * (blockDim.x >> 6) + wib;
     const uint32_t nwave = gridDim.x * (blockDim.x >> 6);
-    const uint32_t L = (uint32_t)f.t.read_len;
     const uint32_t td = (uint32_t)f.t.td, gd = (uint32_t)f.t.gd;
     for (uint32_t grp = wave; grp < ngroups; grp += nwave) {
         const uint32_t g0 = grp * G;
@@ -716,13 +716,12 @@ __global__ __launch_bounds__(256) void k_fused(FusedArgs f, const uint32_t* list
         const uint32_t site = act ? list[g0 + lane] : 0u;
         const uint32_t n = act ? (uint32_t)f.a.nrec[site] : 0u;
         wave_lds_sync();  // the previous group's rows have been read
-        // phase 1: sort into LDS
+        // phase 1: sort, dependency counts, park in LDS
         if constexpr (S <= 64) {
-            // all G*S record slots of the group are loaded first (S per lane),
-            // so the loads are in flight together; then the P = 64/S sites of
-            // each pass are sorted by lane groups of S
             constexpr uint32_t P = 64 / S, NP = G / P;
-            const uint32_t sub = lane / S, e = lane % S;
+            const uint32_t sub = lane / S, e = lane % S, gbase = lane & ~(uint32_t)(S - 1);
+            const uint64_t gmask = (S == 64 ? ~0ull : ((1ull << S) - 1ull)) << gbase;  // lanes of my group
+            const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
             uint32_t v[NP];
 #pragma unroll
             for (uint32_t k = 0; k < NP; k++) {
@@ -736,101 +735,107 @@ __global__ __launch_bounds__(256) void k_fused(FusedArgs f, const uint32_t* list
                 const uint32_t gi = k * P + sub;
                 const uint32_t nn = (uint32_t)__shfl((int)n, (int)gi);
                 bitonic_all<2, S>(v[k], e);
-                if (e < nn) srt[gi * RS + e] = v[k];
+                // earlier records of the same key: the lanes of my group whose
+                // key agrees with mine on all eleven key bits (one ballot per
+                // bit), below my lane; padding lanes (e >= nn) take no part
+                uint64_t peers = __ballot(e < nn) & gmask;
+#pragma unroll
+                for (int b = 0; b < 11; b++) {
+                    const uint32_t bit = b == 0 ? 8u : b < 9 ? (uint32_t)(8 + b) : (uint32_t)(14 + b);  // 8, 9..16, 23, 24
+                    const bool one = (v[k] >> bit) & 1u;
+                    const uint64_t bal = __ballot(one);
+                    peers &= one ? bal : ~bal;
+                }
+                const uint32_t t = (uint32_t)__popcll(peers & below);
+                if (e < nn) srt[gi * RS + e] = (v[k] & ~0xFFu) | t;
             }
         } else {
             constexpr int R = S / 64;
-            for (uint32_t gi = 0; gi < gcount; gi++) {
+            uint32_t v[G][R];
+#pragma unroll
+            for (uint32_t gi = 0; gi < G; gi++) {
                 const uint32_t st = (uint32_t)__shfl((int)site, (int)gi);
                 const uint32_t nn = (uint32_t)__shfl((int)n, (int)gi);
-                const uint32_t* row = f.a.rec + (size_t)st * kSlots;
-                uint32_t v[R];
 #pragma unroll
                 for (int r = 0; r < R; r++) {
                     const uint32_t e = (uint32_t)r * 64u + lane;
-                    v[r] = e < nn ? row[e] : 0xFFFFFFFFu;
+                    v[gi][r] = e < nn ? f.a.rec[(size_t)st * kSlots + e] : 0xFFFFFFFFu;
                 }
+            }
 #pragma unroll
-                for (int r = 0; r < R; r++) bitonic_all<2, 64>(v[r], (uint32_t)(r * 64) + lane);
-                regs_merge<R, 128>(v, lane);
-                if constexpr (S == 256) regs_merge<R, 256>(v, lane);
+            for (uint32_t gi = 0; gi < G; gi++) {
+                const uint32_t nn = (uint32_t)__shfl((int)n, (int)gi);
+#pragma unroll
+                for (int r = 0; r < R; r++) bitonic_all<2, 64>(v[gi][r], (uint32_t)(r * 64) + lane);
+                regs_merge<R, 128>(v[gi], lane);
+                if constexpr (S == 256) regs_merge<R, 256>(v[gi], lane);
 #pragma unroll
                 for (int r = 0; r < R; r++) {
                     const uint32_t e = (uint32_t)r * 64u + lane;
-                    if (e < nn) srt[gi * RS + e] = v[r];
+                    if (e < nn) srt[gi * RS + e] = v[gi][r];
+                }
+            }
+            wave_lds_sync();
+            // dependency counts from the parked rows (keys are not touched by
+            // the low-byte updates, so the scan reads stable keys)
+#pragma unroll
+            for (uint32_t gi = 0; gi < G; gi++) {
+                const uint32_t nn = (uint32_t)__shfl((int)n, (int)gi);
+                uint32_t t[R];
+#pragma unroll
+                for (int r = 0; r < R; r++) t[r] = 0;
+                for (uint32_t j = 0; j < nn; j++) {
+                    const uint32_t kj = srt[gi * RS + j] & kRecKeyMask;
+#pragma unroll
+                    for (int r = 0; r < R; r++) {
+                        const uint32_t e = (uint32_t)r * 64u + lane;
+                        t[r] += (j < e && kj == (v[gi][r] & kRecKeyMask)) ? 1u : 0u;
+                    }
+                }
+                wave_lds_sync();
+#pragma unroll
+                for (int r = 0; r < R; r++) {
+                    const uint32_t e = (uint32_t)r * 64u + lane;
+                    if (e < nn) srt[gi * RS + e] = (v[gi][r] & ~0xFFu) | t[r];
                 }
             }
         }
         wave_lds_sync();
-        // phase 2a, one lane per site: dependency counts t, g of each record
-        // in sorted order; the record is replaced by its adjusted-quality
-        // table index and score-row fields, (qidx << 10) | (cycle << 2) | o
-        uint32_t* row = srt + lane * RS;
-        for (uint32_t k = 0; k < f.sw; k++) seen[k] = 0u;
-        int last_o = -1, gdc = -1;
-        uint32_t bad = 0, prev_w = 0xFFFFFFFFu, run_start = 0;
-        for (uint32_t i = 0; i < n; i++) {
-            const uint32_t w = row[i];
-            const uint32_t o = rec_base(w);
-            uint32_t q = rec_q(w);
-            const uint32_t cyc = rec_cycle(w);
-            if (w != prev_w) {
-                run_start = i;
-                prev_w = w;
-            }
-            if ((int)o != last_o) {
-                if (last_o >= 0)
-                    for (uint32_t k = 0; k < f.sw; k++) seen[k] = 0u;
-                gdc = -1;
-                last_o = (int)o;
-            }
-            uint32_t key = rec_strand(w) * L + cyc;
-            if (cyc >= L) {  // cannot come from k_site_build (cycle < read length)
-                bad |= kErrRecordOverflow;
-                key = 0;
-            }
-            const uint32_t wi = key >> 4, sh = (key & 15u) * 2u;
-            const uint32_t word = seen[wi];
-            const uint32_t stt = (word >> sh) & 3u;
-            uint32_t prior = stt;
-            if (stt >= 2u) {
-                // third occurrence or later.  The earlier copies precede it in
-                // sorted order: the same key with a higher quality, or the same
-                // word earlier in its run.  The LDS row already holds rewritten
-                // entries, so the higher-quality copies are counted in the
-                // unsorted row in device memory.
-                prior = i - run_start;
-                const uint32_t* hrow = f.a.rec + (size_t)site * kSlots;
-                const uint32_t kv = w & kRecKeyMask, qf = w & kRecQMask;
-                for (uint32_t j = 0; j < n; j++) {
-                    const uint32_t x = hrow[j];
-                    prior += ((x & kRecKeyMask) == kv && (x & kRecQMask) < qf) ? 1u : 0u;
-                }
-            } else {
-                seen[wi] = word + (1u << sh);
-            }
-            if (prior == 0) gdc += 1;
-            if (q > 63) {
-                bad |= kErrQadjRange;
-                q = 63;
-            }
-            const uint32_t tm1 = prior < td ? prior : td - 1;
-            const uint32_t gi = (uint32_t)gdc < gd ? (uint32_t)gdc : gd - 1;
-            const uint32_t qidx = (q * td + tm1) * gd + gi;
-            row[i] = (qidx << 10) | (cyc << 2) | o;
-        }
-        // phase 2b: adjusted qualities and score rows (one 128-byte line each),
-        // summed in record order; four records per step so their table loads
-        // are in flight together
+        // phase 2: one lane per site, records in sorted order; g counts the
+        // first occurrences of the base segment, t - 1 sits in the low byte
+        const uint32_t* row = srt + lane * RS;
         double acc[10];
 #pragma unroll
         for (int g = 0; g < 10; g++) acc[g] = 0.0;
+        int last_o = -1, gdc = -1;
+        uint32_t bad = 0;
         for (uint32_t i = 0; i < n; i += 4) {
-            uint32_t x[4], qa[4];
+            uint32_t qi[4], cyo[4];
 #pragma unroll
-            for (int j = 0; j < 4; j++) x[j] = i + j < n ? row[i + j] : 0u;
+            for (int j = 0; j < 4; j++) {
+                const uint32_t w = i + j < n ? row[i + j] : 0u;
+                const uint32_t o = rec_base(w);
+                uint32_t q = rec_q(w);
+                if (i + j < n) {
+                    if ((int)o != last_o) {
+                        gdc = -1;
+                        last_o = (int)o;
+                    }
+                    if ((w & 0xFFu) == 0) gdc += 1;
+                    if (q > 63) {
+                        bad |= kErrQadjRange;
+                        q = 63;
+                    }
+                }
+                const uint32_t prior = w & 0xFFu;
+                const uint32_t tm1 = prior < td ? prior : td - 1;
+                const uint32_t gq = (uint32_t)gdc < gd ? (uint32_t)gdc : gd - 1;
+                qi[j] = (q * td + tm1) * gd + gq;
+                cyo[j] = (rec_cycle(w) << 2) | o;
+            }
+            uint32_t qa[4];
 #pragma unroll
-            for (int j = 0; j < 4; j++) qa[j] = i + j < n ? f.t.qadj[x[j] >> 10] : 1u;
+            for (int j = 0; j < 4; j++) qa[j] = i + j < n ? f.t.qadj[qi[j]] : 1u;
             double2 v[4][5];
 #pragma unroll
             for (int j = 0; j < 4; j++) {
@@ -840,7 +845,7 @@ __global__ __launch_bounds__(256) void k_fused(FusedArgs f, const uint32_t* list
                     a = 1;
                 }
                 const double2* sr =
-                    reinterpret_cast<const double2*>(f.t.score + (size_t)((a << 10) | (x[j] & 1023u)) * kScoreRowStride);
+                    reinterpret_cast<const double2*>(f.t.score + (size_t)((a << 10) | cyo[j]) * kScoreRowStride);
 #pragma unroll
                 for (int h = 0; h < 5; h++) v[j][h] = i + j < n ? sr[h] : make_double2(0.0, 0.0);
             }
@@ -873,7 +878,7 @@ static void launch_fused_class(hipStream_t st, const FusedArgs& f, const uint32_
                                uint32_t max_sites, uint32_t want_wpb, uint32_t max_blocks)
 {
     constexpr uint32_t G = fused_group<S>();
-    const uint32_t per_wave = (G * fused_row_stride<S>() + 64u * f.sw) * 4u;
+    const uint32_t per_wave = G * fused_row_stride<S>() * 4u;
     uint32_t wpb = 4u * per_wave <= 65536u ? 4u : 2u * per_wave <= 65536u ? 2u : 1u;
     if (want_wpb && want_wpb < wpb) wpb = want_wpb;
     const uint32_t groups = (max_sites + G - 1) / G;
