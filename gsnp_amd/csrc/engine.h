@@ -144,6 +144,8 @@ struct Ctx {
     DevBuf<uint32_t> rowlen;
     DevBuf<uint64_t> blocksum, total;
     DevBuf<uint64_t> ranges;  // read interval per 256-site block (site build)
+    DevBuf<uint32_t> gidx, gcount;  // rows whose %#g columns need the exact routine
+    DevBuf<GText> gtext;
     bool keep_lk = false;     // batched form: also store the per-site log-likelihoods
     FusedTune tune;           // launch shape of the fused stage (gsnp_autotune)
     // batched span and its formatting replay as graphs while shape and buffers

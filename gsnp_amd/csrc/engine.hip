@@ -115,6 +115,8 @@ int Ctx::alloc_window(uint32_t n_sites)
     HIPCHK(rowlen.alloc(n));
     HIPCHK(blocksum.alloc(n / kRowBlock + 2));
     HIPCHK(ranges.alloc(2 * (n / 256 + 2)));
+    HIPCHK(gidx.alloc(n));
+    HIPCHK(gtext.alloc(n));
     HIPCHK(hipMemsetAsync(sites.nrec.p, 0, n * 4, stream));
     HIPCHK(hipMemsetAsync(sites.depth.p, 0, n * 4, stream));
     HIPCHK(hipMemsetAsync(sites.repeat.p, 0, n * 4, stream));
@@ -879,15 +881,20 @@ int Ctx::format_text(uint32_t offset, uint32_t ws_abs, uint32_t n_sites, uint32_
         text_cap = need;
         buf_gen++;
     }
+    if (gcount.n < 1) HIPCHK(gcount.alloc(1));
     RowCtx rc;
     rc.name = d_name.p;
     rc.name_len = (int)name_len;
     rc.pos_base = pos_base;
+    rc.gidx = gidx.p + offset;
+    rc.gtext = gtext.p;
     SiteArrays a = site_view(offset);
     Results res = result_view(offset);
     const uint32_t nb = (n_sites + kRowBlock - 1) / kRowBlock;
     auto enqueue = [&] {
         timer.begin(kStFormat, stream);
+        launch_fill_u32(stream, gcount.p, 1, 0u);
+        launch_gtext(stream, ref_view(), ws_abs, a, res, n_sites, gidx.p + offset, gtext.p, gcount.p);
         launch_rowlen(stream, ref_view(), ws_abs, a, res, n_sites, rc, rowlen.p, blocksum.p);
         launch_scan_blocks(stream, blocksum.p, nb, total.p);
         launch_format(stream, ref_view(), ws_abs, a, res, n_sites, rc, rowlen.p, blocksum.p, text.p);

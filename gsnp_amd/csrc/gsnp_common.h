@@ -205,14 +205,48 @@ GSNP_HD uint64_t round_scaled(uint64_t m, int e, int k, bool trunc)
     return round_scaled_wide(m, e, k, trunc);
 }
 
-GSNP_HD int fmt_g(double v, char* o)
+// Six significant digits D (10^5 <= D < 10^6) with decimal exponent X, as
+// %#g lays them out.
+GSNP_HD int fmt_g_digits(uint64_t D, int X, char* o, int n)
 {
-    // the two values most .cns rows carry (rank-sum 1, copy number 1 or 255)
-    if (v == 1.0 || v == 255.0) {
-        const char* s = v == 1.0 ? "1.00000" : "255.000";
-        for (int i = 0; i < 7; i++) o[i] = s[i];
-        return 7;
+    char dg[6];
+    for (int i = 5; i >= 0; i--) {
+        dg[i] = (char)('0' + D % 10u);
+        D /= 10u;
     }
+    if (X >= -4 && X < 6) {
+        if (X >= 0) {
+            for (int i = 0; i <= X; i++) o[n++] = dg[i];
+            o[n++] = '.';
+            for (int i = X + 1; i < 6; i++) o[n++] = dg[i];
+        } else {
+            o[n++] = '0';
+            o[n++] = '.';
+            for (int i = 0; i < -X - 1; i++) o[n++] = '0';
+            for (int i = 0; i < 6; i++) o[n++] = dg[i];
+        }
+    } else {
+        o[n++] = dg[0];
+        o[n++] = '.';
+        for (int i = 1; i < 6; i++) o[n++] = dg[i];
+        o[n++] = 'e';
+        int ax = X;
+        if (ax < 0) {
+            o[n++] = '-';
+            ax = -ax;
+        } else {
+            o[n++] = '+';
+        }
+        if (ax < 10) o[n++] = '0';
+        n += fmt_u32((uint32_t)ax, o + n);
+    }
+    return n;
+}
+
+// Exact %#g of any double: the decimal exponent and the rounded digits come
+// from integer arithmetic on the binary value (kept out of line).
+__host__ __device__ __noinline__ inline int fmt_g_exact(double v, char* o)
+{
     int n = 0;
     uint64_t bits;
     __builtin_memcpy(&bits, &v, 8);
@@ -263,38 +297,50 @@ GSNP_HD int fmt_g(double v, char* o)
         X += 1;
         D = 100000ull;
     }
-    char dg[6];
-    for (int i = 5; i >= 0; i--) {
-        dg[i] = (char)('0' + D % 10u);
-        D /= 10u;
-    }
-    if (X >= -4 && X < 6) {
-        if (X >= 0) {
-            for (int i = 0; i <= X; i++) o[n++] = dg[i];
-            o[n++] = '.';
-            for (int i = X + 1; i < 6; i++) o[n++] = dg[i];
-        } else {
-            o[n++] = '0';
-            o[n++] = '.';
-            for (int i = 0; i < -X - 1; i++) o[n++] = '0';
-            for (int i = 0; i < 6; i++) o[n++] = dg[i];
-        }
-    } else {
-        o[n++] = dg[0];
-        o[n++] = '.';
-        for (int i = 1; i < 6; i++) o[n++] = dg[i];
-        o[n++] = 'e';
-        int ax = X;
-        if (ax < 0) {
-            o[n++] = '-';
-            ax = -ax;
-        } else {
-            o[n++] = '+';
-        }
-        if (ax < 10) o[n++] = '0';
-        n += fmt_u32((uint32_t)ax, o + n);
-    }
-    return n;
+    return fmt_g_digits(D, X, o, n);
 }
+
+// %#g as glibc's printf writes it (6 significant digits, trailing zeros and
+// the point kept, round half to even on the exact binary value).  Values in
+// [1, 10^6) -- copy numbers -- take an FP64 path (fmt_g_fast): X from exact
+// powers of ten, y = v * 10^(5-X) rounded once (error below 2^-33 since
+// y < 2^20), and the digit rounding is exact unless y sits within that error
+// of a half, which goes to the exact routine.
+// The cases settled without the exact routine; -1 for the others.
+GSNP_HD int fmt_g_fast(double v, char* o)
+{
+    // the two values most .cns rows carry (rank-sum 1, copy number 1 or 255)
+    if (v == 1.0 || v == 255.0) {
+        const char* s = v == 1.0 ? "1.00000" : "255.000";
+        for (int i = 0; i < 7; i++) o[i] = s[i];
+        return 7;
+    }
+    if (v >= 1.0 && v < 1e6) {
+        const int X = v < 10.0 ? 0 : v < 100.0 ? 1 : v < 1e3 ? 2 : v < 1e4 ? 3 : v < 1e5 ? 4 : 5;
+        const double scale = X == 0 ? 1e5 : X == 1 ? 1e4 : X == 2 ? 1e3 : X == 3 ? 1e2 : X == 4 ? 1e1 : 1.0;
+        const double y = v * scale;
+        const double fl = floor(y);
+        const double frac = y - fl;  // exact: y < 2^20
+        if (fabs(frac - 0.5) > 2.5e-10) {
+            uint64_t D = (uint64_t)fl + (frac > 0.5 ? 1u : 0u);
+            if (D < 1000000ull) return fmt_g_digits(D, X, o, 0);
+        }
+    }
+    return -1;
+}
+
+GSNP_HD int fmt_g(double v, char* o)
+{
+    const int n = fmt_g_fast(v, o);
+    return n >= 0 ? n : fmt_g_exact(v, o);
+}
+
+// The %#g text of a row's rank-sum and copy-number columns when either needs
+// the exact routine (written by a pre-pass, so the row kernels stay small).
+// Byte 15 of each field holds its length.
+struct GText {
+    char rank[16];
+    char copy[16];
+};
 
 }  // namespace gsnp

@@ -73,10 +73,14 @@ struct FixJob {
     double T1, T2, dtp, table_rank;
 };
 
+struct GText;
+
 struct RowCtx {
     const char* name = nullptr;
     int name_len = 0;
-    uint32_t pos_base = 0;  // 1-based position of local site 0
+    uint32_t pos_base = 0;         // 1-based position of local site 0
+    const uint32_t* gidx = nullptr;  // per site: entry of gtext (valid where the fast %#g path fails)
+    const GText* gtext = nullptr;
 };
 
 // error flags written by kernels
@@ -130,6 +134,10 @@ void launch_apply_fix(hipStream_t s, const FixJob* fixes, const double* rank, co
 void launch_rowlen(hipStream_t s, const RefView& ref, uint32_t ws, const SiteArrays& a, const Results& res,
                    uint32_t n_sites, RowCtx rc, uint32_t* rowlen, uint64_t* block_sum);
 void launch_scan_blocks(hipStream_t s, uint64_t* block_sum, uint32_t n_blocks, uint64_t* total);
+// exact %#g text of the rows whose rank-sum or copy number the fast path cannot
+// settle; gcount must be zero on entry
+void launch_gtext(hipStream_t s, const RefView& ref, uint32_t ws, const SiteArrays& a, const Results& res,
+                  uint32_t n_sites, uint32_t* gidx, GText* gtext, uint32_t* gcount);
 void launch_format(hipStream_t s, const RefView& ref, uint32_t ws, const SiteArrays& a, const Results& res,
                    uint32_t n_sites, RowCtx rc, const uint32_t* rowlen, const uint64_t* block_off, char* out);
 void launch_recycle(hipStream_t s, const SiteArrays& a, uint32_t win, uint32_t L);

@@ -664,9 +664,11 @@ void launch_posterior(hipStream_t st, const Tables& t, const RefView& ref, uint3
 //  2. One lane per site walks its row in record order: g counts the
 //     segment's first occurrences, (q, t, g) indexes the adjusted-quality
 //     table and the ten score values of (q_adj, cycle, base) -- one 128-byte
-//     line -- are added in that order, as in k_likelihood; four records per
-//     step so that their table loads are in flight together.  (Spreading a
-//     row's loads over five lanes measured slower on MI355X.)
+//     line -- are added in that order, as in k_likelihood; two records per
+//     step so that their table loads are in flight together (four or eight
+//     cost occupancy and measured slower, as did spreading a row's loads over
+//     five lanes).  Records are read once, with non-temporal loads, so they
+//     do not push the score table out of L2.
 //  3. posterior_site calls the site from the sums held in registers.
 
 constexpr uint32_t kRecKeyMask = 0x0181FF00u;  // base, cycle, strand fields of a record
@@ -728,7 +730,7 @@ __global__ __launch_bounds__(256) void k_fused(FusedArgs f, const uint32_t* list
                 const uint32_t gi = k * P + sub;
                 const uint32_t st = (uint32_t)__shfl((int)site, (int)gi);
                 const uint32_t nn = (uint32_t)__shfl((int)n, (int)gi);
-                v[k] = e < nn ? f.a.rec[(size_t)st * kSlots + e] : 0xFFFFFFFFu;
+                v[k] = e < nn ? __builtin_nontemporal_load(f.a.rec + (size_t)st * kSlots + e) : 0xFFFFFFFFu;
             }
 #pragma unroll
             for (uint32_t k = 0; k < NP; k++) {
@@ -809,10 +811,10 @@ __global__ __launch_bounds__(256) void k_fused(FusedArgs f, const uint32_t* list
         for (int g = 0; g < 10; g++) acc[g] = 0.0;
         int last_o = -1, gdc = -1;
         uint32_t bad = 0;
-        for (uint32_t i = 0; i < n; i += 4) {
-            uint32_t qi[4], cyo[4];
+        for (uint32_t i = 0; i < n; i += 2) {
+            uint32_t qi[2], cyo[2];
 #pragma unroll
-            for (int j = 0; j < 4; j++) {
+            for (int j = 0; j < 2; j++) {
                 const uint32_t w = i + j < n ? row[i + j] : 0u;
                 const uint32_t o = rec_base(w);
                 uint32_t q = rec_q(w);
@@ -833,12 +835,12 @@ __global__ __launch_bounds__(256) void k_fused(FusedArgs f, const uint32_t* list
                 qi[j] = (q * td + tm1) * gd + gq;
                 cyo[j] = (rec_cycle(w) << 2) | o;
             }
-            uint32_t qa[4];
+            uint32_t qa[2];
 #pragma unroll
-            for (int j = 0; j < 4; j++) qa[j] = i + j < n ? f.t.qadj[qi[j]] : 1u;
-            double2 v[4][5];
+            for (int j = 0; j < 2; j++) qa[j] = i + j < n ? f.t.qadj[qi[j]] : 1u;
+            double2 v[2][5];
 #pragma unroll
-            for (int j = 0; j < 4; j++) {
+            for (int j = 0; j < 2; j++) {
                 uint32_t a = qa[j];
                 if (a == 0) {
                     bad |= kErrQadjRange;
@@ -850,7 +852,7 @@ __global__ __launch_bounds__(256) void k_fused(FusedArgs f, const uint32_t* list
                 for (int h = 0; h < 5; h++) v[j][h] = i + j < n ? sr[h] : make_double2(0.0, 0.0);
             }
 #pragma unroll
-            for (int j = 0; j < 4; j++) {
+            for (int j = 0; j < 2; j++) {
                 if (i + j < n) {
 #pragma unroll
                     for (int h = 0; h < 5; h++) {
@@ -1067,7 +1069,13 @@ void launch_apply_fix(hipStream_t st, const FixJob* fixes, const double* rank, c
 // the contiguous block text out with wide stores (rows of very long
 // chromosome names fall back to direct stores).
 
+__device__ __forceinline__ double copy_number(int depth, int repeat)
+{
+    return depth == 0 ? 255.0 : (double)repeat / depth;
+}
+
 struct RowData {
+    uint32_t site;
     bool nrow;
     uint32_t ori, type1, b1, b2;
     int q_cns, depth, repeat;
@@ -1079,6 +1087,7 @@ __device__ __forceinline__ RowData load_row(const RefView& ref, uint32_t ws, con
                                             uint32_t s)
 {
     RowData d;
+    d.site = s;
     d.ori = ref.code[ws + s] & 0xF;
     d.depth = a.depth[s];
     d.nrow = (d.ori & 4) && d.depth == 0;
@@ -1158,11 +1167,23 @@ __device__ __forceinline__ int emit_row(const RowData& d, const RowCtx& rc, uint
     k = fmt_i32(d.depth, buf);
     for (int i = 0; i < k; i++) put(n++, buf[i]);
     put(n++, '\t');
-    k = fmt_g(d.rank, buf);
-    for (int i = 0; i < k; i++) put(n++, buf[i]);
+    k = fmt_g_fast(d.rank, buf);
+    if (k < 0) {
+        const char* t = rc.gtext[rc.gidx[d.site]].rank;
+        k = t[15];
+        for (int i = 0; i < k; i++) put(n++, t[i]);
+    } else {
+        for (int i = 0; i < k; i++) put(n++, buf[i]);
+    }
     put(n++, '\t');
-    k = fmt_g(d.depth == 0 ? 255.0 : (double)d.repeat / d.depth, buf);
-    for (int i = 0; i < k; i++) put(n++, buf[i]);
+    k = fmt_g_fast(copy_number(d.depth, d.repeat), buf);
+    if (k < 0) {
+        const char* t = rc.gtext[rc.gidx[d.site]].copy;
+        k = t[15];
+        for (int i = 0; i < k; i++) put(n++, t[i]);
+    } else {
+        for (int i = 0; i < k; i++) put(n++, buf[i]);
+    }
     put(n++, '\t');
     put(n++, (d.ori & 8) ? '1' : '0');
     put(n++, '\n');
@@ -1237,7 +1258,34 @@ void launch_scan_blocks(hipStream_t st, uint64_t* block_sum, uint32_t n_blocks, 
     hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(1024), 0, st, block_sum, n_blocks, total);
 }
 
-constexpr uint32_t kFmtLds = 40960;
+__global__ __launch_bounds__(256) void k_gtext(RefView ref, uint32_t ws, SiteArrays a, Results res, uint32_t n_sites,
+                                               uint32_t* gidx, GText* gtext, uint32_t* gcount)
+{
+    const uint32_t s = blockIdx.x * 256u + threadIdx.x;
+    if (s >= n_sites) return;
+    const uint32_t ori = ref.code[ws + s] & 0xF;
+    const int depth = a.depth[s];
+    if ((ori & 4) && depth == 0) return;  // N row literal
+    if (res.base1[s] >= 4) return;        // N row literal
+    const double rank = res.rank[s], copy = copy_number(depth, a.repeat[s]);
+    char buf[24];
+    if (fmt_g_fast(rank, buf) >= 0 && fmt_g_fast(copy, buf) >= 0) return;
+    const uint32_t j = atomicAdd(gcount, 1u);
+    GText& g = gtext[j];
+    g.rank[15] = (char)fmt_g(rank, g.rank);
+    g.copy[15] = (char)fmt_g(copy, g.copy);
+    gidx[s] = j;
+}
+
+void launch_gtext(hipStream_t st, const RefView& ref, uint32_t ws, const SiteArrays& a, const Results& res,
+                  uint32_t n_sites, uint32_t* gidx, GText* gtext, uint32_t* gcount)
+{
+    const uint32_t blocks = (n_sites + 255) / 256;
+    if (blocks)
+        hipLaunchKernelGGL(k_gtext, dim3(blocks), dim3(256), 0, st, ref, ws, a, res, n_sites, gidx, gtext, gcount);
+}
+
+constexpr uint32_t kFmtLds = 24576;  // 256 rows of up to 96 bytes; larger blocks write directly
 
 __global__ __launch_bounds__(256) void k_format(RefView ref, uint32_t ws, SiteArrays a, Results res, uint32_t n_sites,
                                                 RowCtx rc, const uint32_t* rowlen, const uint64_t* block_off, char* out)
