@@ -113,7 +113,7 @@ struct Ctx {
     std::string chr_name;
     uint32_t chr_len = 0;
     DevBuf<uint8_t> ref;
-    DevBuf<uint32_t> snp_pos;
+    DevBuf<uint32_t> snp_pos, snp_first;
     DevBuf<double> snp_lp;
     uint32_t n_snp = 0;
     DevBuf<char> d_name;
@@ -143,7 +143,7 @@ struct Ctx {
     // text
     DevBuf<uint32_t> rowlen;
     DevBuf<uint64_t> blocksum, total;
-    DevBuf<uint64_t> ranges;  // read interval per 256-site block (site build)
+    DevBuf<uint64_t> ranges;  // read interval per 64-site block (site build)
     DevBuf<uint32_t> gidx, gcount;  // rows whose %#g columns need the exact routine
     DevBuf<GText> gtext;
     bool keep_lk = false;     // batched form: also store the per-site log-likelihoods

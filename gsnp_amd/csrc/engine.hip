@@ -114,7 +114,7 @@ int Ctx::alloc_window(uint32_t n_sites)
     HIPCHK(fixes.alloc(n));
     HIPCHK(rowlen.alloc(n));
     HIPCHK(blocksum.alloc(n / kRowBlock + 2));
-    HIPCHK(ranges.alloc(2 * (n / 256 + 2)));
+    HIPCHK(ranges.alloc(2 * (n / 64 + 2)));
     HIPCHK(gidx.alloc(n));
     HIPCHK(gtext.alloc(n));
     HIPCHK(hipMemsetAsync(sites.nrec.p, 0, n * 4, stream));
@@ -176,6 +176,7 @@ RefView Ctx::ref_view() const
     r.code = ref.p;
     r.len = chr_len;
     r.snp_pos = snp_pos.p;
+    r.snp_first = snp_first.p;
     r.snp_lp = snp_lp.p;
     r.n_snp = n_snp;
     return r;
@@ -383,6 +384,16 @@ int gsnp_set_chromosome(gsnp_ctx* c, const char* name, const uint8_t* codes, uin
         HIPCHK(c->snp_pos.alloc(n_snp));
         HIPCHK(c->snp_lp.alloc(lp.size()));
         HIPCHK(hipMemcpyAsync(c->snp_pos.p, snp_pos, 4ull * n_snp, hipMemcpyHostToDevice, c->stream));
+        // first known SNP at or after each 256-site block (lookup index of the posterior)
+        const size_t nb = (size_t)len / 256 + 2;
+        std::vector<uint32_t> first(nb);
+        uint32_t j = 0;
+        for (size_t b = 0; b < nb; b++) {
+            while (j < n_snp && snp_pos[j] < (uint64_t)b * 256) j++;
+            first[b] = j;
+        }
+        HIPCHK(c->snp_first.alloc(nb));
+        HIPCHK(hipMemcpyAsync(c->snp_first.p, first.data(), 4 * nb, hipMemcpyHostToDevice, c->stream));
         HIPCHK(hipMemcpyAsync(c->snp_lp.p, lp.data(), 8 * lp.size(), hipMemcpyHostToDevice, c->stream));
     }
     const size_t nl = std::strlen(name);

@@ -37,6 +37,7 @@ struct RefView {
     const uint32_t* snp_pos = nullptr;
     const double* snp_lp = nullptr;  // [n_snp][10]
     uint32_t n_snp = 0;
+    const uint32_t* snp_first = nullptr;  // [len/256 + 2]: first known SNP at or after each 256-site block
 };
 
 struct ReadView {
@@ -112,7 +113,7 @@ struct FusedTune {
 
 void launch_fill_u32(hipStream_t s, void* p, uint32_t n_words, uint32_t v);
 
-// range: 2 u64 per 256-site block (read interval of the block)
+// range: 2 u64 per 64-site block (read interval of the block)
 void launch_site_build(hipStream_t s, const ReadView& r, uint64_t r0, uint64_t r1, uint32_t ws, uint32_t n_sites,
                        int64_t lo_min, int64_t hi_max, uint32_t carry, const SiteArrays& a, int4* head_qs,
                        uint64_t* range);

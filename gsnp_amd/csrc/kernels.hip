@@ -31,7 +31,8 @@ __device__ uint64_t lower_pos(const int32_t* pos, uint64_t lo, uint64_t hi, int6
 }  // namespace
 
 // ---------------------------------------------------------------------------
-// (a) site matrix.  One thread per site, 256 consecutive sites per block.  The
+// (a) site matrix.  One thread per site, 64 consecutive sites per block (one
+// wave: a read covers more of a narrow block's lanes).  The
 // block walks, in input order, every read of [r0, r1) that can cover one of
 // its sites.  The reads' positions, metadata and packed bytes are first
 // staged into LDS with wide coalesced loads (in chunks when a pileup is
@@ -51,21 +52,22 @@ __device__ uint64_t lower_pos(const int32_t* pos, uint64_t lo, uint64_t hi, int6
 // its q_sum before the reads that start on it, which is what the per-window
 // path holds in its first tail site when the window is called.
 
-constexpr uint32_t kStageBytes = 32768;  // packed read bytes staged per chunk
-constexpr uint32_t kStageReads = 512;
+constexpr uint32_t kBuildSites = 64;     // sites per block (one wave)
+constexpr uint32_t kStageBytes = 8192;   // packed read bytes staged per chunk
+constexpr uint32_t kStageReads = 128;
 
-// read interval [b, e) of each 256-site block: reads starting in
-// [ws + s0 - (L-1), ws + s0 + 255] clipped to [lo_min, hi_max]
+// read interval [b, e) of each 64-site block: reads starting in
+// [ws + s0 - (L-1), ws + s0 + 63] clipped to [lo_min, hi_max]
 __global__ __launch_bounds__(256) void k_read_ranges(const int32_t* pos, uint64_t r0, uint64_t r1, uint32_t ws,
                                                      uint32_t n_blocks, uint32_t L, int64_t lo_min, int64_t hi_max,
                                                      uint64_t* range)
 {
     const uint32_t blk = blockIdx.x * 256u + threadIdx.x;
     if (blk >= n_blocks) return;
-    const uint32_t s0 = blk * 256u;
+    const uint32_t s0 = blk * kBuildSites;
     int64_t lo = (int64_t)ws + (int64_t)s0 - ((int64_t)L - 1);
     if (lo < lo_min) lo = lo_min;
-    int64_t hi = (int64_t)ws + (int64_t)s0 + 255;
+    int64_t hi = (int64_t)ws + (int64_t)s0 + (kBuildSites - 1);
     if (hi > hi_max) hi = hi_max;
     const uint64_t b = lower_pos(pos, r0, r1, lo);
     uint64_t e = lower_pos(pos, b, r1, hi + 1);
@@ -74,13 +76,13 @@ __global__ __launch_bounds__(256) void k_read_ranges(const int32_t* pos, uint64_
     range[2 * blk + 1] = e;
 }
 
-__global__ __launch_bounds__(256) void k_site_build(ReadView r, const uint64_t* range, uint32_t ws, uint32_t n_sites,
+__global__ __launch_bounds__(kBuildSites) void k_site_build(ReadView r, const uint64_t* range, uint32_t ws, uint32_t n_sites,
                                                     uint32_t carry, SiteArrays a, int4* head_qs)
 {
     __shared__ __align__(16) uint8_t sseq[kStageBytes + 32];
     __shared__ int32_t spos[kStageReads];
     __shared__ uint32_t smeta[kStageReads];
-    const uint32_t s = blockIdx.x * 256u + threadIdx.x;
+    const uint32_t s = blockIdx.x * kBuildSites + threadIdx.x;
     const uint32_t pitch = r.pitch;
     const uint32_t kr = kStageBytes / pitch < kStageReads ? kStageBytes / pitch : kStageReads;
     // block-uniform bounds: keep the read loop on scalar registers
@@ -116,7 +118,7 @@ __global__ __launch_bounds__(256) void k_site_build(ReadView r, const uint64_t* 
     for (uint64_t c0 = b; c0 < e; c0 += kr) {
         const uint32_t cn = (uint32_t)(e - c0 < kr ? e - c0 : kr);
         __syncthreads();  // the previous chunk has been consumed
-        for (uint32_t i = threadIdx.x; i < cn; i += 256) {
+        for (uint32_t i = threadIdx.x; i < cn; i += kBuildSites) {
             spos[i] = r.pos[c0 + i];
             smeta[i] = r.meta[c0 + i];
         }
@@ -128,7 +130,7 @@ __global__ __launch_bounds__(256) void k_site_build(ReadView r, const uint64_t* 
         const uint32_t nvec = (uint32_t)((g0 + (uint64_t)cn * pitch - a0 + 15) >> 4);
         const uint4* src = reinterpret_cast<const uint4*>(r.seq + a0);
         uint4* dst = reinterpret_cast<uint4*>(sseq);
-        for (uint32_t v = threadIdx.x; v < nvec; v += 256) dst[v] = src[v];
+        for (uint32_t v = threadIdx.x; v < nvec; v += kBuildSites) dst[v] = src[v];
         __syncthreads();
         for (uint32_t j = 0; j < cn; j++) {
             const int64_t start = spos[j];
@@ -185,11 +187,11 @@ void launch_site_build(hipStream_t st, const ReadView& r, uint64_t r0, uint64_t 
                        int64_t lo_min, int64_t hi_max, uint32_t carry, const SiteArrays& a, int4* head_qs,
                        uint64_t* range)
 {
-    const uint32_t blocks = (n_sites + 255) / 256;
+    const uint32_t blocks = (n_sites + kBuildSites - 1) / kBuildSites;
     if (!blocks) return;
     hipLaunchKernelGGL(k_read_ranges, dim3((blocks + 255) / 256), dim3(256), 0, st, r.pos, r0, r1, ws, blocks, r.pitch,
                        lo_min, hi_max, range);
-    hipLaunchKernelGGL(k_site_build, dim3(blocks), dim3(256), 0, st, r, range, ws, n_sites, carry, a, head_qs);
+    hipLaunchKernelGGL(k_site_build, dim3(blocks), dim3(kBuildSites), 0, st, r, range, ws, n_sites, carry, a, head_qs);
 }
 
 // ---------------------------------------------------------------------------
@@ -537,8 +539,9 @@ __device__ __forceinline__ void posterior_site(const Tables& t, const RefView& r
     // log10 priors
     const double* lp = t.lprior + (ori & 7) * 10;
     if ((ori & 8) && t.refine && ref.n_snp) {
-        uint32_t lo = 0, hi = ref.n_snp;
+        // the known SNPs of the site's 256-site block, then a search among them
         const uint32_t p = ws + s;
+        uint32_t lo = ref.snp_first[p >> 8], hi = ref.snp_first[(p >> 8) + 1];
         while (lo < hi) {
             uint32_t m = (lo + hi) >> 1;
             if (ref.snp_pos[m] < p)
