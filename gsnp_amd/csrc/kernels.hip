@@ -328,6 +328,36 @@ __device__ __forceinline__ void bitonic_all(uint32_t& v, uint32_t e)
     if constexpr (K < S) bitonic_all<K * 2, S>(v, e);
 }
 
+// Two independent networks step by step, so that each cross-lane exchange of
+// one overlaps the other's.
+template <int K, int J>
+__device__ __forceinline__ void bitonic_step2(uint32_t& v0, uint32_t& v1, uint32_t e)
+{
+    const uint32_t w0 = xor_lane<J>(v0);
+    const uint32_t w1 = xor_lane<J>(v1);
+    const bool take_min = ((e & K) == 0) == ((e & J) == 0);
+    const uint32_t mn0 = v0 < w0 ? v0 : w0, mx0 = v0 < w0 ? w0 : v0;
+    const uint32_t mn1 = v1 < w1 ? v1 : w1, mx1 = v1 < w1 ? w1 : v1;
+    v0 = take_min ? mn0 : mx0;
+    v1 = take_min ? mn1 : mx1;
+}
+
+template <int K, int J>
+struct Stage2 {
+    __device__ __forceinline__ static void run(uint32_t& v0, uint32_t& v1, uint32_t e)
+    {
+        bitonic_step2<K, J>(v0, v1, e);
+        if constexpr (J > 1) Stage2<K, J / 2>::run(v0, v1, e);
+    }
+};
+
+template <int K, int S>
+__device__ __forceinline__ void bitonic_all2(uint32_t& v0, uint32_t& v1, uint32_t e)
+{
+    Stage2<K, K / 2>::run(v0, v1, e);
+    if constexpr (K < S) bitonic_all2<K * 2, S>(v0, v1, e);
+}
+
 // widths 8..64: 64/S sites per wave, one record per lane
 template <int S>
 __global__ __launch_bounds__(256) void k_sort_lanes(uint32_t* rec, const int32_t* nrec, const uint32_t* list,
@@ -727,32 +757,41 @@ __global__ __launch_bounds__(256) void k_fused(FusedArgs f, const uint32_t* list
             const uint32_t sub = lane / S, e = lane % S, gbase = lane & ~(uint32_t)(S - 1);
             const uint64_t gmask = (S == 64 ? ~0ull : ((1ull << S) - 1ull)) << gbase;  // lanes of my group
             const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-            uint32_t v[NP];
+            // in chunks of CH passes (bounds the registers of the wide class)
+            constexpr uint32_t CH = NP < 32 ? NP : 32;
 #pragma unroll
-            for (uint32_t k = 0; k < NP; k++) {
-                const uint32_t gi = k * P + sub;
-                const uint32_t st = (uint32_t)__shfl((int)site, (int)gi);
-                const uint32_t nn = (uint32_t)__shfl((int)n, (int)gi);
-                v[k] = e < nn ? __builtin_nontemporal_load(f.a.rec + (size_t)st * kSlots + e) : 0xFFFFFFFFu;
-            }
+            for (uint32_t k0 = 0; k0 < NP; k0 += CH) {
+                uint32_t v[CH];
 #pragma unroll
-            for (uint32_t k = 0; k < NP; k++) {
-                const uint32_t gi = k * P + sub;
-                const uint32_t nn = (uint32_t)__shfl((int)n, (int)gi);
-                bitonic_all<2, S>(v[k], e);
-                // earlier records of the same key: the lanes of my group whose
-                // key agrees with mine on all eleven key bits (one ballot per
-                // bit), below my lane; padding lanes (e >= nn) take no part
-                uint64_t peers = __ballot(e < nn) & gmask;
-#pragma unroll
-                for (int b = 0; b < 11; b++) {
-                    const uint32_t bit = b == 0 ? 8u : b < 9 ? (uint32_t)(8 + b) : (uint32_t)(14 + b);  // 8, 9..16, 23, 24
-                    const bool one = (v[k] >> bit) & 1u;
-                    const uint64_t bal = __ballot(one);
-                    peers &= one ? bal : ~bal;
+                for (uint32_t k = 0; k < CH; k++) {
+                    const uint32_t gi = (k0 + k) * P + sub;
+                    const uint32_t st = (uint32_t)__shfl((int)site, (int)gi);
+                    const uint32_t nn = (uint32_t)__shfl((int)n, (int)gi);
+                    v[k] = e < nn ? __builtin_nontemporal_load(f.a.rec + (size_t)st * kSlots + e) : 0xFFFFFFFFu;
                 }
-                const uint32_t t = (uint32_t)__popcll(peers & below);
-                if (e < nn) srt[gi * RS + e] = (v[k] & ~0xFFu) | t;
+#pragma unroll
+                for (uint32_t k = 0; k < CH; k += 2) bitonic_all2<2, S>(v[k], v[k + 1], e);  // CH is even
+#pragma unroll
+                for (uint32_t k = 0; k < CH; k++) {
+                    const uint32_t gi = (k0 + k) * P + sub;
+                    const uint32_t nn = (uint32_t)__shfl((int)n, (int)gi);
+                    // earlier records of the same key: the lanes of my group whose
+                    // key agrees with mine on all eleven key bits (one ballot per
+                    // bit), below my lane; padding lanes (e >= nn) take no part
+                    // (32-bit halves, the ballot xor my bit replicated: four VALU per bit)
+                    const uint64_t live = __ballot(e < nn) & gmask & below;
+                    uint32_t plo = (uint32_t)live, phi = (uint32_t)(live >> 32);
+#pragma unroll
+                    for (int b = 0; b < 11; b++) {
+                        const uint32_t bit = b == 0 ? 8u : b < 9 ? (uint32_t)(8 + b) : (uint32_t)(14 + b);  // 8, 9..16, 23, 24
+                        const uint32_t m = 0u - ((v[k] >> bit) & 1u);  // all ones where my bit is set
+                        const uint64_t bal = __ballot(m != 0u);
+                        plo &= ~((uint32_t)bal ^ m);  // lanes whose bit equals mine
+                        phi &= ~((uint32_t)(bal >> 32) ^ m);
+                    }
+                    const uint32_t t = (uint32_t)(__popc(plo) + __popc(phi));
+                    if (e < nn) srt[gi * RS + e] = (v[k] & ~0xFFu) | t;
+                }
             }
         } else {
             constexpr int R = S / 64;
