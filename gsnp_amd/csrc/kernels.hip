@@ -686,25 +686,30 @@ void launch_posterior(hipStream_t st, const Tables& t, const RefView& ref, uint3
 //  1. Sort: every record slot of the group is loaded first (the loads are in
 //     flight together), then for S <= 64 lane groups of S sort 64/S sites per
 //     pass with the k_sort_lanes networks, for S = 128/256 one site per pass
-//     with S/64 records per lane.  Each sorted record then learns its
-//     dependency count t - 1: how many records before it in sorted order
-//     share its (base, strand, cycle) key -- the same base means the same
-//     segment.  For S <= 64 one ballot per key bit gives each lane the set of
-//     lanes in its group holding its key, and t - 1 is the count of those
-//     below it; for the wide classes the lanes scan the parked row.  t - 1 (< 255)
-//     replaces the record's occurrence field (always 1 from k_site_build) and
-//     the sorted rows are parked in LDS as [G][S+1] words.
-//  2. One lane per site walks its row in record order: g counts the
-//     segment's first occurrences, (q, t, g) indexes the adjusted-quality
-//     table and the ten score values of (q_adj, cycle, base) -- one 128-byte
-//     line -- are added in that order, as in k_likelihood; two records per
-//     step so that their table loads are in flight together (four or eight
-//     cost occupancy and measured slower, as did spreading a row's loads over
-//     five lanes).  Records are read once, with non-temporal loads, so they
-//     do not push the score table out of L2.
-//  3. posterior_site calls the site from the sums held in registers.
+//     with S/64 records per lane.  Each sorted record then learns, in
+//     parallel (kernel/likelihood_kernel.cu:168-256 walks them in order):
+//       t - 1: how many records before it share its (base, strand, cycle)
+//              key -- the same base means the same segment;
+//       g:     how many records of its base segment up to and including it
+//              are first occurrences (t = 1), minus one.
+//     For S <= 64 one ballot per key bit gives each lane the lanes of its
+//     site holding its key (all eleven bits) and its base (the two base
+//     bits), and one more ballot the first occurrences; for the wide classes
+//     the lanes scan the parked row.  (q, t - 1, g) indexes the adjusted-
+//     quality table and the record is parked in LDS ([G][RS] words) as its
+//     score row (q_adj << 10 | cycle << 2 | base).
+//  2. Five lanes per site, two genotypes each, add the site's score values
+//     in record order: each load instruction reads whole 80-byte rows of
+//     twelve sites (five contiguous lanes per row, one line) instead of one
+//     16-byte piece of 64 rows, which divides the address work of the
+//     gathers by five.
+//     Records are read once, with non-temporal loads, so they do not push
+//     the score table out of L2.
+//  3. The ten sums of a site go through its LDS row to the lane that owns
+//     the site; posterior_site calls it from registers.
 
 constexpr uint32_t kRecKeyMask = 0x0181FF00u;  // base, cycle, strand fields of a record
+constexpr uint32_t kRecBaseMask = 0x01800000u;
 
 // G: sites per wave group.  The wide classes hold few sites, so their groups
 // are small to spread them over more waves.
@@ -715,11 +720,11 @@ constexpr uint32_t fused_group()
 }
 
 // LDS words per site row: odd, so that row walks of the 64 lanes are
-// conflict-free
+// conflict-free, and at least 20 so that a row can hand over the ten sums
 template <int S>
 constexpr uint32_t fused_row_stride()
 {
-    return (uint32_t)S + 1u;
+    return S + 1 < 21 ? 21u : (uint32_t)S + 1u;
 }
 
 __device__ __forceinline__ void wave_lds_sync()
@@ -727,6 +732,31 @@ __device__ __forceinline__ void wave_lds_sync()
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// (q, t - 1, g) table index << 10 | cycle << 2 | base of a sorted record
+__device__ __forceinline__ uint32_t qadj_slot(uint32_t w, uint32_t t, uint32_t g, uint32_t td, uint32_t gd,
+                                              uint32_t& bad)
+{
+    uint32_t q = rec_q(w);
+    if (q > 63) {
+        bad |= kErrQadjRange;
+        q = 63;
+    }
+    const uint32_t tm1 = t < td ? t : td - 1;
+    const uint32_t gq = g < gd ? g : gd - 1;
+    return (((q * td + tm1) * gd + gq) << 10) | (rec_cycle(w) << 2) | rec_base(w);  // index < 2^22
+}
+
+// score row of a qadj_slot value
+__device__ __forceinline__ uint32_t score_row(const uint8_t* qadj, uint32_t slot, uint32_t& bad)
+{
+    uint32_t qa = qadj[slot >> 10];
+    if (qa == 0) {
+        bad |= kErrQadjRange;
+        qa = 1;
+    }
+    return (qa << 10) | (slot & 1023u);
 }
 
 template <int S>
@@ -743,6 +773,10 @@ __global__ __launch_bounds__(256) void k_fused(FusedArgs f, const uint32_t* list
     const uint32_t wave = blockIdx.x * (blockDim.x >> 6) + wib;
     const uint32_t nwave = gridDim.x * (blockDim.x >> 6);
     const uint32_t td = (uint32_t)f.t.td, gd = (uint32_t)f.t.gd;
+    // phase-2 roles: lane = 5 * site slot + 16-byte piece of the row (lanes 60..63 idle)
+    const uint32_t sl = lane / 5u, cl = lane - 5u * sl;
+    const char* score = reinterpret_cast<const char*>(f.t.score);
+    uint32_t bad = 0;
     for (uint32_t grp = wave; grp < ngroups; grp += nwave) {
         const uint32_t g0 = grp * G;
         const uint32_t gcount = count - g0 < G ? count - g0 : G;
@@ -751,12 +785,13 @@ __global__ __launch_bounds__(256) void k_fused(FusedArgs f, const uint32_t* list
         const uint32_t site = act ? list[g0 + lane] : 0u;
         const uint32_t n = act ? (uint32_t)f.a.nrec[site] : 0u;
         wave_lds_sync();  // the previous group's rows have been read
-        // phase 1: sort, dependency counts, park in LDS
+        // phase 1: sort, dependency counts, score rows parked in LDS
         if constexpr (S <= 64) {
             constexpr uint32_t P = 64 / S, NP = G / P;
             const uint32_t sub = lane / S, e = lane % S, gbase = lane & ~(uint32_t)(S - 1);
             const uint64_t gmask = (S == 64 ? ~0ull : ((1ull << S) - 1ull)) << gbase;  // lanes of my group
             const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+            const uint64_t upto = below | (1ull << lane);
             // in chunks of CH passes (bounds the registers of the wide class)
             constexpr uint32_t CH = NP < 32 ? NP : 32;
 #pragma unroll
@@ -773,25 +808,40 @@ __global__ __launch_bounds__(256) void k_fused(FusedArgs f, const uint32_t* list
                 for (uint32_t k = 0; k < CH; k += 2) bitonic_all2<2, S>(v[k], v[k + 1], e);  // CH is even
 #pragma unroll
                 for (uint32_t k = 0; k < CH; k++) {
-                    const uint32_t gi = (k0 + k) * P + sub;
-                    const uint32_t nn = (uint32_t)__shfl((int)n, (int)gi);
-                    // earlier records of the same key: the lanes of my group whose
-                    // key agrees with mine on all eleven key bits (one ballot per
-                    // bit), below my lane; padding lanes (e >= nn) take no part
-                    // (32-bit halves, the ballot xor my bit replicated: four VALU per bit)
-                    const uint64_t live = __ballot(e < nn) & gmask & below;
-                    uint32_t plo = (uint32_t)live, phi = (uint32_t)(live >> 32);
+                    // the lanes of my site that agree with me on all eleven key
+                    // bits (one ballot per bit, 32-bit halves, the ballot xor my
+                    // bit replicated: four VALU per bit), below me; and those
+                    // with my base, up to me; padding lanes (all ones: a record's
+                    // occurrence byte is 1) take no part
+                    const bool my = v[k] != 0xFFFFFFFFu;
+                    const uint64_t live = __ballot(my) & gmask;
+                    uint32_t plo = (uint32_t)(live & below), phi = (uint32_t)((live & below) >> 32);
+                    uint32_t olo = (uint32_t)(live & upto), ohi = (uint32_t)((live & upto) >> 32);
 #pragma unroll
                     for (int b = 0; b < 11; b++) {
                         const uint32_t bit = b == 0 ? 8u : b < 9 ? (uint32_t)(8 + b) : (uint32_t)(14 + b);  // 8, 9..16, 23, 24
                         const uint32_t m = 0u - ((v[k] >> bit) & 1u);  // all ones where my bit is set
                         const uint64_t bal = __ballot(m != 0u);
-                        plo &= ~((uint32_t)bal ^ m);  // lanes whose bit equals mine
-                        phi &= ~((uint32_t)(bal >> 32) ^ m);
+                        const uint32_t eql = ~((uint32_t)bal ^ m), eqh = ~((uint32_t)(bal >> 32) ^ m);
+                        plo &= eql;  // lanes whose bit equals mine
+                        phi &= eqh;
+                        if (b >= 9) {
+                            olo &= eql;
+                            ohi &= eqh;
+                        }
                     }
                     const uint32_t t = (uint32_t)(__popc(plo) + __popc(phi));
-                    if (e < nn) srt[gi * RS + e] = (v[k] & ~0xFFu) | t;
+                    const uint64_t first = __ballot(my && t == 0);
+                    const uint32_t g =
+                        (uint32_t)(__popc(olo & (uint32_t)first) + __popc(ohi & (uint32_t)(first >> 32))) - 1u;
+                    // (all ones stays free: t - 1 = 255 and g = 255 exclude each other)
+                    uint32_t b2 = 0;
+                    v[k] = my ? qadj_slot(v[k], t, g, td, gd, b2) : 0xFFFFFFFFu;
+                    bad |= b2;
                 }
+#pragma unroll
+                for (uint32_t k = 0; k < CH; k++)
+                    if (v[k] != 0xFFFFFFFFu) srt[((k0 + k) * P + sub) * RS + e] = score_row(f.t.qadj, v[k], bad);
             }
         } else {
             constexpr int R = S / 64;
@@ -820,8 +870,8 @@ __global__ __launch_bounds__(256) void k_fused(FusedArgs f, const uint32_t* list
                 }
             }
             wave_lds_sync();
-            // dependency counts from the parked rows (keys are not touched by
-            // the low-byte updates, so the scan reads stable keys)
+            // t - 1 from the parked rows (the scan reads keys only), into the
+            // low byte of the record (its occurrence field, always 1)
 #pragma unroll
             for (uint32_t gi = 0; gi < G; gi++) {
                 const uint32_t nn = (uint32_t)__shfl((int)n, (int)gi);
@@ -840,81 +890,103 @@ __global__ __launch_bounds__(256) void k_fused(FusedArgs f, const uint32_t* list
 #pragma unroll
                 for (int r = 0; r < R; r++) {
                     const uint32_t e = (uint32_t)r * 64u + lane;
-                    if (e < nn) srt[gi * RS + e] = (v[gi][r] & ~0xFFu) | t[r];
+                    v[gi][r] = (v[gi][r] & ~0xFFu) | t[r];
+                    if (e < nn) srt[gi * RS + e] = v[gi][r];
+                }
+            }
+            wave_lds_sync();
+            // g from the rows holding t - 1, then the score rows
+#pragma unroll
+            for (uint32_t gi = 0; gi < G; gi++) {
+                const uint32_t nn = (uint32_t)__shfl((int)n, (int)gi);
+                uint32_t g[R];
+#pragma unroll
+                for (int r = 0; r < R; r++) g[r] = 0;
+                for (uint32_t j = 0; j < nn; j++) {
+                    const uint32_t wj = srt[gi * RS + j];
+                    const uint32_t first_base = (wj & 0xFFu) == 0 ? (wj & kRecBaseMask) : 0xFFFFFFFFu;
+#pragma unroll
+                    for (int r = 0; r < R; r++) {
+                        const uint32_t e = (uint32_t)r * 64u + lane;
+                        g[r] += (j <= e && first_base == (v[gi][r] & kRecBaseMask)) ? 1u : 0u;
+                    }
+                }
+#pragma unroll
+                for (int r = 0; r < R; r++) {
+                    const uint32_t e = (uint32_t)r * 64u + lane;
+                    uint32_t b2 = 0;
+                    v[gi][r] = qadj_slot(v[gi][r], v[gi][r] & 0xFFu, g[r] - 1u, td, gd, b2);
+                    if (e < nn) bad |= b2;
+                }
+            }
+            wave_lds_sync();
+#pragma unroll
+            for (uint32_t gi = 0; gi < G; gi++) {
+                const uint32_t nn = (uint32_t)__shfl((int)n, (int)gi);
+#pragma unroll
+                for (int r = 0; r < R; r++) {
+                    const uint32_t e = (uint32_t)r * 64u + lane;
+                    if (e < nn) srt[gi * RS + e] = score_row(f.t.qadj, v[gi][r], bad);
                 }
             }
         }
         wave_lds_sync();
-        // phase 2: one lane per site, records in sorted order; g counts the
-        // first occurrences of the base segment, t - 1 sits in the low byte
-        const uint32_t* row = srt + lane * RS;
-        double acc[10];
+        // phase 2: twelve sites at a time, lane (sl, cl) adds genotypes 2cl and
+        // 2cl+1 of site r * 12 + sl in record order, 16 records in flight; the
+        // sums replace words 4cl..4cl+3 of the site's row once every lane is
+        // past the row
+        constexpr uint32_t U = 16;
+#pragma unroll 1
+        for (uint32_t r = 0; r < (G + 11) / 12; r++) {
+            const uint32_t gi = r * 12u + sl;
+            const uint32_t ng = (uint32_t)__shfl((int)n, (int)(gi & 63u));
+            const bool on = lane < 60u && gi < gcount;
+            const uint32_t m = on ? ng : 0u;
+            const uint32_t* rw = srt + (on ? gi : 0u) * RS;
+            double a0 = 0.0, a1 = 0.0;
+            // i + u < i + U <= S rounded up to U: inside the row (RS > S, RS >= 21);
+            // slots past the site's records load row 0 and are not added
+            for (uint32_t i = 0; i < m; i += U) {
+                double2 x[U];
 #pragma unroll
-        for (int g = 0; g < 10; g++) acc[g] = 0.0;
-        int last_o = -1, gdc = -1;
-        uint32_t bad = 0;
-        for (uint32_t i = 0; i < n; i += 2) {
-            uint32_t qi[2], cyo[2];
-#pragma unroll
-            for (int j = 0; j < 2; j++) {
-                const uint32_t w = i + j < n ? row[i + j] : 0u;
-                const uint32_t o = rec_base(w);
-                uint32_t q = rec_q(w);
-                if (i + j < n) {
-                    if ((int)o != last_o) {
-                        gdc = -1;
-                        last_o = (int)o;
-                    }
-                    if ((w & 0xFFu) == 0) gdc += 1;
-                    if (q > 63) {
-                        bad |= kErrQadjRange;
-                        q = 63;
-                    }
+                for (uint32_t u = 0; u < U; u++) {
+                    const uint32_t ri = rw[i + u];
+                    x[u] = *reinterpret_cast<const double2*>(score + (((i + u < m ? ri : 0u) << 7) | (cl << 4)));
                 }
-                const uint32_t prior = w & 0xFFu;
-                const uint32_t tm1 = prior < td ? prior : td - 1;
-                const uint32_t gq = (uint32_t)gdc < gd ? (uint32_t)gdc : gd - 1;
-                qi[j] = (q * td + tm1) * gd + gq;
-                cyo[j] = (rec_cycle(w) << 2) | o;
+#pragma unroll
+                for (uint32_t u = 0; u < U; u++)
+                    if (i + u < m) {
+                        a0 += x[u].x;
+                        a1 += x[u].y;
+                    }
             }
-            uint32_t qa[2];
-#pragma unroll
-            for (int j = 0; j < 2; j++) qa[j] = i + j < n ? f.t.qadj[qi[j]] : 1u;
-            double2 v[2][5];
-#pragma unroll
-            for (int j = 0; j < 2; j++) {
-                uint32_t a = qa[j];
-                if (a == 0) {
-                    bad |= kErrQadjRange;
-                    a = 1;
-                }
-                const double2* sr =
-                    reinterpret_cast<const double2*>(f.t.score + (size_t)((a << 10) | cyo[j]) * kScoreRowStride);
-#pragma unroll
-                for (int h = 0; h < 5; h++) v[j][h] = i + j < n ? sr[h] : make_double2(0.0, 0.0);
-            }
-#pragma unroll
-            for (int j = 0; j < 2; j++) {
-                if (i + j < n) {
-#pragma unroll
-                    for (int h = 0; h < 5; h++) {
-                        acc[2 * h] += v[j][h].x;
-                        acc[2 * h + 1] += v[j][h].y;
-                    }
-                }
+            wave_lds_sync();
+            if (on) {
+                const uint64_t b0 = (uint64_t)__double_as_longlong(a0), b1 = (uint64_t)__double_as_longlong(a1);
+                uint32_t* d = srt + gi * RS + 4u * cl;
+                d[0] = (uint32_t)b0;
+                d[1] = (uint32_t)(b0 >> 32);
+                d[2] = (uint32_t)b1;
+                d[3] = (uint32_t)(b1 >> 32);
             }
         }
-        if (bad) atomicOr(f.err, bad);
+        wave_lds_sync();
         if (!act) continue;
+        // phase 3: posterior from the sums
+        double acc[10];
+#pragma unroll
+        for (int g = 0; g < 10; g++)
+            acc[g] = __longlong_as_double((long long)(((uint64_t)srt[lane * RS + 2 * g + 1] << 32) |
+                                                      srt[lane * RS + 2 * g]));
         if (f.write_lk) {
             double2* out = reinterpret_cast<double2*>(f.out.lk + (size_t)site * 10);
 #pragma unroll
             for (int h = 0; h < 5; h++) out[h] = make_double2(acc[2 * h], acc[2 * h + 1]);
         }
-        // phase 3: posterior from the sums in registers
         posterior_site(f.t, f.ref, f.ws, f.a, site, [&](int g) { return acc[g]; }, f.head_qs, f.out, f.jobs,
                        f.job_count);
     }
+    if (bad) atomicOr(f.err, bad);
 }
 
 template <int S>
