@@ -307,26 +307,28 @@ __host__ __device__ __noinline__ inline int fmt_g_exact(double v, char* o)
 // y < 2^20), and the digit rounding is exact unless y sits within that error
 // of a half, which goes to the exact routine.
 // The cases settled without the exact routine; -1 for the others.
+// The six rounded digits D and the exponent X (0..5) of v's %#g text when the
+// FP64 path settles them; false for the others.
+GSNP_HD bool g_fast_digits(double v, uint32_t& D, int& X)
+{
+    if (!(v >= 1.0 && v < 1e6)) return false;
+    X = v < 10.0 ? 0 : v < 100.0 ? 1 : v < 1e3 ? 2 : v < 1e4 ? 3 : v < 1e5 ? 4 : 5;
+    const double scale = X == 0 ? 1e5 : X == 1 ? 1e4 : X == 2 ? 1e3 : X == 3 ? 1e2 : X == 4 ? 1e1 : 1.0;
+    const double y = v * scale;
+    const double fl = floor(y);
+    const double frac = y - fl;  // exact: y < 2^20
+    if (!(fabs(frac - 0.5) > 2.5e-10)) return false;
+    const uint64_t d = (uint64_t)fl + (frac > 0.5 ? 1u : 0u);
+    if (d >= 1000000ull) return false;
+    D = (uint32_t)d;
+    return true;
+}
+
 GSNP_HD int fmt_g_fast(double v, char* o)
 {
-    // the two values most .cns rows carry (rank-sum 1, copy number 1 or 255)
-    if (v == 1.0 || v == 255.0) {
-        const char* s = v == 1.0 ? "1.00000" : "255.000";
-        for (int i = 0; i < 7; i++) o[i] = s[i];
-        return 7;
-    }
-    if (v >= 1.0 && v < 1e6) {
-        const int X = v < 10.0 ? 0 : v < 100.0 ? 1 : v < 1e3 ? 2 : v < 1e4 ? 3 : v < 1e5 ? 4 : 5;
-        const double scale = X == 0 ? 1e5 : X == 1 ? 1e4 : X == 2 ? 1e3 : X == 3 ? 1e2 : X == 4 ? 1e1 : 1.0;
-        const double y = v * scale;
-        const double fl = floor(y);
-        const double frac = y - fl;  // exact: y < 2^20
-        if (fabs(frac - 0.5) > 2.5e-10) {
-            uint64_t D = (uint64_t)fl + (frac > 0.5 ? 1u : 0u);
-            if (D < 1000000ull) return fmt_g_digits(D, X, o, 0);
-        }
-    }
-    return -1;
+    uint32_t D;
+    int X;
+    return g_fast_digits(v, D, X) ? fmt_g_digits(D, X, o, 0) : -1;
 }
 
 GSNP_HD int fmt_g(double v, char* o)

@@ -1232,6 +1232,59 @@ constexpr int kNRowLen = sizeof(kNRow) - 1;
 __constant__ char kACTGN[] = "ACTGNNNN";
 __constant__ char kAbbv[] = "AMWRMCYSWYTKRSKGN";
 
+// Row output: characters go through put(pos, c), each digit straight to its
+// position (a character array read at run-time indexes would compile to
+// compare/select chains over the whole array).
+__device__ __forceinline__ int digits_u32(uint32_t v)
+{
+    return v < 10u ? 1 : v < 100u ? 2 : v < 1000u ? 3 : v < 10000u ? 4 : v < 100000u ? 5 : v < 1000000u ? 6
+         : v < 10000000u ? 7 : v < 100000000u ? 8 : v < 1000000000u ? 9 : 10;
+}
+
+template <class Put>
+__device__ __forceinline__ int put_u32(uint32_t v, int n, Put& put)
+{
+    const int k = digits_u32(v);
+    for (int p = n + k - 1; p >= n; p--) {
+        put(p, (char)('0' + v % 10u));
+        v /= 10u;
+    }
+    return n + k;
+}
+
+template <class Put>
+__device__ __forceinline__ int put_i32(int v, int n, Put& put)
+{
+    if (v < 0) {
+        put(n, '-');
+        return put_u32(0u - (uint32_t)v, n + 1, put);
+    }
+    return put_u32((uint32_t)v, n, put);
+}
+
+// %#g text: six digits with the point after digit X (fmt_g_digits for
+// 0 <= X <= 5), or the pre-pass text of the site (rank field 0, copy number
+// 1) when the FP64 path does not settle it
+template <class Put>
+__device__ __forceinline__ int put_g(double v, const RowCtx& rc, uint32_t site, int field, int n, Put& put)
+{
+    uint32_t D;
+    int X;
+    if (g_fast_digits(v, D, X)) {
+        for (int i = 5; i >= 0; i--) {
+            put(n + i + (i > X ? 1 : 0), (char)('0' + D % 10u));
+            D /= 10u;
+        }
+        put(n + X + 1, '.');
+        return n + 7;
+    }
+    const GText& gt = rc.gtext[rc.gidx[site]];
+    const char* t = field == 0 ? gt.rank : gt.copy;
+    const int k = t[15];
+    for (int i = 0; i < k; i++) put(n + i, t[i]);
+    return n + k;
+}
+
 // Writes the row through `put(pos, char)`; returns its length.
 template <class Put>
 __device__ __forceinline__ int emit_row(const RowData& d, const RowCtx& rc, uint32_t pos1, Put put)
@@ -1239,65 +1292,50 @@ __device__ __forceinline__ int emit_row(const RowData& d, const RowCtx& rc, uint
     int n = 0;
     for (int i = 0; i < rc.name_len; i++) put(n++, rc.name[i]);
     put(n++, '\t');
-    char buf[24];
-    int k = fmt_u32(pos1, buf);
-    for (int i = 0; i < k; i++) put(n++, buf[i]);
+    n = put_u32(pos1, n, put);
     if (d.nrow) {
-        for (int i = 0; i < kNRowLen; i++) put(n++, kNRow[i]);
-        return n;
+#pragma unroll
+        for (int i = 0; i < kNRowLen; i++) put(n + i, kNRow[i]);
+        return n + kNRowLen;
     }
     put(n++, '\t');
     put(n++, kACTGN[d.ori & 7]);
     put(n++, '\t');
     put(n++, kAbbv[d.type1]);
     put(n++, '\t');
-    k = fmt_i32(d.q_cns, buf);
-    for (int i = 0; i < k; i++) put(n++, buf[i]);
+    n = put_i32(d.q_cns, n, put);
     put(n++, '\t');
     put(n++, kACTGN[d.b1]);
-    const int v1[3] = {d.avg1, d.cu1, d.ca1};
-    for (int j = 0; j < 3; j++) {
-        put(n++, '\t');
-        k = fmt_i32(v1[j], buf);
-        for (int i = 0; i < k; i++) put(n++, buf[i]);
-    }
+    put(n++, '\t');
+    n = put_i32(d.avg1, n, put);
+    put(n++, '\t');
+    n = put_i32(d.cu1, n, put);
+    put(n++, '\t');
+    n = put_i32(d.ca1, n, put);
     put(n++, '\t');
     if (d.b2 < 4) {
         put(n++, kACTGN[d.b2]);
-        const int v2[3] = {d.avg2, d.cu2, d.ca2};
-        for (int j = 0; j < 3; j++) {
-            put(n++, '\t');
-            k = fmt_i32(v2[j], buf);
-            for (int i = 0; i < k; i++) put(n++, buf[i]);
-        }
+        put(n++, '\t');
+        n = put_i32(d.avg2, n, put);
+        put(n++, '\t');
+        n = put_i32(d.cu2, n, put);
+        put(n++, '\t');
+        n = put_i32(d.ca2, n, put);
     } else {
         put(n++, 'N');
-        for (int j = 0; j < 3; j++) {
-            put(n++, '\t');
-            put(n++, '0');
-        }
+        put(n++, '\t');
+        put(n++, '0');
+        put(n++, '\t');
+        put(n++, '0');
+        put(n++, '\t');
+        put(n++, '0');
     }
     put(n++, '\t');
-    k = fmt_i32(d.depth, buf);
-    for (int i = 0; i < k; i++) put(n++, buf[i]);
+    n = put_i32(d.depth, n, put);
     put(n++, '\t');
-    k = fmt_g_fast(d.rank, buf);
-    if (k < 0) {
-        const char* t = rc.gtext[rc.gidx[d.site]].rank;
-        k = t[15];
-        for (int i = 0; i < k; i++) put(n++, t[i]);
-    } else {
-        for (int i = 0; i < k; i++) put(n++, buf[i]);
-    }
+    n = put_g(d.rank, rc, d.site, 0, n, put);
     put(n++, '\t');
-    k = fmt_g_fast(copy_number(d.depth, d.repeat), buf);
-    if (k < 0) {
-        const char* t = rc.gtext[rc.gidx[d.site]].copy;
-        k = t[15];
-        for (int i = 0; i < k; i++) put(n++, t[i]);
-    } else {
-        for (int i = 0; i < k; i++) put(n++, buf[i]);
-    }
+    n = put_g(copy_number(d.depth, d.repeat), rc, d.site, 1, n, put);
     put(n++, '\t');
     put(n++, (d.ori & 8) ? '1' : '0');
     put(n++, '\n');
