@@ -905,8 +905,9 @@ int Ctx::format_text(uint32_t offset, uint32_t ws_abs, uint32_t n_sites, uint32_
     auto enqueue = [&] {
         timer.begin(kStFormat, stream);
         launch_fill_u32(stream, gcount.p, 1, 0u);
-        launch_gtext(stream, ref_view(), ws_abs, a, res, n_sites, gidx.p + offset, gtext.p, gcount.p);
-        launch_rowlen(stream, ref_view(), ws_abs, a, res, n_sites, rc, rowlen.p, blocksum.p);
+        launch_rowlen(stream, ref_view(), ws_abs, a, res, n_sites, rc, gidx.p + offset, gtext.p, gcount.p, rowlen.p,
+                      blocksum.p);
+        launch_gtext(stream, a, res, n_sites, gtext.p, gcount.p, rowlen.p, blocksum.p);
         launch_scan_blocks(stream, blocksum.p, nb, total.p);
         launch_format(stream, ref_view(), ws_abs, a, res, n_sites, rc, rowlen.p, blocksum.p, text.p);
         timer.end(kStFormat, stream);

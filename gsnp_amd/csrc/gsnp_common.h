@@ -343,6 +343,7 @@ GSNP_HD int fmt_g(double v, char* o)
 struct GText {
     char rank[16];
     char copy[16];
+    uint32_t site;  // the row's site
 };
 
 }  // namespace gsnp

@@ -132,13 +132,15 @@ void launch_rank(hipStream_t s, const Tables& t, const SiteArrays& a, const Rank
                  uint32_t max_jobs, Results& out, FixJob* fixes, uint32_t* fix_count);
 void launch_apply_fix(hipStream_t s, const FixJob* fixes, const double* rank, const int32_t* qcns, uint32_t n,
                       Results& out);
+// row lengths and block sums; rows that need the exact %#g routine are listed
+// in gtext (gcount must be zero on entry) for launch_gtext
 void launch_rowlen(hipStream_t s, const RefView& ref, uint32_t ws, const SiteArrays& a, const Results& res,
-                   uint32_t n_sites, RowCtx rc, uint32_t* rowlen, uint64_t* block_sum);
+                   uint32_t n_sites, RowCtx rc, uint32_t* gidx, GText* gtext, uint32_t* gcount, uint32_t* rowlen,
+                   uint64_t* block_sum);
+// exact %#g text of the listed rows; their lengths are added to rowlen and block_sum
+void launch_gtext(hipStream_t s, const SiteArrays& a, const Results& res, uint32_t n_sites, GText* gtext,
+                  const uint32_t* gcount, uint32_t* rowlen, uint64_t* block_sum);
 void launch_scan_blocks(hipStream_t s, uint64_t* block_sum, uint32_t n_blocks, uint64_t* total);
-// exact %#g text of the rows whose rank-sum or copy number the fast path cannot
-// settle; gcount must be zero on entry
-void launch_gtext(hipStream_t s, const RefView& ref, uint32_t ws, const SiteArrays& a, const Results& res,
-                  uint32_t n_sites, uint32_t* gidx, GText* gtext, uint32_t* gcount);
 void launch_format(hipStream_t s, const RefView& ref, uint32_t ws, const SiteArrays& a, const Results& res,
                    uint32_t n_sites, RowCtx rc, const uint32_t* rowlen, const uint64_t* block_off, char* out);
 void launch_recycle(hipStream_t s, const SiteArrays& a, uint32_t win, uint32_t L);

@@ -1353,14 +1353,29 @@ __device__ __forceinline__ uint64_t block_sum_u64(uint64_t v, uint64_t* red)
     return t;
 }
 
+// Row lengths and per-block sums.  A row whose rank-sum or copy number the
+// fast %#g path cannot settle is listed for k_gtext (gidx: site -> entry,
+// entry.site) and sized without those columns; k_gtext adds them.
 __global__ __launch_bounds__(256) void k_rowlen(RefView ref, uint32_t ws, SiteArrays a, Results res, uint32_t n_sites,
-                                                RowCtx rc, uint32_t* rowlen, uint64_t* block_sum)
+                                                RowCtx rc, uint32_t* gidx, GText* gtext, uint32_t* gcount,
+                                                uint32_t* rowlen, uint64_t* block_sum)
 {
     __shared__ uint64_t red[4];
     const uint32_t s = blockIdx.x * 256u + threadIdx.x;
     uint32_t len = 0;
     if (s < n_sites) {
         const RowData d = load_row(ref, ws, a, res, s);
+        if (!d.nrow) {
+            uint32_t D;
+            int X;
+            if (!g_fast_digits(d.rank, D, X) || !g_fast_digits(copy_number(d.depth, d.repeat), D, X)) {
+                const uint32_t j = atomicAdd(gcount, 1u);
+                gidx[s] = j;
+                gtext[j].site = s;
+                gtext[j].rank[15] = 0;
+                gtext[j].copy[15] = 0;
+            }
+        }
         len = (uint32_t)emit_row(d, rc, rc.pos_base + s, [](int, char) {});
         rowlen[s] = len;
     }
@@ -1369,38 +1384,51 @@ __global__ __launch_bounds__(256) void k_rowlen(RefView ref, uint32_t ws, SiteAr
 }
 
 void launch_rowlen(hipStream_t st, const RefView& ref, uint32_t ws, const SiteArrays& a, const Results& res,
-                   uint32_t n_sites, RowCtx rc, uint32_t* rowlen, uint64_t* block_sum)
+                   uint32_t n_sites, RowCtx rc, uint32_t* gidx, GText* gtext, uint32_t* gcount, uint32_t* rowlen,
+                   uint64_t* block_sum)
 {
-    const uint32_t blocks = (n_sites + 255) / 256;
+    const uint32_t blocks = (n_sites + kRowBlock - 1) / kRowBlock;
     if (blocks)
-        hipLaunchKernelGGL(k_rowlen, dim3(blocks), dim3(256), 0, st, ref, ws, a, res, n_sites, rc, rowlen, block_sum);
+        hipLaunchKernelGGL(k_rowlen, dim3(blocks), dim3(kRowBlock), 0, st, ref, ws, a, res, n_sites, rc, gidx, gtext,
+                           gcount, rowlen, block_sum);
 }
 
-// exclusive scan of block sums in place (one block of 1024 threads); writes the total
+// exclusive scan of block sums in place (one block of 1024 threads, 16
+// consecutive values per thread per tile); writes the total
 __global__ __launch_bounds__(1024) void k_scan_blocks(uint64_t* v, uint32_t n, uint64_t* total)
 {
+    constexpr uint32_t K = 16;
     __shared__ uint64_t wsum[16];
-    __shared__ uint64_t carry;
-    if (threadIdx.x == 0) carry = 0;
-    __syncthreads();
     const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
-    for (uint32_t base = 0; base < n; base += 1024) {
-        const uint32_t i = base + threadIdx.x;
-        const uint64_t x = i < n ? v[i] : 0;
-        uint64_t incl = x;
+    uint64_t carry = 0;  // the same in every thread
+    for (uint32_t base = 0; base < n; base += 1024u * K) {
+        const uint32_t i0 = base + threadIdx.x * K;
+        uint64_t x[K], sum = 0;
+#pragma unroll
+        for (uint32_t k = 0; k < K; k++) {
+            x[k] = i0 + k < n ? v[i0 + k] : 0;
+            sum += x[k];
+        }
+        uint64_t incl = sum;
         for (int off = 1; off < 64; off <<= 1) {
             const uint64_t y = __shfl_up(incl, off);
             if ((int)lane >= off) incl += y;
         }
         if (lane == 63) wsum[w] = incl;
         __syncthreads();
-        uint64_t before = 0;
-        for (uint32_t k = 0; k < w; k++) before += wsum[k];
-        const uint64_t c = carry;
-        if (i < n) v[i] = c + before + incl - x;
-        __syncthreads();
-        if (threadIdx.x == 1023) carry = c + before + incl;
-        __syncthreads();
+        uint64_t before = 0, tile = 0;
+        for (uint32_t k = 0; k < 16; k++) {
+            if (k < w) before += wsum[k];
+            tile += wsum[k];
+        }
+        uint64_t run = carry + before + incl - sum;
+#pragma unroll
+        for (uint32_t k = 0; k < K; k++) {
+            if (i0 + k < n) v[i0 + k] = run;
+            run += x[k];
+        }
+        carry += tile;
+        __syncthreads();  // wsum is rewritten by the next tile
     }
     if (threadIdx.x == 0) *total = carry;
 }
@@ -1410,31 +1438,34 @@ void launch_scan_blocks(hipStream_t st, uint64_t* block_sum, uint32_t n_blocks, 
     hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(1024), 0, st, block_sum, n_blocks, total);
 }
 
-__global__ __launch_bounds__(256) void k_gtext(RefView ref, uint32_t ws, SiteArrays a, Results res, uint32_t n_sites,
-                                               uint32_t* gidx, GText* gtext, uint32_t* gcount)
+// Exact %#g text of the listed rows (k_rowlen), and the lengths of the
+// columns that use it added to the row and its block.
+__global__ __launch_bounds__(256) void k_gtext(SiteArrays a, Results res, GText* gtext, const uint32_t* gcount,
+                                               uint32_t* rowlen, uint64_t* block_sum)
 {
-    const uint32_t s = blockIdx.x * 256u + threadIdx.x;
-    if (s >= n_sites) return;
-    const uint32_t ori = ref.code[ws + s] & 0xF;
-    const int depth = a.depth[s];
-    if ((ori & 4) && depth == 0) return;  // N row literal
-    if (res.base1[s] >= 4) return;        // N row literal
-    const double rank = res.rank[s], copy = copy_number(depth, a.repeat[s]);
-    char buf[24];
-    if (fmt_g_fast(rank, buf) >= 0 && fmt_g_fast(copy, buf) >= 0) return;
-    const uint32_t j = atomicAdd(gcount, 1u);
-    GText& g = gtext[j];
-    g.rank[15] = (char)fmt_g(rank, g.rank);
-    g.copy[15] = (char)fmt_g(copy, g.copy);
-    gidx[s] = j;
+    const uint32_t cnt = *gcount;
+    for (uint32_t j = blockIdx.x * 256u + threadIdx.x; j < cnt; j += gridDim.x * 256u) {
+        GText& g = gtext[j];
+        const uint32_t s = g.site;
+        const double rank = res.rank[s], copy = copy_number(a.depth[s], a.repeat[s]);
+        const int lr = fmt_g(rank, g.rank), lc = fmt_g(copy, g.copy);
+        g.rank[15] = (char)lr;
+        g.copy[15] = (char)lc;
+        uint32_t D;
+        int X;
+        const uint32_t add = (g_fast_digits(rank, D, X) ? 0u : (uint32_t)lr) + (g_fast_digits(copy, D, X) ? 0u : (uint32_t)lc);
+        rowlen[s] += add;
+        atomicAdd(&block_sum[s / kRowBlock], (uint64_t)add);
+    }
 }
 
-void launch_gtext(hipStream_t st, const RefView& ref, uint32_t ws, const SiteArrays& a, const Results& res,
-                  uint32_t n_sites, uint32_t* gidx, GText* gtext, uint32_t* gcount)
+void launch_gtext(hipStream_t st, const SiteArrays& a, const Results& res, uint32_t n_sites, GText* gtext,
+                  const uint32_t* gcount, uint32_t* rowlen, uint64_t* block_sum)
 {
-    const uint32_t blocks = (n_sites + 255) / 256;
+    uint32_t blocks = (n_sites + 255) / 256;
+    if (blocks > 1024) blocks = 1024;
     if (blocks)
-        hipLaunchKernelGGL(k_gtext, dim3(blocks), dim3(256), 0, st, ref, ws, a, res, n_sites, gidx, gtext, gcount);
+        hipLaunchKernelGGL(k_gtext, dim3(blocks), dim3(256), 0, st, a, res, gtext, gcount, rowlen, block_sum);
 }
 
 constexpr uint32_t kFmtLds = 24576;  // 256 rows of up to 96 bytes; larger blocks write directly
