@@ -1473,7 +1473,7 @@ constexpr uint32_t kFmtLds = 24576;  // 256 rows of up to 96 bytes; larger block
 __global__ __launch_bounds__(256) void k_format(RefView ref, uint32_t ws, SiteArrays a, Results res, uint32_t n_sites,
                                                 RowCtx rc, const uint32_t* rowlen, const uint64_t* block_off, char* out)
 {
-    __shared__ __align__(16) char stage[kFmtLds];
+    __shared__ __align__(16) char stage[kFmtLds + 16];
     __shared__ uint32_t wsum[4];
     const uint32_t s = blockIdx.x * 256u + threadIdx.x;
     const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
@@ -1498,18 +1498,26 @@ __global__ __launch_bounds__(256) void k_format(RefView ref, uint32_t ws, SiteAr
             emit_row(d, rc, rc.pos_base + s, [&](int i, char c) { stage[my + i] = c; });
         }
         __syncthreads();
-        // copy stage[0, total) to out[base, base+total): bytes up to 4-byte alignment, then words
-        const uint32_t head = (uint32_t)((4 - (base & 3)) & 3) < total ? (uint32_t)((4 - (base & 3)) & 3) : total;
-        for (uint32_t i = threadIdx.x; i < head; i += 256) out[base + i] = stage[i];
-        const uint32_t nw = (total - head) / 4;
-        uint32_t* dst = reinterpret_cast<uint32_t*>(out + base + head);
-        for (uint32_t i = threadIdx.x; i < nw; i += 256) {
-            const uint32_t o = head + 4 * i;
-            const uint32_t x = (uint32_t)(uint8_t)stage[o] | ((uint32_t)(uint8_t)stage[o + 1] << 8) |
-                               ((uint32_t)(uint8_t)stage[o + 2] << 16) | ((uint32_t)(uint8_t)stage[o + 3] << 24);
-            dst[i] = x;
+        // copy stage[0, total) to out[base, base + total): bytes up to 16-byte
+        // alignment of the destination, then 16-byte stores whose four words
+        // are joined from five aligned LDS words, then the tail bytes
+        const uint32_t mis = (uint32_t)(base & 15u);
+        const uint32_t h = mis == 0 ? 0u : (16u - mis < total ? 16u - mis : total);
+        for (uint32_t i = threadIdx.x; i < h; i += 256) out[base + i] = stage[i];
+        const uint32_t nq = (total - h) / 16u, sh = h & 3u;
+        const uint32_t* sw = reinterpret_cast<const uint32_t*>(stage) + (h >> 2);
+        uint4* dst = reinterpret_cast<uint4*>(out + base + h);
+        for (uint32_t i = threadIdx.x; i < nq; i += 256) {
+            const uint32_t* w = sw + 4u * i;  // w[4] lies at most 3 bytes past the text: inside the stage's slack
+            const uint32_t w0 = w[0], w1 = w[1], w2 = w[2], w3 = w[3], w4 = w[4];
+            uint4 q;
+            q.x = __builtin_amdgcn_alignbyte(w1, w0, sh);
+            q.y = __builtin_amdgcn_alignbyte(w2, w1, sh);
+            q.z = __builtin_amdgcn_alignbyte(w3, w2, sh);
+            q.w = __builtin_amdgcn_alignbyte(w4, w3, sh);
+            dst[i] = q;
         }
-        for (uint32_t i = head + 4 * nw + threadIdx.x; i < total; i += 256) out[base + i] = stage[i];
+        for (uint32_t i = h + 16u * nq + threadIdx.x; i < total; i += 256) out[base + i] = stage[i];
     } else if (s < n_sites) {
         const RowData d = load_row(ref, ws, a, res, s);
         char* o = out + base + my;
