@@ -113,13 +113,17 @@ __global__ __launch_bounds__(kBuildSites) void k_site_build(ReadView r, const ui
         }
     }
     const int64_t p = (int64_t)ws + s;
+    const int64_t p0 = (int64_t)ws + (int64_t)blockIdx.x * kBuildSites;  // the block's first site
+    const int32_t lane = (int32_t)threadIdx.x;
     const bool head = head_qs && live && (p % kWin) == 0;
     bool snapped = false;
     for (uint64_t c0 = b; c0 < e; c0 += kr) {
         const uint32_t cn = (uint32_t)(e - c0 < kr ? e - c0 : kr);
         __syncthreads();  // the previous chunk has been consumed
+        // read starts relative to the block's first site (they lie within
+        // [-(L-1), 63]): the read loop works in 32-bit scalar registers
         for (uint32_t i = threadIdx.x; i < cn; i += kBuildSites) {
-            spos[i] = r.pos[c0 + i];
+            spos[i] = (int32_t)((int64_t)r.pos[c0 + i] - p0);
             smeta[i] = r.meta[c0 + i];
         }
         // bytes [c0*pitch, (c0+cn)*pitch) from the 16-byte boundary below;
@@ -133,16 +137,16 @@ __global__ __launch_bounds__(kBuildSites) void k_site_build(ReadView r, const ui
         for (uint32_t v = threadIdx.x; v < nvec; v += kBuildSites) dst[v] = src[v];
         __syncthreads();
         for (uint32_t j = 0; j < cn; j++) {
-            const int64_t start = spos[j];
-            const uint32_t m = smeta[j];
-            if (head && !snapped && start >= p) {
+            const int32_t start = __builtin_amdgcn_readfirstlane(spos[j]);  // block-uniform
+            const uint32_t m = __builtin_amdgcn_readfirstlane(smeta[j]);
+            if (head && !snapped && start >= lane) {
                 head_qs[p / kWin - ws / kWin] = make_int4((int)(qs & 0xffff), (int)((qs >> 16) & 0xffff),
                                                           (int)((qs >> 32) & 0xffff), (int)(qs >> 48));
                 snapped = true;
             }
-            const int64_t coord = p - start;
+            const int32_t coord = lane - start;
             const uint32_t len = rm_len(m);
-            if (!live || coord < 0 || coord >= (int64_t)len) continue;
+            if (!live || coord < 0 || coord >= (int32_t)len) continue;
             const uint32_t hit = rm_hit(m);
             depth += 1;
             repeat += (int)hit;
