@@ -26,6 +26,29 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, 8 TB/s
+TRAFFIC_PROFILE = ROOT / "profiles" / "calling_path_10Mb_hbm_traffic.md"
+# kernels of each stage as named in the traffic profile
+STAGE_KERNELS = {"fused": ("k_bucket", "k_fused<"), "site_build": ("k_read_ranges", "k_site_build"),
+                 "format": ("k_rowlen", "k_gtext", "k_scan_blocks", "k_format")}
+
+
+def measured_traffic(stage: str):
+    """HBM bytes per pass of a stage on the 10 Mb workload (config 2, scale 1),
+    from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary (measured MB
+    read + written, copy-calibrated); None when the summary is absent."""
+    if stage not in STAGE_KERNELS or not TRAFFIC_PROFILE.exists():
+        return None
+    total = 0.0
+    found = False
+    for line in TRAFFIC_PROFILE.read_text().splitlines():
+        cols = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cols) != 5 or not cols[0].startswith("`"):
+            continue
+        name = cols[0].strip("`")
+        if any(name == k or (k.endswith("<") and name.startswith(k)) for k in STAGE_KERNELS[stage]):
+            total += (float(cols[3]) + float(cols[4])) * 1e6
+            found = True
+    return int(total) if found else None
 
 
 def main() -> int:
@@ -133,8 +156,13 @@ def main() -> int:
     a_bytes = algo.get(dom, 0) / max(1e-9, launches[dom])
     t_launch = stage_ms[dom] / 1e3 / max(1e-9, launches[dom])
     achieved = a_bytes / t_launch / 1e9
+    # traffic: measured HBM bytes per pass of the stage, from the committed
+    # counter runs over this same workload (default launch shape); counters
+    # cannot be read inside the timed run
+    traffic = measured_traffic(dom) if (args.config == 2 and args.scale == 1.0 and args.mode == "batched") else None
     roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "traffic_source": TRAFFIC_PROFILE.relative_to(ROOT).as_posix() if traffic else None,
                 "bytes_per_launch": int(a_bytes), "ms_per_launch": round(t_launch * 1e3, 4)}
     # the calling kernels together at B_site = 4n + 80 per called site
     call_ms = sum(stage_ms.get(k, 0) for k in ("sort", "likelihood", "posterior", "rank"))
