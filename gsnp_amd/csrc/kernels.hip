@@ -696,10 +696,10 @@ void launch_posterior(hipStream_t st, const Tables& t, const RefView& ref, uint3
 //              key -- the same base means the same segment;
 //       g:     how many records of its base segment up to and including it
 //              are first occurrences (t = 1), minus one.
-//     For S <= 64 one ballot per key bit gives each lane the lanes of its
-//     site holding its key (all eleven bits) and its base (the two base
-//     bits), and one more ballot the first occurrences; for the wide classes
-//     the lanes scan the parked row.  (q, t - 1, g) indexes the adjusted-
+//     One ballot per key bit gives each lane the lanes of its site holding
+//     its key (all eleven bits) and its base (the two base bits), and one
+//     more ballot the first occurrences; the wide classes, with S/64 records
+//     per lane, take the ballots of every register.  (q, t - 1, g) indexes the adjusted-
 //     quality table and the record is parked in LDS ([G][RS] words) as its
 //     score row (q_adj << 10 | cycle << 2 | base).
 //  2. Five lanes per site, two genotypes each, add the site's score values
@@ -712,8 +712,6 @@ void launch_posterior(hipStream_t st, const Tables& t, const RefView& ref, uint3
 //  3. The ten sums of a site go through its LDS row to the lane that owns
 //     the site; posterior_site calls it from registers.
 
-constexpr uint32_t kRecKeyMask = 0x0181FF00u;  // base, cycle, strand fields of a record
-constexpr uint32_t kRecBaseMask = 0x01800000u;
 
 // G: sites per wave group.  The wide classes hold few sites, so their groups
 // are small to spread them over more waves.
@@ -761,6 +759,58 @@ __device__ __forceinline__ uint32_t score_row(const uint8_t* qadj, uint32_t slot
         qa = 1;
     }
     return (qa << 10) | (slot & 1023u);
+}
+
+// The same for a site sorted over R registers (record e = r * 64 + lane):
+// ballots of every register per key bit, so that record (r, lane) counts the
+// matching lanes of registers r' < r whole and of register r below it.
+template <int R>
+__device__ __forceinline__ void wide_slots(const uint32_t* v, uint64_t below, uint64_t upto, uint32_t td, uint32_t gd,
+                                           uint32_t& bad, uint32_t* slot)
+{
+    uint64_t live[R], eq[R][R], ob[R][R];  // [mine][theirs], theirs <= mine
+#pragma unroll
+    for (int r = 0; r < R; r++) live[r] = __ballot(v[r] != 0xFFFFFFFFu);
+#pragma unroll
+    for (int r = 0; r < R; r++)
+#pragma unroll
+        for (int q = 0; q <= r; q++) eq[r][q] = ob[r][q] = live[q];
+#pragma unroll
+    for (int b = 0; b < 11; b++) {
+        const uint32_t bit = b == 0 ? 8u : b < 9 ? (uint32_t)(8 + b) : (uint32_t)(14 + b);  // 8, 9..16, 23, 24
+        uint64_t m[R], bal[R];
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            m[r] = 0ull - (uint64_t)((v[r] >> bit) & 1u);
+            bal[r] = __ballot(m[r] != 0ull);
+        }
+#pragma unroll
+        for (int r = 0; r < R; r++)
+#pragma unroll
+            for (int q = 0; q <= r; q++) {
+                eq[r][q] &= ~(bal[q] ^ m[r]);
+                if (b >= 9) ob[r][q] &= ~(bal[q] ^ m[r]);
+            }
+    }
+    uint32_t t[R];
+    uint64_t first[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        uint32_t c = (uint32_t)__popcll(eq[r][r] & below);
+#pragma unroll
+        for (int q = 0; q < r; q++) c += (uint32_t)__popcll(eq[r][q]);
+        t[r] = c;
+        first[r] = __ballot(v[r] != 0xFFFFFFFFu && c == 0);
+    }
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        uint32_t g = (uint32_t)__popcll(ob[r][r] & first[r] & upto);
+#pragma unroll
+        for (int q = 0; q < r; q++) g += (uint32_t)__popcll(ob[r][q] & first[q]);
+        uint32_t b2 = 0;
+        slot[r] = v[r] != 0xFFFFFFFFu ? qadj_slot(v[r], t[r], g - 1u, td, gd, b2) : 0xFFFFFFFFu;
+        bad |= b2;
+    }
 }
 
 template <int S>
@@ -860,78 +910,19 @@ __global__ __launch_bounds__(256) void k_fused(FusedArgs f, const uint32_t* list
                     v[gi][r] = e < nn ? f.a.rec[(size_t)st * kSlots + e] : 0xFFFFFFFFu;
                 }
             }
+            const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+            const uint64_t upto = below | (1ull << lane);
 #pragma unroll
             for (uint32_t gi = 0; gi < G; gi++) {
-                const uint32_t nn = (uint32_t)__shfl((int)n, (int)gi);
 #pragma unroll
                 for (int r = 0; r < R; r++) bitonic_all<2, 64>(v[gi][r], (uint32_t)(r * 64) + lane);
                 regs_merge<R, 128>(v[gi], lane);
                 if constexpr (S == 256) regs_merge<R, 256>(v[gi], lane);
+                uint32_t slot[R];
+                wide_slots<R>(v[gi], below, upto, td, gd, bad, slot);
 #pragma unroll
-                for (int r = 0; r < R; r++) {
-                    const uint32_t e = (uint32_t)r * 64u + lane;
-                    if (e < nn) srt[gi * RS + e] = v[gi][r];
-                }
-            }
-            wave_lds_sync();
-            // t - 1 from the parked rows (the scan reads keys only), into the
-            // low byte of the record (its occurrence field, always 1)
-#pragma unroll
-            for (uint32_t gi = 0; gi < G; gi++) {
-                const uint32_t nn = (uint32_t)__shfl((int)n, (int)gi);
-                uint32_t t[R];
-#pragma unroll
-                for (int r = 0; r < R; r++) t[r] = 0;
-                for (uint32_t j = 0; j < nn; j++) {
-                    const uint32_t kj = srt[gi * RS + j] & kRecKeyMask;
-#pragma unroll
-                    for (int r = 0; r < R; r++) {
-                        const uint32_t e = (uint32_t)r * 64u + lane;
-                        t[r] += (j < e && kj == (v[gi][r] & kRecKeyMask)) ? 1u : 0u;
-                    }
-                }
-                wave_lds_sync();
-#pragma unroll
-                for (int r = 0; r < R; r++) {
-                    const uint32_t e = (uint32_t)r * 64u + lane;
-                    v[gi][r] = (v[gi][r] & ~0xFFu) | t[r];
-                    if (e < nn) srt[gi * RS + e] = v[gi][r];
-                }
-            }
-            wave_lds_sync();
-            // g from the rows holding t - 1, then the score rows
-#pragma unroll
-            for (uint32_t gi = 0; gi < G; gi++) {
-                const uint32_t nn = (uint32_t)__shfl((int)n, (int)gi);
-                uint32_t g[R];
-#pragma unroll
-                for (int r = 0; r < R; r++) g[r] = 0;
-                for (uint32_t j = 0; j < nn; j++) {
-                    const uint32_t wj = srt[gi * RS + j];
-                    const uint32_t first_base = (wj & 0xFFu) == 0 ? (wj & kRecBaseMask) : 0xFFFFFFFFu;
-#pragma unroll
-                    for (int r = 0; r < R; r++) {
-                        const uint32_t e = (uint32_t)r * 64u + lane;
-                        g[r] += (j <= e && first_base == (v[gi][r] & kRecBaseMask)) ? 1u : 0u;
-                    }
-                }
-#pragma unroll
-                for (int r = 0; r < R; r++) {
-                    const uint32_t e = (uint32_t)r * 64u + lane;
-                    uint32_t b2 = 0;
-                    v[gi][r] = qadj_slot(v[gi][r], v[gi][r] & 0xFFu, g[r] - 1u, td, gd, b2);
-                    if (e < nn) bad |= b2;
-                }
-            }
-            wave_lds_sync();
-#pragma unroll
-            for (uint32_t gi = 0; gi < G; gi++) {
-                const uint32_t nn = (uint32_t)__shfl((int)n, (int)gi);
-#pragma unroll
-                for (int r = 0; r < R; r++) {
-                    const uint32_t e = (uint32_t)r * 64u + lane;
-                    if (e < nn) srt[gi * RS + e] = score_row(f.t.qadj, v[gi][r], bad);
-                }
+                for (int r = 0; r < R; r++)
+                    if (slot[r] != 0xFFFFFFFFu) srt[gi * RS + (uint32_t)r * 64u + lane] = score_row(f.t.qadj, slot[r], bad);
             }
         }
         wave_lds_sync();
