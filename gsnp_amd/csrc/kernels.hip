@@ -813,7 +813,16 @@ __device__ __forceinline__ void wide_slots(const uint32_t* v, uint64_t below, ui
     }
 }
 
-template <int S>
+// Adjusted-quality tables up to this size are copied into LDS by the
+// classes of S <= 32 (QL), whose rows leave room for them.
+constexpr uint32_t kQadjLdsMax = 20480;
+
+__host__ __device__ constexpr uint32_t qadj_lds_words(uint32_t td, uint32_t gd)
+{
+    return ((64u * td * gd + 15u) / 16u) * 4u;  // whole 16-byte units
+}
+
+template <int S, bool QL = false>
 __global__ __launch_bounds__(256) void k_fused(FusedArgs f, const uint32_t* list, const uint32_t* count_ptr)
 {
     constexpr uint32_t G = fused_group<S>();
@@ -821,7 +830,20 @@ __global__ __launch_bounds__(256) void k_fused(FusedArgs f, const uint32_t* list
     extern __shared__ uint32_t lds[];
     const uint32_t lane = lane_id();
     const uint32_t wib = threadIdx.x >> 6;
-    uint32_t* srt = lds + wib * (G * RS);
+    const uint8_t* qtab = f.t.qadj;
+    uint32_t qwords = 0;
+    if constexpr (QL) {
+        // the block copies the table once (words, then the last bytes)
+        const uint32_t qbytes = 64u * (uint32_t)f.t.td * (uint32_t)f.t.gd, nw = qbytes / 4u;
+        const uint32_t* src = reinterpret_cast<const uint32_t*>(f.t.qadj);
+        for (uint32_t i = threadIdx.x; i < nw; i += blockDim.x) lds[i] = src[i];
+        if (threadIdx.x < (qbytes & 3u))
+            reinterpret_cast<uint8_t*>(lds)[4u * nw + threadIdx.x] = f.t.qadj[4u * nw + threadIdx.x];
+        __syncthreads();
+        qtab = reinterpret_cast<const uint8_t*>(lds);
+        qwords = qadj_lds_words((uint32_t)f.t.td, (uint32_t)f.t.gd);
+    }
+    uint32_t* srt = lds + qwords + wib * (G * RS);
     const uint32_t count = *count_ptr;
     const uint32_t ngroups = (count + G - 1) / G;
     const uint32_t wave = blockIdx.x * (blockDim.x >> 6) + wib;
@@ -895,7 +917,7 @@ __global__ __launch_bounds__(256) void k_fused(FusedArgs f, const uint32_t* list
                 }
 #pragma unroll
                 for (uint32_t k = 0; k < CH; k++)
-                    if (v[k] != 0xFFFFFFFFu) srt[((k0 + k) * P + sub) * RS + e] = score_row(f.t.qadj, v[k], bad);
+                    if (v[k] != 0xFFFFFFFFu) srt[((k0 + k) * P + sub) * RS + e] = score_row(qtab, v[k], bad);
             }
         } else {
             constexpr int R = S / 64;
@@ -990,13 +1012,22 @@ static void launch_fused_class(hipStream_t st, const FusedArgs& f, const uint32_
 {
     constexpr uint32_t G = fused_group<S>();
     const uint32_t per_wave = G * fused_row_stride<S>() * 4u;
-    uint32_t wpb = 4u * per_wave <= 65536u ? 4u : 2u * per_wave <= 65536u ? 2u : 1u;
+    const bool ql = S <= 32 && 64u * (uint32_t)f.t.td * (uint32_t)f.t.gd <= kQadjLdsMax;
+    const uint32_t table = ql ? 4u * qadj_lds_words((uint32_t)f.t.td, (uint32_t)f.t.gd) : 0u;
+    uint32_t wpb = table + 4u * per_wave <= 65536u ? 4u : table + 2u * per_wave <= 65536u ? 2u : 1u;
     if (want_wpb && want_wpb < wpb) wpb = want_wpb;
     const uint32_t groups = (max_sites + G - 1) / G;
     uint32_t blocks = (groups + wpb - 1) / wpb;
     if (blocks > max_blocks) blocks = max_blocks;
     if (!blocks) blocks = 1;
-    hipLaunchKernelGGL(k_fused<S>, dim3(blocks), dim3(64 * wpb), wpb * per_wave, st, f, list, count);
+    if constexpr (S <= 32) {
+        if (ql) {
+            hipLaunchKernelGGL((k_fused<S, true>), dim3(blocks), dim3(64 * wpb), table + wpb * per_wave, st, f, list,
+                               count);
+            return;
+        }
+    }
+    hipLaunchKernelGGL((k_fused<S, false>), dim3(blocks), dim3(64 * wpb), wpb * per_wave, st, f, list, count);
 }
 
 void launch_fused_one(hipStream_t st, const FusedArgs& f, const uint32_t* lists, uint32_t pitch,
