@@ -713,12 +713,13 @@ void launch_posterior(hipStream_t st, const Tables& t, const RefView& ref, uint3
 //     the site; posterior_site calls it from registers.
 
 
-// G: sites per wave group.  The wide classes hold few sites, so their groups
-// are small to spread them over more waves.
+// G: sites per wave group.  The 64-record class takes 32 (its LDS rows would
+// otherwise limit it to nine waves per CU); the wide classes hold few sites,
+// so their groups are small to spread them over more waves.
 template <int S>
 constexpr uint32_t fused_group()
 {
-    return S <= 64 ? 64u : S == 128 ? 16u : 8u;
+    return S <= 32 ? 64u : S == 64 ? 32u : S == 128 ? 16u : 8u;
 }
 
 // LDS words per site row: odd, so that row walks of the 64 lanes are
