@@ -713,13 +713,14 @@ void launch_posterior(hipStream_t st, const Tables& t, const RefView& ref, uint3
 //     the site; posterior_site calls it from registers.
 
 
-// G: sites per wave group.  The 64-record class takes 32 (its LDS rows would
-// otherwise limit it to nine waves per CU); the wide classes hold few sites,
-// so their groups are small to spread them over more waves.
+// G: sites per wave group.  The 32- and 64-record classes take 32: with
+// 16-pass chunks and eight gathers in flight they fit in about 100 VGPRs, and
+// their smaller LDS rows then let 16 waves share a CU; the wide classes hold
+// few sites, so their groups are small to spread them over more waves.
 template <int S>
 constexpr uint32_t fused_group()
 {
-    return S <= 32 ? 64u : S == 64 ? 32u : S == 128 ? 16u : 8u;
+    return S <= 16 ? 64u : S <= 64 ? 32u : S == 128 ? 16u : 8u;
 }
 
 // LDS words per site row: odd, so that row walks of the 64 lanes are
@@ -869,8 +870,8 @@ __global__ __launch_bounds__(256) void k_fused(FusedArgs f, const uint32_t* list
             const uint64_t gmask = (S == 64 ? ~0ull : ((1ull << S) - 1ull)) << gbase;  // lanes of my group
             const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
             const uint64_t upto = below | (1ull << lane);
-            // in chunks of CH passes (bounds the registers of the wide class)
-            constexpr uint32_t CH = NP < 32 ? NP : 32;
+            // in chunks of CH passes (bounds the registers)
+            constexpr uint32_t CH = NP < 16 ? NP : 16;
 #pragma unroll
             for (uint32_t k0 = 0; k0 < NP; k0 += CH) {
                 uint32_t v[CH];
@@ -950,10 +951,10 @@ __global__ __launch_bounds__(256) void k_fused(FusedArgs f, const uint32_t* list
         }
         wave_lds_sync();
         // phase 2: twelve sites at a time, lane (sl, cl) adds genotypes 2cl and
-        // 2cl+1 of site r * 12 + sl in record order, 16 records in flight; the
+        // 2cl+1 of site r * 12 + sl in record order, eight records in flight; the
         // sums replace words 4cl..4cl+3 of the site's row once every lane is
         // past the row
-        constexpr uint32_t U = 16;
+        constexpr uint32_t U = 8;
 #pragma unroll 1
         for (uint32_t r = 0; r < (G + 11) / 12; r++) {
             const uint32_t gi = r * 12u + sl;
