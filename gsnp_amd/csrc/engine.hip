@@ -651,7 +651,7 @@ int gsnp_autotune(gsnp_ctx* c, uint32_t start, uint32_t n, int reps, uint32_t* c
         }
         return best;
     };
-    static const uint32_t kWpb[3] = {1, 2, 4};
+    static const uint32_t kWpb[4] = {1, 2, 4, 8};
     static const uint32_t kCaps[5] = {512, 1024, 2048, 4096, 8192};
     FusedTune best_tune;
     float best_total = 3.4e38f;

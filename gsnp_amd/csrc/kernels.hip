@@ -825,7 +825,7 @@ __host__ __device__ constexpr uint32_t qadj_lds_words(uint32_t td, uint32_t gd)
 }
 
 template <int S, bool QL = false>
-__global__ __launch_bounds__(256) void k_fused(FusedArgs f, const uint32_t* list, const uint32_t* count_ptr)
+__global__ __launch_bounds__(512) void k_fused(FusedArgs f, const uint32_t* list, const uint32_t* count_ptr)
 {
     constexpr uint32_t G = fused_group<S>();
     constexpr uint32_t RS = fused_row_stride<S>();
@@ -1016,7 +1016,11 @@ static void launch_fused_class(hipStream_t st, const FusedArgs& f, const uint32_
     const uint32_t per_wave = G * fused_row_stride<S>() * 4u;
     const bool ql = S <= 32 && 64u * (uint32_t)f.t.td * (uint32_t)f.t.gd <= kQadjLdsMax;
     const uint32_t table = ql ? 4u * qadj_lds_words((uint32_t)f.t.td, (uint32_t)f.t.gd) : 0u;
-    uint32_t wpb = table + 4u * per_wave <= 65536u ? 4u : table + 2u * per_wave <= 65536u ? 2u : 1u;
+    // waves per block: eight when the LDS table is shared (one copy per block)
+    uint32_t wpb = (ql && table + 8u * per_wave <= 65536u) ? 8u
+                 : table + 4u * per_wave <= 65536u         ? 4u
+                 : table + 2u * per_wave <= 65536u         ? 2u
+                                                            : 1u;
     if (want_wpb && want_wpb < wpb) wpb = want_wpb;
     const uint32_t groups = (max_sites + G - 1) / G;
     uint32_t blocks = (groups + wpb - 1) / wpb;

@@ -156,7 +156,7 @@ int gsnp_batched_call(gsnp_ctx* ctx, uint32_t start, uint32_t n_sites, int emit_
 
 /* Autotuner of the batched form: builds the span's site matrix, then times
  * the fused launch of every depth class on it for each smallest network width
- * (8, 16, 32: the lowest bucket edge), waves per block (1, 2, 4) and grid cap
+ * (8, 16, 32: the lowest bucket edge), waves per block (1, 2, 4, 8) and grid cap
  * (512..8192 blocks), and keeps the fastest shape in the context for later
  * gsnp_batched_call / gsnp_job_call.  reps timings per candidate, best kept.
  * config (13 entries, may be NULL) receives min class, waves per block of the
