@@ -105,11 +105,16 @@ def test_batched_replays_match_oracle(datasets, name, tune):
                                         ("c2w3", {"pcr_dependency": 1.0, "global_dependency": 1.0}),
                                         ("c2w3", {"pcr_dependency": 0.8, "global_dependency": 0.95,
                                                   "het_novel_r": 0.01})])
-def test_parameter_variants(datasets, tmp_path, name, flags):
+@pytest.mark.parametrize("batched", [0, 4])
+def test_parameter_variants(datasets, tmp_path, name, flags, batched):
+    """Per-window and batched forms under other parameters; the dependency
+    coefficients give adjusted-quality tables of 64 bytes (-p 1 -g 1, copied
+    into LDS by the fused kernel) and of 85 KB (-p 0.8 -g 0.95, read from
+    global memory)."""
     ds = datasets(name)
-    print(f"[parity] {name} {flags}", flush=True)
+    print(f"[parity] {name} {flags} batched={batched}", flush=True)
     out = tmp_path / f"{name}.var.cns"
-    _gpu_run(ds, out, flags=flags)
+    _gpu_run(ds, out, flags=flags, batched=batched)
     ref = ds.oracle_text(**flags)
     assert out.read_bytes() == ref.read_bytes(), _first_diff(out, ref)
 
