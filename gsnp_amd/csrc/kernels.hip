@@ -82,6 +82,7 @@ __global__ __launch_bounds__(kBuildSites) void k_site_build(ReadView r, const ui
     __shared__ __align__(16) uint8_t sseq[kStageBytes + 32];
     __shared__ int32_t spos[kStageReads];
     __shared__ uint32_t smeta[kStageReads];
+    __shared__ int sca[kBuildSites * 4];  // count_all per lane and base (one LDS add per base)
     const uint32_t s = blockIdx.x * kBuildSites + threadIdx.x;
     const uint32_t pitch = r.pitch;
     const uint32_t kr = kStageBytes / pitch < kStageReads ? kStageBytes / pitch : kStageReads;
@@ -112,6 +113,10 @@ __global__ __launch_bounds__(kBuildSites) void k_site_build(ReadView r, const ui
             pb0 = v.x, pb1 = v.y, pb2 = v.z, pb3 = v.w;
         }
     }
+    sca[threadIdx.x * 4 + 0] = ca0;
+    sca[threadIdx.x * 4 + 1] = ca1;
+    sca[threadIdx.x * 4 + 2] = ca2;
+    sca[threadIdx.x * 4 + 3] = ca3;
     const int64_t p = (int64_t)ws + s;
     const int64_t p0 = (int64_t)ws + (int64_t)blockIdx.x * kBuildSites;  // the block's first site
     const int32_t lane = (int32_t)threadIdx.x;
@@ -168,10 +173,7 @@ __global__ __launch_bounds__(kBuildSites) void k_site_build(ReadView r, const ui
                 cu += 1u << (8 * o);
                 qs += (uint64_t)q << (16 * o);
             }
-            ca0 += o == 0;
-            ca1 += o == 1;
-            ca2 += o == 2;
-            ca3 += o == 3;
+            atomicAdd(&sca[threadIdx.x * 4 + o], 1);
         }
     }
     if (!live) return;
@@ -184,7 +186,8 @@ __global__ __launch_bounds__(kBuildSites) void k_site_build(ReadView r, const ui
     a.nrec[s] = nrec;
     a.cu[s] = make_int4((int)(cu & 0xff), (int)((cu >> 8) & 0xff), (int)((cu >> 16) & 0xff), (int)(cu >> 24));
     a.qs[s] = make_int4((int)(qs & 0xffff), (int)((qs >> 16) & 0xffff), (int)((qs >> 32) & 0xffff), (int)(qs >> 48));
-    a.ca[s] = make_int4(ca0, ca1, ca2, ca3);
+    a.ca[s] = make_int4(sca[threadIdx.x * 4 + 0], sca[threadIdx.x * 4 + 1], sca[threadIdx.x * 4 + 2],
+                        sca[threadIdx.x * 4 + 3]);
 }
 
 void launch_site_build(hipStream_t st, const ReadView& r, uint64_t r0, uint64_t r1, uint32_t ws, uint32_t n_sites,
