@@ -513,12 +513,18 @@ int gsnp_window_likelihood(gsnp_ctx* c, uint32_t C)
         if (rc) return rc;
     }
     SiteArrays a = c->site_view(0);
+    // the fused kernel without its posterior: every site of the window is
+    // bucketed, sorted in registers (the records in memory keep their order:
+    // nothing downstream depends on it) and its ten log-likelihoods stored
     c->timer.begin(kStSort, c->stream);
-    launch_bucket(c->stream, a.nrec, C, c->lists.p, c->win_cap, c->counts.p, 0, c->stat.p);
-    launch_sort(c->stream, a.rec, a.nrec, c->lists.p, c->win_cap, c->counts.p, C);
+    launch_bucket(c->stream, a.nrec, C, c->lists.p, c->win_cap, c->counts.p, 1, c->stat.p, (int)c->tune.min_class);
     c->timer.end(kStSort, c->stream);
+    FusedArgs f = c->fused_args(c->ws);
+    f.head_qs = nullptr;
+    f.write_lk = 1;
+    f.posterior = 0;
     c->timer.begin(kStLikelihood, c->stream);
-    launch_likelihood(c->stream, c->tables(), a, C, c->lk.p, c->err.p);
+    launch_fused(c->stream, f, c->lists.p, c->win_cap, c->counts.p, C, c->tune);
     c->timer.end(kStLikelihood, c->stream);
     HIPCHK(hipGetLastError());
     return GSNP_OK;
@@ -877,7 +883,6 @@ FusedArgs Ctx::fused_args(uint32_t start)
     f.job_count = job_count.p;
     f.err = err.p;
     f.write_lk = keep_lk ? 1 : 0;
-    f.sw = ((uint32_t)(p.read_length + 7) / 8) | 1u;
     return f;
 }
 

@@ -98,8 +98,8 @@ struct FusedArgs {
     RankJob* jobs = nullptr;
     uint32_t* job_count = nullptr;
     uint32_t* err = nullptr;
-    int write_lk = 0;  // also store the ten log-likelihoods of each site
-    uint32_t sw = 1;   // words per lane of the dependency-count table, odd, >= ceil(L/8)
+    int write_lk = 0;   // also store the ten log-likelihoods of each site
+    int posterior = 1;  // call the sites (0: likelihoods only, for the per-window path)
 };
 
 // Launch shape of the fused stage: bucket classes below min_class run on

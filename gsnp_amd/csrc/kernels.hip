@@ -1005,8 +1005,9 @@ __global__ __launch_bounds__(512) void k_fused(FusedArgs f, const uint32_t* list
 #pragma unroll
             for (int h = 0; h < 5; h++) out[h] = make_double2(acc[2 * h], acc[2 * h + 1]);
         }
-        posterior_site(f.t, f.ref, f.ws, f.a, site, [&](int g) { return acc[g]; }, f.head_qs, f.out, f.jobs,
-                       f.job_count);
+        if (f.posterior)
+            posterior_site(f.t, f.ref, f.ws, f.a, site, [&](int g) { return acc[g]; }, f.head_qs, f.out, f.jobs,
+                           f.job_count);
     }
     if (bad) atomicOr(f.err, bad);
 }
