@@ -687,14 +687,15 @@ void launch_posterior(hipStream_t st, const Tables& t, const RefView& ref, uint3
 }
 
 // ---------------------------------------------------------------------------
-// Batched form: sort + likelihood + posterior fused per depth class.  A wave
-// takes a group of G sites of one class (record count <= S) from the bucket
-// lists; lane i owns site i of the group.
-//  1. Sort: every record slot of the group is loaded first (the loads are in
-//     flight together), then for S <= 64 lane groups of S sort 64/S sites per
-//     pass with the k_sort_lanes networks, for S = 128/256 one site per pass
-//     with S/64 records per lane.  Each sorted record then learns, in
-//     parallel (kernel/likelihood_kernel.cu:168-256 walks them in order):
+// Batched form: sort + likelihood + posterior fused per depth class (the
+// per-window path runs the same launch without the posterior).  A wave takes
+// a group of G sites of one class (record count <= S) from the bucket lists;
+// lane i owns site i of the group.
+//  1. Sort: the record slots of a chunk of passes are loaded first (the loads
+//     are in flight together), then for S <= 64 lane groups of S sort 64/S
+//     sites per pass with the k_sort_lanes networks, for S = 128/256 one site
+//     per pass with S/64 records per lane.  Each sorted record then learns,
+//     in parallel (kernel/likelihood_kernel.cu:168-256 walks them in order):
 //       t - 1: how many records before it share its (base, strand, cycle)
 //              key -- the same base means the same segment;
 //       g:     how many records of its base segment up to and including it
@@ -702,19 +703,18 @@ void launch_posterior(hipStream_t st, const Tables& t, const RefView& ref, uint3
 //     One ballot per key bit gives each lane the lanes of its site holding
 //     its key (all eleven bits) and its base (the two base bits), and one
 //     more ballot the first occurrences; the wide classes, with S/64 records
-//     per lane, take the ballots of every register.  (q, t - 1, g) indexes the adjusted-
-//     quality table and the record is parked in LDS ([G][RS] words) as its
-//     score row (q_adj << 10 | cycle << 2 | base).
+//     per lane, take the ballots of every register.  (q, t - 1, g) indexes
+//     the adjusted-quality table (copied into LDS by the classes up to 32
+//     records when it is small) and the record is parked in LDS ([G][RS]
+//     words) as its score row (q_adj << 10 | cycle << 2 | base).
 //  2. Five lanes per site, two genotypes each, add the site's score values
-//     in record order: each load instruction reads whole 80-byte rows of
-//     twelve sites (five contiguous lanes per row, one line) instead of one
-//     16-byte piece of 64 rows, which divides the address work of the
-//     gathers by five.
-//     Records are read once, with non-temporal loads, so they do not push
-//     the score table out of L2.
+//     in record order, eight records in flight: each load instruction reads
+//     whole 80-byte rows of twelve sites (five contiguous lanes per row, one
+//     line) instead of one 16-byte piece of 64 rows, which divides the
+//     address work of the gathers by five.  Records are read once, with
+//     non-temporal loads, so they do not push the score table out of L2.
 //  3. The ten sums of a site go through its LDS row to the lane that owns
 //     the site; posterior_site calls it from registers.
-
 
 // G: sites per wave group.  The 32- and 64-record classes take 32: with
 // 16-pass chunks and eight gathers in flight they fit in about 100 VGPRs, and
