@@ -309,14 +309,28 @@ __device__ __forceinline__ uint32_t xor_lane(uint32_t v)
     }
 }
 
+// median of three: with c = 0 the smaller of a and b, with c = ~0 the
+// larger (one v_med3_u32; c depends on the lane and the stage only)
+__device__ __forceinline__ uint32_t umed3(uint32_t a, uint32_t b, uint32_t c)
+{
+    const uint32_t lo = a < b ? a : b, hi = a < b ? b : a;
+    const uint32_t t = hi < c ? hi : c;
+    return lo > t ? lo : t;
+}
+
+// all ones where the lane keeps the larger value of a compare-exchange at
+// distance J within merge K (ascending where (e & K) == 0)
+template <int K, int J>
+__device__ __forceinline__ uint32_t keeps_max(uint32_t e)
+{
+    return ((e & K) == 0) == ((e & J) == 0) ? 0u : 0xFFFFFFFFu;
+}
+
 template <int K, int J>
 __device__ __forceinline__ void bitonic_step(uint32_t& v, uint32_t e)
 {
     const uint32_t w = xor_lane<J>(v);
-    const bool up = (e & K) == 0;
-    const bool low = (e & J) == 0;
-    const uint32_t mn = v < w ? v : w, mx = v < w ? w : v;
-    v = (up == low) ? mn : mx;
+    v = umed3(v, w, keeps_max<K, J>(e));
 }
 
 template <int K, int J>
@@ -342,11 +356,9 @@ __device__ __forceinline__ void bitonic_step2(uint32_t& v0, uint32_t& v1, uint32
 {
     const uint32_t w0 = xor_lane<J>(v0);
     const uint32_t w1 = xor_lane<J>(v1);
-    const bool take_min = ((e & K) == 0) == ((e & J) == 0);
-    const uint32_t mn0 = v0 < w0 ? v0 : w0, mx0 = v0 < w0 ? w0 : v0;
-    const uint32_t mn1 = v1 < w1 ? v1 : w1, mx1 = v1 < w1 ? w1 : v1;
-    v0 = take_min ? mn0 : mx0;
-    v1 = take_min ? mn1 : mx1;
+    const uint32_t c = keeps_max<K, J>(e);
+    v0 = umed3(v0, w0, c);
+    v1 = umed3(v1, w1, c);
 }
 
 template <int K, int J>
