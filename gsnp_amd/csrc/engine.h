@@ -149,10 +149,9 @@ struct Ctx {
     bool keep_lk = false;     // batched form: also store the per-site log-likelihoods
     FusedTune tune;           // launch shape of the fused stage (gsnp_autotune)
     // batched span and its formatting replay as graphs while shape and buffers
-    // stay the same; buf_gen counts buffer (re)allocations.  Off: a run that
-    // replayed a captured span did not finish and the cause is not known, so
-    // the spans are launched eagerly until it is.
-    bool use_graphs = false;
+    // stay the same; buf_gen counts buffer (re)allocations and chromosome or
+    // read changes, each made after a stream sync, and drops the cached graphs
+    bool use_graphs = true;
     uint64_t buf_gen = 0, graphs_gen = 0;
     std::vector<std::unique_ptr<GraphSlot>> graphs;  // captured spans, at most kMaxGraphs
     GraphSlot& graph_slot(const std::vector<uint64_t>& key);

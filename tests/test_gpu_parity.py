@@ -69,16 +69,16 @@ def _job_text(job) -> bytes:
 
 
 def _rows_by_contig(ref: bytes) -> dict:
-    parts: dict[bytes, bytes] = {}
+    """The oracle's rows of each contig, in output order.  Lines are gathered
+    in lists and joined once: appending to a growing bytes object copies it
+    every time, which is quadratic and takes the better part of an hour on the
+    600 kb set."""
+    rows: dict[bytes, list] = {}
     for line in ref.splitlines(keepends=True):
-        key = line.split(b"\t", 1)[0]
-        parts[key] = parts.get(key, b"") + line
-    return parts
+        rows.setdefault(line.split(b"\t", 1)[0], []).append(line)
+    return {k: b"".join(v) for k, v in rows.items()}
 
 
-@pytest.mark.skip(reason="its first GPU run (c2w3: Job, autotune, one batched call, text fetch) went silent and was "
-                         "killed after 180 s; cause not found by reading the tuner, the batched call or the text "
-                         "fetch, so this case stays off the GPU until it is")
 @pytest.mark.parametrize("name", ["c2w3", "stress"])
 def test_tuned_batched_matches_oracle(datasets, name):
     """The launch shape the autotuner keeps (smallest network width, so small
@@ -103,9 +103,6 @@ def test_tuned_batched_matches_oracle(datasets, name):
         job.close()
 
 
-@pytest.mark.skip(reason="repeated batched calls with per-window passes between them did not finish on the GPU "
-                         "(run killed for silence) while span graphs were on; cause not found, graphs are off and "
-                         "this case stays off the GPU until it is")
 @pytest.mark.parametrize("name,tune", [("c2w3", False), ("stress", True)])
 def test_batched_replays_match_oracle(datasets, name, tune):
     """The batched span runs as a captured graph from its second call on; every
@@ -120,27 +117,14 @@ def test_batched_replays_match_oracle(datasets, name, tune):
         job.select(0)
         if tune:
             print(f"[parity] {name} tuned {job.autotune(reps=1)}", flush=True)
-        L = gsnp_amd.lib()
-
-        def text() -> bytes:
-            n = ctypes.c_uint64()
-            gsnp_amd.check(L.gsnp_fetch_text(job.ctx, None, 0, ctypes.byref(n)))
-            buf = ctypes.create_string_buffer(max(1, n.value))
-            gsnp_amd.check(L.gsnp_fetch_text(job.ctx, buf, n.value, ctypes.byref(n)))
-            return buf.raw[: n.value]
-
-        # the oracle's rows of each contig, in output order
-        parts: dict[bytes, bytes] = {}
-        for line in ref.splitlines(keepends=True):
-            key = line.split(b"\t", 1)[0]
-            parts[key] = parts.get(key, b"") + line
+        parts = _rows_by_contig(ref)
         assert len(parts) == len(job)
         for i in range(len(job)):
             job.select(i)
             want = parts[job.info(i)[0].encode()]
             for rep in range(4):
                 job.call(batched=64)
-                out = text()
+                out = _job_text(job)
                 assert out == want, f"contig {i} replay {rep}: {len(out)} vs {len(want)} bytes"
                 job.call(batched=0)  # eager per-window pass between replays
     finally:
