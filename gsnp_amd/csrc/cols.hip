@@ -1,0 +1,733 @@
+// cols.hip -- GSNP's -C column files, encoded on the device (Column_* and
+// extract_*, cpp/compress.h:33-1312; codecs kernel/compress_kernel.cu:85-393
+// and cpp/dict_compression.h:47-296).  Every reference window of a called
+// span becomes one chunk in each of the five column files; the device builds
+// those chunks byte for byte and the host only writes them out.
+//
+// Stages over a span cut into windows of GSNP_WIN sites (window w of every
+// per-window array starts at w*GSNP_WIN):
+//   k_col_count   per 256-site block: 16 counters (the sparse lists, the N
+//                 lists of columns 3 and 10, the run heads of the six
+//                 run-length columns, the sites that are not N rows)
+//   k_col_scan    per window: exclusive scan of the block counters
+//   k_col_emit    per site: the sparse lists in site order, run values and
+//                 starts, the 2-bit words of columns 3 and 10
+//   k_col_runlen  run lengths from the run starts
+//   k_col_dict    per (window, array), one workgroup: sorted-unique
+//                 dictionary and packed ranks of the 15 dictionary-coded
+//                 arrays (a hash set gives the distinct values, each one's
+//                 rank is the number of distinct values below it)
+//   k_col_sizes   chunk sizes and their offsets in each file
+//   k_col_write   the chunks
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <string>
+
+#include "../../include/gsnp_abi.h"
+#include "cols.h"
+#include "cwriter.h"
+#include "engine.h"
+
+namespace gsnp {
+
+namespace {
+
+constexpr uint32_t kWin = GSNP_WIN;
+constexpr uint32_t kCB = 256;               // sites per block of the per-site kernels
+constexpr uint32_t kWinBlocks = kWin / kCB;  // 1024
+constexpr int kNC = 16;                      // counters per block
+constexpr int kNDict = 15;                   // dictionary-coded arrays per window
+constexpr uint32_t kTabStride = 2 * kWin;    // hash slots per (window, array)
+constexpr uint32_t kDictThreads = 1024;
+constexpr uint32_t kSubWindows = 16;         // windows per encoder pass
+constexpr uint64_t kEmpty = ~0ull;           // never a key: int keys use 32 bits, f64 keys are not NaN
+
+// counters: sparse lists, N lists, non-N sites, then the run heads of
+// columns 5, 7, 8, 9, 14, 16
+enum : int { kC4 = 0, kC6, kCN3, kCN10, kCCom, kCX12, kCX13, kC15, kC17, kCNotN, kCRun0 };
+
+// dictionary-coded arrays of a window: 0-4 run values of columns 5, 7, 8, 9,
+// 14 (int32), 5 run values of column 16 (f64), 6-11 the six run lengths,
+// 12-14 columns 11, 12, 13 of the common list
+
+__constant__ char kColACTGN[8] = {'A', 'C', 'T', 'G', 'N', 'N', 'N', 'N'};
+__constant__ char kColAbbv[17] = {'A', 'M', 'W', 'R', 'M', 'C', 'Y', 'S', 'W',
+                                  'Y', 'T', 'K', 'R', 'S', 'K', 'G', 'N'};
+__constant__ uint32_t kColCode2[4] = {0, 1, 3, 2};  // 2-bit code (A0 C1 G2 T3) of "ACTG"[i]
+
+struct ColIn {
+    const uint8_t* ref;  // reference codes of site 0
+    SiteArrays a;
+    Results res;
+    uint32_t n;
+};
+
+struct ColSite {
+    bool nflag, nrow, n3, n10, com, x12, x13, f15, f17;
+    uint32_t c3, c4, c6, code3, code10;
+    int32_t v[5];
+    double cn;
+    int32_t a, b, d;
+    double rank;
+};
+
+__device__ __forceinline__ int pick(int4 v, uint32_t b) { return b == 0 ? v.x : b == 1 ? v.y : b == 2 ? v.z : v.w; }
+
+// The column values of site j (Call_win::call_cns2's -C branch,
+// cpp/compress.h:187-1312, restated by the host writer in cwriter.cpp).
+__device__ ColSite col_site(const ColIn& in, uint32_t j)
+{
+    ColSite s;
+    const uint32_t ori = in.ref[j] & 0xF;
+    const int32_t depth = in.a.depth[j];
+    const uint32_t b1 = in.res.base1[j], b2 = in.res.base2[j];
+    s.nflag = (ori & 4) && depth == 0;
+    s.nrow = s.nflag || b1 >= 4;
+    s.c3 = s.nrow ? 'N' : (uint32_t)kColACTGN[ori & 7];
+    s.c4 = s.nrow ? 'N' : (uint32_t)kColAbbv[min((uint32_t)in.res.type1[j], 16u)];
+    s.c6 = s.nrow ? 'N' : (uint32_t)kColACTGN[b1 & 7];
+    s.n3 = s.c3 == 'N';
+    s.code3 = s.n3 ? 0u : kColCode2[ori & 3];
+    s.n10 = s.nrow || b2 >= 4;
+    s.code10 = s.n10 ? 0u : kColCode2[b2 & 3];
+    const int4 qs = in.a.qs[j], cu = in.a.cu[j], ca = in.a.ca[j];
+    if (!s.nrow) {
+        const int qsb = pick(qs, b1), cub = pick(cu, b1);
+        s.v[0] = in.res.q_cns[j];
+        s.v[1] = (qsb == 0 || cub == 0) ? 0 : qsb / cub;
+        s.v[2] = cub;
+        s.v[3] = pick(ca, b1);
+        s.v[4] = depth;
+        s.cn = depth == 0 ? 255.0 : (double)in.a.repeat[j] / (double)depth;
+    } else {
+        s.v[0] = s.v[1] = s.v[2] = s.v[3] = s.v[4] = 0;
+        s.cn = 255.0;
+    }
+    s.a = s.b = s.d = 0;
+    if (!s.n10) {
+        const int qsb = pick(qs, b2);
+        s.b = pick(cu, b2);
+        s.a = (qsb == 0 || s.b == 0) ? 0 : qsb / s.b;
+        s.d = pick(ca, b2);
+    }
+    s.com = s.a != 0;
+    s.x12 = !s.com && s.a != s.b;
+    s.x13 = !s.com && !s.x12 && s.a != s.d;
+    s.rank = in.res.rank[j];
+    s.f15 = !s.nrow && s.rank != 1.0;
+    s.f17 = !s.nrow && (ori & 8);
+    return s;
+}
+
+// Site j's counter bits (bit c for counter c); run heads compare with the
+// previous site of the window through LDS.  Every thread of the block calls it.
+__device__ uint32_t col_flags(const ColIn& in, uint32_t j, bool ok, ColSite& s, int32_t (*sv)[kCB], double* sd)
+{
+    const uint32_t t = threadIdx.x;
+    uint32_t f = 0;
+    if (ok) {
+        s = col_site(in, j);
+        f |= (uint32_t)(s.c4 != s.c3) << kC4;
+        f |= (uint32_t)(s.c6 != s.c3) << kC6;
+        f |= (uint32_t)s.n3 << kCN3;
+        f |= (uint32_t)s.n10 << kCN10;
+        f |= (uint32_t)s.com << kCCom;
+        f |= (uint32_t)s.x12 << kCX12;
+        f |= (uint32_t)s.x13 << kCX13;
+        f |= (uint32_t)s.f15 << kC15;
+        f |= (uint32_t)s.f17 << kC17;
+        f |= (uint32_t)!s.nflag << kCNotN;
+        for (int c = 0; c < 5; c++) sv[c][t] = s.v[c];
+        sd[t] = s.cn;
+    }
+    __syncthreads();
+    if (ok) {
+        const uint32_t i = j % kWin;
+        if (i == 0) {
+            f |= 0x3Fu << kCRun0;
+        } else {
+            int32_t pv[5];
+            double pc;
+            if (t > 0) {
+                for (int c = 0; c < 5; c++) pv[c] = sv[c][t - 1];
+                pc = sd[t - 1];
+            } else {
+                const ColSite p = col_site(in, j - 1);
+                for (int c = 0; c < 5; c++) pv[c] = p.v[c];
+                pc = p.cn;
+            }
+            for (int c = 0; c < 5; c++) f |= (uint32_t)(s.v[c] != pv[c]) << (kCRun0 + c);
+            f |= (uint32_t)(s.cn != pc) << (kCRun0 + 5);
+        }
+    }
+    return f;
+}
+
+__global__ __launch_bounds__(256) void k_col_count(ColIn in, uint32_t* cnt)
+{
+    __shared__ int32_t sv[5][kCB];
+    __shared__ double sd[kCB];
+    __shared__ uint32_t wc[4][kNC];
+    const uint32_t t = threadIdx.x, j = blockIdx.x * kCB + t;
+    ColSite s;
+    const uint32_t f = col_flags(in, j, j < in.n, s, sv, sd);
+    for (int c = 0; c < kNC; c++) {
+        const uint64_t b = __ballot((f >> c) & 1u);
+        if ((t & 63) == 0) wc[t >> 6][c] = (uint32_t)__popcll(b);
+    }
+    __syncthreads();
+    if (t < kNC) cnt[(size_t)blockIdx.x * kNC + t] = wc[0][t] + wc[1][t] + wc[2][t] + wc[3][t];
+}
+
+// per window: exclusive scan of each counter over the window's blocks
+__global__ __launch_bounds__(256) void k_col_scan(const uint32_t* cnt, uint32_t n, uint32_t* boff, uint32_t* tot)
+{
+    __shared__ uint32_t wsum[4];
+    const uint32_t w = blockIdx.x, t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    const uint32_t C = min(kWin, n - w * kWin);
+    const uint32_t nb = (C + kCB - 1) / kCB;
+    const size_t b0 = (size_t)w * kWinBlocks;
+    for (int c = 0; c < kNC; c++) {
+        uint32_t v[4], sum = 0;
+        for (int k = 0; k < 4; k++) {
+            const uint32_t b = t * 4 + k;
+            v[k] = b < nb ? cnt[(b0 + b) * kNC + c] : 0u;
+            sum += v[k];
+        }
+        uint32_t incl = sum;
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint32_t y = __shfl_up(incl, off);
+            if ((int)lane >= off) incl += y;
+        }
+        if (lane == 63) wsum[wv] = incl;
+        __syncthreads();
+        uint32_t run = incl - sum;
+        for (uint32_t k = 0; k < wv; k++) run += wsum[k];
+        for (int k = 0; k < 4; k++) {
+            const uint32_t b = t * 4 + k;
+            if (b < nb) boff[(b0 + b) * kNC + c] = run;
+            run += v[k];
+        }
+        if (t == 255) tot[(size_t)w * kNC + c] = run;
+        __syncthreads();
+    }
+}
+
+struct EmitOut {
+    int32_t *id4, *id6, *idn, *nidx, *x12i, *x12v, *x13i, *x13v, *cid, *id15, *id17;
+    uint8_t *v4, *v6;
+    uint32_t *code3, *code10, *rstart, *in32;
+    uint64_t* in64;
+    double* val15;
+    size_t stride;  // elements per array of rstart / in32
+};
+
+__global__ __launch_bounds__(256) void k_col_emit(ColIn in, const uint32_t* boff, EmitOut o)
+{
+    __shared__ int32_t sv[5][kCB];
+    __shared__ double sd[kCB];
+    __shared__ uint32_t wc[4][kNC];
+    const uint32_t t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    const uint32_t j = blockIdx.x * kCB + t;
+    const bool ok = j < in.n;
+    ColSite s;
+    const uint32_t f = col_flags(in, j, ok, s, sv, sd);
+    uint64_t bal[kNC];
+    for (int c = 0; c < kNC; c++) {
+        bal[c] = __ballot((f >> c) & 1u);
+        if (lane == 0) wc[wv][c] = (uint32_t)__popcll(bal[c]);
+    }
+    __syncthreads();
+    const uint32_t w = j / kWin, i = j % kWin;
+    const size_t seg = (size_t)w * kWin;
+    const uint64_t lt = (1ull << lane) - 1ull;
+    auto pos = [&](int c) -> size_t {
+        uint32_t p = boff[(size_t)blockIdx.x * kNC + c] + (uint32_t)__popcll(bal[c] & lt);
+        for (uint32_t k = 0; k < wv; k++) p += wc[k][c];
+        return seg + p;
+    };
+    if (ok) {
+        if (f & (1u << kC4)) {
+            const size_t p = pos(kC4);
+            o.id4[p] = (int32_t)i;
+            o.v4[p] = (uint8_t)s.c4;
+        }
+        if (f & (1u << kC6)) {
+            const size_t p = pos(kC6);
+            o.id6[p] = (int32_t)i;
+            o.v6[p] = (uint8_t)s.c6;
+        }
+        if (f & (1u << kCN3)) o.idn[pos(kCN3)] = (int32_t)i;
+        if (f & (1u << kCN10)) o.nidx[pos(kCN10)] = (int32_t)i;
+        if (f & (1u << kCCom)) {
+            const size_t p = pos(kCCom);
+            o.cid[p] = (int32_t)i;
+            o.in32[12 * o.stride + p] = (uint32_t)s.a;
+            o.in32[13 * o.stride + p] = (uint32_t)s.b;
+            o.in32[14 * o.stride + p] = (uint32_t)s.d;
+        }
+        if (f & (1u << kCX12)) {
+            const size_t p = pos(kCX12);
+            o.x12i[p] = (int32_t)i;
+            o.x12v[p] = s.b;
+        }
+        if (f & (1u << kCX13)) {
+            const size_t p = pos(kCX13);
+            o.x13i[p] = (int32_t)i;
+            o.x13v[p] = s.d;
+        }
+        if (f & (1u << kC15)) {
+            const size_t p = pos(kC15);
+            o.id15[p] = (int32_t)i;
+            o.val15[p] = s.rank;
+        }
+        if (f & (1u << kC17)) o.id17[pos(kC17)] = (int32_t)i;
+        for (int c = 0; c < 6; c++) {
+            if (!(f & (1u << (kCRun0 + c)))) continue;
+            const size_t p = pos(kCRun0 + c);
+            o.rstart[c * o.stride + p] = i;
+            if (c < 5)
+                o.in32[c * o.stride + p] = (uint32_t)s.v[c];
+            else
+                o.in64[p] = (uint64_t)__double_as_longlong(s.cn);
+        }
+    }
+    // 2-bit words of columns 3 and 10: 16 sites per word, the first site in
+    // the high bits, the window's last word right-aligned
+    // (kernel/compress_kernel.cu:104-153)
+    const uint32_t C = ok ? min(kWin, in.n - w * kWin) : 0u;
+    const uint32_t last = min((i & ~15u) + 15u, C - 1u);
+    uint32_t x3 = ok ? s.code3 << (2 * (last - i)) : 0u;
+    uint32_t x10 = ok ? s.code10 << (2 * (last - i)) : 0u;
+    for (int m = 1; m < 16; m <<= 1) {
+        x3 |= __shfl_xor(x3, m);
+        x10 |= __shfl_xor(x10, m);
+    }
+    if (ok && (i & 15) == 0) {
+        o.code3[(size_t)w * (kWin / 16) + i / 16] = x3;
+        o.code10[(size_t)w * (kWin / 16) + i / 16] = x10;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_col_runlen(uint32_t n, const uint32_t* tot, const uint32_t* rstart,
+                                                    uint32_t* in32, size_t stride)
+{
+    const uint32_t w = blockIdx.x, c = blockIdx.y;
+    const uint32_t C = min(kWin, n - w * kWin);
+    const uint32_t nrun = tot[(size_t)w * kNC + kCRun0 + c];
+    const uint32_t* rs = rstart + c * stride + (size_t)w * kWin;
+    uint32_t* len = in32 + (6 + c) * stride + (size_t)w * kWin;
+    for (uint32_t r = threadIdx.x; r < nrun; r += blockDim.x) len[r] = (r + 1 < nrun ? rs[r + 1] : C) - rs[r];
+}
+
+struct DictArgs {
+    const uint32_t* tot;
+    const uint32_t* in32;
+    const uint64_t* in64;
+    size_t stride;
+    uint64_t* tab;
+    uint32_t* trank;
+    uint64_t* dkeys;
+    uint32_t* dslot;
+    uint32_t* words;
+    uint32_t* dict32;
+    uint64_t* dict64;
+    uint32_t* nd;
+    uint32_t* nw;
+    uint8_t pow2bits[32];  // index bits of a dictionary of 2^e entries, as the host's libm gives them
+};
+
+__device__ __forceinline__ uint32_t slot_of(uint64_t key, int lg)
+{
+    return (uint32_t)((key * 0x9E3779B97F4A7C15ull) >> (64 - lg));
+}
+
+__device__ __forceinline__ uint64_t tab_load(const uint64_t* p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// One workgroup per (window, array): distinct values through an open-addressed
+// hash set (load at most 1/2, so every probe sequence meets a free slot), each
+// distinct value's rank = the count of distinct values below it, then the
+// ranks packed floor(32/bits) per word, MSB first, the last word right-aligned
+// (dict_encode, cpp/dict_compression.h:116-166).
+__global__ __launch_bounds__(kDictThreads) void k_col_dict(DictArgs a)
+{
+    __shared__ uint32_t s_nd;
+    __shared__ uint64_t tile[2048];
+    const uint32_t w = blockIdx.x, k = blockIdx.y, t = threadIdx.x;
+    const uint32_t* tw = a.tot + (size_t)w * kNC;
+    const bool all_n = tw[kCNotN] == 0;
+    const uint32_t m = all_n ? 0u : k < 12 ? tw[kCRun0 + k % 6] : tw[kCCom];
+    const size_t wk = (size_t)w * kNDict + k;
+    if (m == 0) {
+        if (t == 0) a.nd[wk] = a.nw[wk] = 0;
+        return;
+    }
+    const bool dbl = k == 5;
+    const uint32_t* in = a.in32 + k * a.stride + (size_t)w * kWin;
+    const uint64_t* in64 = a.in64 + (size_t)w * kWin;
+    auto key_at = [&](uint32_t e) -> uint64_t { return dbl ? in64[e] : (uint64_t)in[e]; };
+    uint32_t cap = 64;
+    while (cap < 2 * m) cap <<= 1;
+    const int lg = __ffs(cap) - 1;
+    uint64_t* tab = a.tab + wk * kTabStride;
+    uint32_t* trank = a.trank + wk * kTabStride;
+    uint64_t* dk = a.dkeys + wk * kWin;
+    uint32_t* ds = a.dslot + wk * kWin;
+    if (t == 0) s_nd = 0;
+    for (uint32_t h = t; h < cap; h += kDictThreads) tab[h] = kEmpty;
+    __syncthreads();
+    for (uint32_t e = t; e < m; e += kDictThreads) {
+        const uint64_t key = key_at(e);
+        uint32_t h = slot_of(key, lg);
+        for (uint32_t probe = 0; probe < cap; probe++) {
+            uint64_t cur = tab_load(&tab[h]);
+            if (cur == kEmpty) {
+                cur = atomicCAS(reinterpret_cast<unsigned long long*>(&tab[h]), (unsigned long long)kEmpty,
+                                (unsigned long long)key);
+                if (cur == kEmpty) {
+                    const uint32_t p = atomicAdd(&s_nd, 1u);
+                    dk[p] = key;
+                    ds[p] = h;
+                    break;
+                }
+            }
+            if (cur == key) break;
+            h = (h + 1) & (cap - 1);
+        }
+    }
+    __syncthreads();
+    const uint32_t nd = s_nd;
+    for (uint32_t base = 0; base < nd; base += kDictThreads) {
+        const uint32_t d = base + t;
+        const uint64_t x = d < nd ? dk[d] : 0ull;
+        uint32_t r = 0;
+        for (uint32_t t0 = 0; t0 < nd; t0 += 2048) {
+            const uint32_t len = min(2048u, nd - t0);
+            for (uint32_t u = t; u < len; u += kDictThreads) tile[u] = dk[t0 + u];
+            __syncthreads();
+            if (d < nd) {
+                if (dbl) {
+                    const double xd = __longlong_as_double((long long)x);
+                    for (uint32_t u = 0; u < len; u++) r += __longlong_as_double((long long)tile[u]) < xd;
+                } else {
+                    const int32_t xi = (int32_t)(uint32_t)x;
+                    for (uint32_t u = 0; u < len; u++) r += (int32_t)(uint32_t)tile[u] < xi;
+                }
+            }
+            __syncthreads();
+        }
+        if (d < nd) {
+            trank[ds[d]] = r;
+            if (dbl)
+                a.dict64[(size_t)w * kWin + r] = x;
+            else
+                a.dict32[wk * kWin + r] = (uint32_t)x;
+        }
+    }
+    __syncthreads();
+    const int bits = (nd & (nd - 1)) == 0 ? (int)a.pow2bits[__ffs(nd) - 1] : 32 - __clz(nd - 1);
+    const uint32_t per = 32u / (uint32_t)bits;
+    const uint32_t nwords = (m + per - 1) / per;
+    uint32_t* wo = a.words + wk * kWin;
+    for (uint32_t q = t; q < nwords; q += kDictThreads) {
+        const uint32_t e0 = q * per, e1 = min(e0 + per, m);
+        uint32_t acc = 0;
+        for (uint32_t e = e0; e < e1; e++) {
+            const uint64_t key = key_at(e);
+            uint32_t h = slot_of(key, lg);
+            for (uint32_t probe = 0; probe < cap && tab_load(&tab[h]) != key; probe++) h = (h + 1) & (cap - 1);
+            acc = bits >= 32 ? 0u : (acc << bits);
+            acc |= trank[h];
+        }
+        wo[q] = acc;
+    }
+    if (t == 0) {
+        a.nd[wk] = nd;
+        a.nw[wk] = nwords;
+    }
+}
+
+struct WriteArgs {
+    uint32_t n;
+    const uint32_t* tot;
+    const uint32_t *nd, *nw;
+    const int32_t *id4, *id6, *idn, *nidx, *x12i, *x12v, *x13i, *x13v, *cid, *id15, *id17;
+    const uint8_t *v4, *v6;
+    const uint32_t *code3, *code10, *words, *dict32;
+    const uint64_t* dict64;
+    const double* val15;
+    uint64_t* foff;  // [window][file]
+    uint64_t* ftot;  // [file]
+    uint8_t* out;    // [file][cap]
+    uint64_t cap;
+};
+
+// The pieces of window w's chunk in file f, in file order
+// (cpp/compress.h:151-1312; the host writer in cwriter.cpp writes the same).
+template <class Sink>
+__device__ void chunk_pieces(const WriteArgs& a, uint32_t w, int f, Sink& s)
+{
+    const uint32_t* tw = a.tot + (size_t)w * kNC;
+    const uint64_t C = min(kWin, a.n - w * kWin);
+    const bool all_n = tw[kCNotN] == 0;
+    const size_t seg = (size_t)w * kWin;
+    const uint64_t nind = all_n ? 1u : 0u;
+    const uint64_t ncode = (C + 15) / 16;
+    if (f == 0) {  // .3_4_6
+        if (all_n) {
+            const uint64_t p[7] = {1, 0, C, 0, 0, 0, 0};
+            s.vals(p, 7, 56);
+            return;
+        }
+        const uint64_t n4 = tw[kC4], n6 = tw[kC6], nN = tw[kCN3];
+        const uint64_t p[7] = {0, 5 * n4 + 5 * n6 + 4 * nN + 4 * ncode, C, ncode, nN, n4, n6};
+        s.vals(p, 7, 56);
+        s.bytes(a.id4 + seg, 4 * n4);
+        s.bytes(a.v4 + seg, n4);
+        s.bytes(a.id6 + seg, 4 * n6);
+        s.bytes(a.v6 + seg, n6);
+        s.bytes(a.idn + seg, 4 * nN);
+        s.bytes(a.code3 + (size_t)w * (kWin / 16), 4 * ncode);
+        return;
+    }
+    s.vals(&nind, 1, 4);
+    if (all_n) return;
+    const size_t wk0 = (size_t)w * kNDict;
+    if (f == 1) {  // .5_7_8_9_14_16
+        for (int c = 0; c < 6; c++) {
+            const size_t kv = wk0 + c, kl = wk0 + 6 + c;
+            const uint64_t sz = c == 5 ? 8 : 4;
+            const uint64_t nwv = a.nw[kv], ndv = a.nd[kv], nwl = a.nw[kl], ndl = a.nd[kl];
+            const uint64_t p[6] = {tw[kCRun0 + c], nwv, nwl, ndv, ndl, 4 * nwv + 4 * nwl + sz * ndv + 4 * ndl};
+            s.vals(p, 6, 48);
+            s.bytes(a.words + kv * kWin, 4 * nwv);
+            s.bytes(a.words + kl * kWin, 4 * nwl);
+            if (c == 5)
+                s.bytes(a.dict64 + seg, 8 * ndv);
+            else
+                s.bytes(a.dict32 + kv * kWin, 4 * ndv);
+            s.bytes(a.dict32 + kl * kWin, 4 * ndl);
+        }
+    } else if (f == 2) {  // .10
+        const uint64_t nN = tw[kCN10];
+        const uint64_t p[3] = {nN, ncode, C};
+        s.vals(p, 3, 24);
+        s.bytes(a.code10 + (size_t)w * (kWin / 16), 4 * ncode);
+        s.bytes(a.nidx + seg, 4 * nN);
+    } else if (f == 3) {  // .11_12_13
+        const uint64_t ncom = tw[kCCom], n12 = tw[kCX12], n13 = tw[kCX13];
+        const uint64_t p[4] = {C, ncom, n12, n13};
+        s.vals(p, 4, 32);
+        s.bytes(a.x12i + seg, 4 * n12);
+        s.bytes(a.x12v + seg, 4 * n12);
+        s.bytes(a.x13i + seg, 4 * n13);
+        s.bytes(a.x13v + seg, 4 * n13);
+        s.bytes(a.cid + seg, 4 * ncom);
+        for (int k = 12; k < 15; k++) {
+            const size_t kk = wk0 + k;
+            const uint64_t dp[2] = {a.nw[kk], a.nd[kk]};
+            s.vals(dp, 2, 16);
+            s.bytes(a.words + kk * kWin, 4 * dp[0]);
+            s.bytes(a.dict32 + kk * kWin, 4 * dp[1]);
+        }
+    } else {  // .15_17
+        const uint64_t n15 = tw[kC15], n17 = tw[kC17];
+        s.vals(&n15, 1, 4);
+        s.bytes(a.id15 + seg, 4 * n15);
+        s.bytes(a.val15 + seg, 8 * n15);
+        s.vals(&n17, 1, 4);
+        s.bytes(a.id17 + seg, 4 * n17);
+    }
+}
+
+struct SizeSink {
+    uint64_t n = 0;
+    __device__ void vals(const uint64_t*, int, uint64_t nbytes) { n += nbytes; }
+    __device__ void bytes(const void*, uint64_t nbytes) { n += nbytes; }
+};
+
+// one thread per (window, file): chunk sizes, then their offsets per file
+__global__ __launch_bounds__(128) void k_col_sizes(WriteArgs a, uint32_t nwin)
+{
+    __shared__ uint64_t sz[kSubWindows * 5];
+    const uint32_t t = threadIdx.x;
+    if (t < nwin * 5) {
+        SizeSink s;
+        chunk_pieces(a, t / 5, (int)(t % 5), s);
+        sz[t] = s.n;
+    }
+    __syncthreads();
+    if (t < 5) {
+        uint64_t run = 0;
+        for (uint32_t w = 0; w < nwin; w++) {
+            a.foff[w * 5 + t] = run;
+            run += sz[w * 5 + t];
+        }
+        a.ftot[t] = run;
+    }
+}
+
+// the whole block copies one piece after the other into its chunk
+struct CopySink {
+    uint8_t* dst;
+    uint64_t* stage;  // LDS, 8 values
+    __device__ void vals(const uint64_t* v, int count, uint64_t nbytes)
+    {
+        if (threadIdx.x < (uint32_t)count) stage[threadIdx.x] = v[threadIdx.x];
+        __syncthreads();
+        const uint8_t* src = reinterpret_cast<const uint8_t*>(stage);
+        for (uint32_t b = threadIdx.x; b < nbytes; b += blockDim.x) dst[b] = src[b];
+        __syncthreads();
+        dst += nbytes;
+    }
+    __device__ void bytes(const void* p, uint64_t nbytes)
+    {
+        const uint8_t* src = static_cast<const uint8_t*>(p);
+        if (((reinterpret_cast<uintptr_t>(src) | reinterpret_cast<uintptr_t>(dst)) & 3) == 0) {
+            const uint64_t nw = nbytes / 4;
+            for (uint64_t q = threadIdx.x; q < nw; q += blockDim.x)
+                reinterpret_cast<uint32_t*>(dst)[q] = reinterpret_cast<const uint32_t*>(src)[q];
+            for (uint64_t b = nw * 4 + threadIdx.x; b < nbytes; b += blockDim.x) dst[b] = src[b];
+        } else {
+            for (uint64_t b = threadIdx.x; b < nbytes; b += blockDim.x) dst[b] = src[b];
+        }
+        dst += nbytes;
+    }
+};
+
+__global__ __launch_bounds__(256) void k_col_write(WriteArgs a)
+{
+    __shared__ uint64_t stage[8];
+    const uint32_t w = blockIdx.x;
+    const int f = (int)blockIdx.y;
+    CopySink s;
+    s.dst = a.out + (size_t)f * a.cap + a.foff[w * 5 + f];
+    s.stage = stage;
+    chunk_pieces(a, w, f, s);
+}
+
+}  // namespace
+
+#define HIPCHK(x)                                                                                 \
+    do {                                                                                          \
+        hipError_t e_ = (x);                                                                      \
+        if (e_ != hipSuccess) return fail(GSNP_E_DEVICE, std::string(#x) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+static int alloc_cols(ColBufs& b, uint32_t win)
+{
+    if (b.win >= win) return GSNP_OK;
+    const size_t S = (size_t)win * kWin;
+    const size_t nb = (size_t)win * kWinBlocks;
+    const size_t wk = (size_t)win * kNDict;
+    b.win = 0;
+    HIPCHK(b.cnt.alloc(nb * kNC));
+    HIPCHK(b.boff.alloc(nb * kNC));
+    HIPCHK(b.tot.alloc((size_t)win * kNC));
+    for (DevBuf<int32_t>* p : {&b.id4, &b.id6, &b.idn, &b.nidx, &b.x12i, &b.x12v, &b.x13i, &b.x13v, &b.cid, &b.id15,
+                               &b.id17})
+        HIPCHK(p->alloc(S));
+    HIPCHK(b.v4.alloc(S));
+    HIPCHK(b.v6.alloc(S));
+    HIPCHK(b.code3.alloc(S / 16));
+    HIPCHK(b.code10.alloc(S / 16));
+    HIPCHK(b.val15.alloc(S));
+    HIPCHK(b.rstart.alloc(6 * S));
+    HIPCHK(b.in32.alloc(kNDict * S));
+    HIPCHK(b.in64.alloc(S));
+    HIPCHK(b.tab.alloc(wk * kTabStride));
+    HIPCHK(b.trank.alloc(wk * kTabStride));
+    HIPCHK(b.dkeys.alloc(wk * kWin));
+    HIPCHK(b.dslot.alloc(wk * kWin));
+    HIPCHK(b.words.alloc(wk * kWin));
+    HIPCHK(b.dict32.alloc(wk * kWin));
+    HIPCHK(b.dict64.alloc(S));
+    HIPCHK(b.nd.alloc(wk));
+    HIPCHK(b.nw.alloc(wk));
+    HIPCHK(b.foff.alloc((size_t)win * 5));
+    HIPCHK(b.ftot.alloc(5));
+    b.win = win;
+    return GSNP_OK;
+}
+
+int encode_columns(Ctx& c, uint32_t offset, uint32_t ws_abs, uint32_t n, ColumnChunks& out)
+{
+    if (!n || ws_abs % kWin) return fail(GSNP_E_ARG, "column chunks start on a window boundary and hold sites");
+    hipStream_t st = c.stream;
+    ColBufs& b = c.cols;
+    const uint32_t want = std::min<uint32_t>(kSubWindows, (n + kWin - 1) / kWin);
+    if (b.win < want) {
+        HIPCHK(hipStreamSynchronize(st));
+        int rc = alloc_cols(b, want);
+        if (rc) return rc;
+    }
+    DictArgs da{};
+    for (int e = 0; e < 32; e++) da.pow2bits[e] = (uint8_t)dict_bits(1ull << e);
+    const size_t S = (size_t)b.win * kWin;
+    for (uint32_t s0 = 0; s0 < n; s0 += kSubWindows * kWin) {
+        const uint32_t ns = std::min<uint32_t>(n - s0, kSubWindows * kWin);
+        const uint32_t nwin = (ns + kWin - 1) / kWin;
+        const uint32_t nblk = (ns + kCB - 1) / kCB;
+        ColIn in;
+        in.ref = c.ref.p + ws_abs + s0;
+        in.a = c.site_view(offset + s0);
+        in.res = c.result_view(offset + s0);
+        in.n = ns;
+        hipLaunchKernelGGL(k_col_count, dim3(nblk), dim3(kCB), 0, st, in, b.cnt.p);
+        hipLaunchKernelGGL(k_col_scan, dim3(nwin), dim3(256), 0, st, b.cnt.p, ns, b.boff.p, b.tot.p);
+        EmitOut eo;
+        eo.id4 = b.id4.p, eo.id6 = b.id6.p, eo.idn = b.idn.p, eo.nidx = b.nidx.p;
+        eo.x12i = b.x12i.p, eo.x12v = b.x12v.p, eo.x13i = b.x13i.p, eo.x13v = b.x13v.p;
+        eo.cid = b.cid.p, eo.id15 = b.id15.p, eo.id17 = b.id17.p;
+        eo.v4 = b.v4.p, eo.v6 = b.v6.p, eo.code3 = b.code3.p, eo.code10 = b.code10.p;
+        eo.rstart = b.rstart.p, eo.in32 = b.in32.p, eo.in64 = b.in64.p, eo.val15 = b.val15.p;
+        eo.stride = S;
+        hipLaunchKernelGGL(k_col_emit, dim3(nblk), dim3(kCB), 0, st, in, b.boff.p, eo);
+        hipLaunchKernelGGL(k_col_runlen, dim3(nwin, 6), dim3(256), 0, st, ns, b.tot.p, b.rstart.p, b.in32.p, S);
+        da.tot = b.tot.p, da.in32 = b.in32.p, da.in64 = b.in64.p, da.stride = S;
+        da.tab = b.tab.p, da.trank = b.trank.p, da.dkeys = b.dkeys.p, da.dslot = b.dslot.p;
+        da.words = b.words.p, da.dict32 = b.dict32.p, da.dict64 = b.dict64.p, da.nd = b.nd.p, da.nw = b.nw.p;
+        hipLaunchKernelGGL(k_col_dict, dim3(nwin, kNDict), dim3(kDictThreads), 0, st, da);
+        WriteArgs wa;
+        wa.n = ns, wa.tot = b.tot.p, wa.nd = b.nd.p, wa.nw = b.nw.p;
+        wa.id4 = b.id4.p, wa.id6 = b.id6.p, wa.idn = b.idn.p, wa.nidx = b.nidx.p;
+        wa.x12i = b.x12i.p, wa.x12v = b.x12v.p, wa.x13i = b.x13i.p, wa.x13v = b.x13v.p;
+        wa.cid = b.cid.p, wa.id15 = b.id15.p, wa.id17 = b.id17.p, wa.v4 = b.v4.p, wa.v6 = b.v6.p;
+        wa.code3 = b.code3.p, wa.code10 = b.code10.p, wa.words = b.words.p, wa.dict32 = b.dict32.p;
+        wa.dict64 = b.dict64.p, wa.val15 = b.val15.p, wa.foff = b.foff.p, wa.ftot = b.ftot.p;
+        wa.out = b.stream.p, wa.cap = b.stream_cap;
+        hipLaunchKernelGGL(k_col_sizes, dim3(1), dim3(128), 0, st, wa, nwin);
+        HIPCHK(hipGetLastError());
+        uint64_t tot[5];
+        HIPCHK(hipMemcpyAsync(tot, b.ftot.p, sizeof tot, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        const uint64_t need = (*std::max_element(tot, tot + 5) + 255) & ~255ull;  // files stay 256-byte aligned
+        if (need > b.stream_cap) {
+            HIPCHK(b.stream.alloc(5 * need));
+            b.stream_cap = need;
+            wa.out = b.stream.p, wa.cap = need;
+        }
+        hipLaunchKernelGGL(k_col_write, dim3(nwin, 5), dim3(256), 0, st, wa);
+        HIPCHK(hipGetLastError());
+        for (int f = 0; f < 5; f++) {
+            std::vector<uint8_t>& v = out.file[f];
+            const size_t at = v.size();
+            v.resize(at + tot[f]);
+            if (tot[f])
+                HIPCHK(hipMemcpyAsync(v.data() + at, b.stream.p + (size_t)f * b.stream_cap, tot[f],
+                                      hipMemcpyDeviceToHost, st));
+        }
+        HIPCHK(hipStreamSynchronize(st));
+        for (uint32_t w = 0; w < nwin; w++) {
+            out.starts.push_back((uint64_t)ws_abs + s0 + (uint64_t)w * kWin + 1);
+            out.sizes.push_back(std::min<uint32_t>(kWin, ns - w * kWin));
+        }
+    }
+    return GSNP_OK;
+}
+
+}  // namespace gsnp

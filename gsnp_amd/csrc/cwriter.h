@@ -8,6 +8,8 @@
 #include <string>
 #include <vector>
 
+#include "cols.h"
+
 namespace gsnp {
 
 // Host copy of one called window, the inputs of every column.
@@ -41,6 +43,8 @@ class CWriter {
 public:
     int open(const std::string& prefix, std::string& err);
     void write_window(const std::string& name, uint32_t ws, const WindowCols& w);
+    // chunks encoded on the device (encode_columns), appended in window order
+    void write_chunks(const std::string& name, const ColumnChunks& ch);
     int close(std::string& err);
 
 private:

@@ -100,6 +100,25 @@ struct GraphSlot {
     ~GraphSlot() { reset(); }
 };
 
+// Device buffers of the -C column encoder, sized for `win` windows of
+// GSNP_WIN sites (segment w of each per-window array starts at w*GSNP_WIN).
+struct ColBufs {
+    uint32_t win = 0;
+    DevBuf<uint32_t> cnt, boff, tot;  // per 256-site block and per window: 16 counters
+    DevBuf<int32_t> id4, id6, idn, nidx, x12i, x12v, x13i, x13v, cid, id15, id17;
+    DevBuf<uint8_t> v4, v6;
+    DevBuf<uint32_t> code3, code10;  // 2-bit packed columns 3 and 10
+    DevBuf<double> val15;
+    DevBuf<uint32_t> rstart;  // run starts of the six run-length columns
+    DevBuf<uint32_t> in32;    // the 15 dictionary-coded arrays (32-bit ones)
+    DevBuf<uint64_t> in64;    // run values of column 16 (f64 bits)
+    DevBuf<uint64_t> tab, dkeys;
+    DevBuf<uint32_t> trank, dslot, words, dict32, nd, nw;
+    DevBuf<uint64_t> dict64, foff, ftot;
+    DevBuf<uint8_t> stream;
+    uint64_t stream_cap = 0;
+};
+
 struct Ctx {
     gsnp_params p{};
     hipStream_t stream = nullptr;
@@ -175,6 +194,7 @@ struct Ctx {
     DevBuf<int32_t> dict_mm, dict_vals;
     DevBuf<uint32_t> dict_bits, dict_wpre, dict_out;
     DevBuf<uint64_t> dict_bsum;
+    ColBufs cols;  // -C column encoder
 
     int alloc_window(uint32_t n_sites);
     SiteArrays site_view(uint32_t offset);

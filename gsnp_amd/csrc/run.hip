@@ -11,6 +11,7 @@
 #include <vector>
 
 #include "../../include/gsnp_abi.h"
+#include "cols.h"
 #include "cwriter.h"
 #include "engine.h"
 #include "inputs.h"
@@ -22,30 +23,6 @@ namespace gsnp {
         hipError_t e_ = (x);                                                                      \
         if (e_ != hipSuccess) return fail(GSNP_E_DEVICE, std::string(#x) + ": " + hipGetErrorString(e_)); \
     } while (0)
-
-// Device results of sites [offset, offset+C) to host columns.
-int fetch_cols(Ctx& c, uint32_t offset, uint32_t ws_abs, uint32_t C, const uint8_t* ref_codes, WindowCols& w)
-{
-    w.resize(C);
-    if (!C) return GSNP_OK;
-    std::memcpy(w.ori.data(), ref_codes + ws_abs, C);
-    for (uint32_t j = 0; j < C; j++) w.ori[j] &= 0xF;
-    auto cp = [&](void* dst, const void* src, size_t bytes) {
-        return hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c.stream);
-    };
-    HIPCHK(cp(w.base1.data(), c.base1.p + offset, C));
-    HIPCHK(cp(w.base2.data(), c.base2.p + offset, C));
-    HIPCHK(cp(w.type1.data(), c.type1.p + offset, C));
-    HIPCHK(cp(w.q_cns.data(), c.qcns.p + offset, 4ull * C));
-    HIPCHK(cp(w.rank.data(), c.rank.p + offset, 8ull * C));
-    HIPCHK(cp(w.depth.data(), c.sites.depth.p + offset, 4ull * C));
-    HIPCHK(cp(w.repeat.data(), c.sites.repeat.p + offset, 4ull * C));
-    HIPCHK(cp(w.qs.data(), c.sites.qs.p + offset, 16ull * C));
-    HIPCHK(cp(w.cu.data(), c.sites.cu.p + offset, 16ull * C));
-    HIPCHK(cp(w.ca.data(), c.sites.ca.p + offset, 16ull * C));
-    HIPCHK(hipStreamSynchronize(c.stream));
-    return GSNP_OK;
-}
 
 namespace {
 
@@ -203,7 +180,7 @@ int run_pipeline(const char* fasta, const char* soap, const char* dbsnp, const c
     CWriter cw;
     if (compress && cw.open(out, err)) return fail(GSNP_E_IO, err);
     std::vector<char> host_text;
-    WindowCols cols;
+    ColumnChunks chunks;
     gsnp_ctx* ctx = job->ctx;
     for (int bi = 0; bi < (int)job->blocks.size(); bi++) {
         rc = job_select(job.get(), bi);
@@ -213,14 +190,17 @@ int run_pipeline(const char* fasta, const char* soap, const char* dbsnp, const c
         const uint32_t N = (uint32_t)chr.code.size();
         auto on_results = [&](uint32_t offset, uint32_t ws, uint32_t n) -> int {
             if (!compress) return GSNP_OK;
-            // -C chunks are per reference window of the span
-            for (uint32_t s = ws; s < ws + n || (n == 0 && s == ws); s += GSNP_WIN) {
-                const uint32_t C = std::min<uint32_t>(GSNP_WIN, ws + n - s);
-                int r2 = fetch_cols(*ctx, offset + (s - ws), s, C, chr.code.data(), cols);
-                if (r2) return r2;
-                cw.write_window(chr.name, s, cols);
-                if (n == 0) break;
+            if (n == 0) {  // the empty window after a chromosome that fills its last window
+                cw.write_window(chr.name, ws, WindowCols());
+                return GSNP_OK;
             }
+            // one -C chunk per reference window of the span, encoded on the device
+            for (auto& f : chunks.file) f.clear();
+            chunks.starts.clear();
+            chunks.sizes.clear();
+            int r2 = encode_columns(*ctx, offset, ws, n, chunks);
+            if (r2) return r2;
+            cw.write_chunks(chr.name, chunks);
             return GSNP_OK;
         };
         auto on_text = [&](uint64_t nb) -> int {

@@ -149,14 +149,17 @@ def test_parameter_variants(datasets, tmp_path, name, flags, batched):
     assert out.read_bytes() == ref.read_bytes(), _first_diff(out, ref)
 
 
-@pytest.mark.parametrize("name", ["stress", "c2w3"])
-def test_compressed_output(datasets, tmp_path, name):
+@pytest.mark.parametrize("name,per", [("stress", 0), ("c2w3", 0), ("stress", 4), ("c2w3", 4)])
+def test_compressed_output(datasets, tmp_path, name, per):
+    """-C files (encoded on the device, one chunk per reference window; with
+    per > 0 the windows of a batched span are encoded together) are the
+    oracle's bytes, and decompress to the oracle's text."""
     import gsnp_amd
 
     ds = datasets(name)
-    print(f"[parity] -C {name}", flush=True)
+    print(f"[parity] -C {name} batched={per}", flush=True)
     gpu = tmp_path / f"{name}.gpu"
-    _gpu_run(ds, gpu, compress=True)
+    _gpu_run(ds, gpu, compress=True, batched=per)
     cpu = tmp_path / f"{name}.cpu"
     oracle_lib.run(ds.fasta, ds.soap, cpu, dbsnp=ds.dbsnp, compress=True, **ds.flags)
     for sfx in SUFFIXES:
