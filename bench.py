@@ -66,6 +66,7 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="keep the default fused launch shape")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint leg (2 Mb miscalibrated workload)")
+    ap.add_argument("--no-columns", action="store_true", help="skip the -C leg (column chunks instead of text)")
     ap.add_argument("--adjoint-evals", type=int, default=10)
     ap.add_argument("--fd-entries", type=int, default=256)
     ap.add_argument("--refit-steps", type=int, default=50)
@@ -168,6 +169,25 @@ def main() -> int:
     call_ms = sum(stage_ms.get(k, 0) for k in ("sort", "likelihood", "posterior", "rank"))
     call_gbs = (4 * recs + 80 * sites) / (call_ms / 1e3) / 1e9 if call_ms else None
 
+    # -C leg: the same pass with each window's column chunks encoded on the
+    # device and copied to host memory (no file writes) instead of the text
+    columns = None
+    if not args.no_columns:
+        batched = per if args.mode == "batched" else 0
+        job.columns(batched=batched)
+        job.profile(enable=True)
+        t0 = time.perf_counter()
+        c_bytes = 0
+        for _ in range(args.steps):
+            c_bytes = job.columns(batched=batched)
+        c_secs = (time.perf_counter() - t0) / args.steps
+        c_prof = job.profile(enable=False)
+        enc_ms = c_prof["columns"][0] / args.steps
+        columns = {"metric": "called sites/sec with -C column chunks", "value": round(chr_len / c_secs, 1),
+                   "unit": "sites/s", "ms_per_step": round(c_secs * 1e3, 3), "encoder_ms_per_step": round(enc_ms, 4),
+                   "chunk_bytes": c_bytes,
+                   "note": "encoder = the device kernels of the chunks; the step also copies them to host memory"}
+
     cpu = None
     if not args.no_cpu:
         sys.path.insert(0, str(ROOT / "tests"))
@@ -212,6 +232,8 @@ def main() -> int:
     }
     if other:
         line["other_mode"] = other
+    if columns:
+        line["columns"] = columns
     if adjoint:
         line["adjoint"] = adjoint
     print(json.dumps(line), flush=True)

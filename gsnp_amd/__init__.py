@@ -126,6 +126,7 @@ SIGNATURES = {
                                      ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint64)]),
     "gsnp_job_select": (ctypes.c_int, [_P, ctypes.c_int]),
     "gsnp_job_call": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_uint64)]),
+    "gsnp_job_columns": (ctypes.c_int, [_P, ctypes.c_int, ctypes.POINTER(ctypes.c_uint64)]),
     "gsnp_job_context": (_P, [_P]),
     "gsnp_job_close": (None, [_P]),
     "gsnp_synth_write": (ctypes.c_longlong, [ctypes.c_int, ctypes.c_ulonglong, ctypes.c_double, ctypes.c_char_p,
@@ -217,7 +218,7 @@ def format_g(v: float) -> str:
     return buf.raw[:n].decode()
 
 
-STAGES = ["site_build", "sort", "likelihood", "posterior", "rank", "format", "recycle", "fused", "adjoint"]
+STAGES = ["site_build", "sort", "likelihood", "posterior", "rank", "format", "recycle", "fused", "adjoint", "columns"]
 
 
 class Job:
@@ -249,6 +250,13 @@ class Job:
     def call(self, batched: int = 0, emit_text: bool = True) -> int:
         nb = ctypes.c_uint64()
         check(lib().gsnp_job_call(self.h, int(batched), int(bool(emit_text)), ctypes.byref(nb)))
+        return nb.value
+
+    def columns(self, batched: int = 0) -> int:
+        """A calling pass whose -C chunks are encoded on the device and copied to
+        host memory without writing files (gsnp_job_columns); returns their bytes."""
+        nb = ctypes.c_uint64()
+        check(lib().gsnp_job_columns(self.h, int(batched), ctypes.byref(nb)))
         return nb.value
 
     @property

@@ -677,6 +677,7 @@ int encode_columns(Ctx& c, uint32_t offset, uint32_t ws_abs, uint32_t n, ColumnC
         in.a = c.site_view(offset + s0);
         in.res = c.result_view(offset + s0);
         in.n = ns;
+        c.timer.begin(kStColumns, st);
         hipLaunchKernelGGL(k_col_count, dim3(nblk), dim3(kCB), 0, st, in, b.cnt.p);
         hipLaunchKernelGGL(k_col_scan, dim3(nwin), dim3(256), 0, st, b.cnt.p, ns, b.boff.p, b.tot.p);
         EmitOut eo;
@@ -701,6 +702,7 @@ int encode_columns(Ctx& c, uint32_t offset, uint32_t ws_abs, uint32_t n, ColumnC
         wa.dict64 = b.dict64.p, wa.val15 = b.val15.p, wa.foff = b.foff.p, wa.ftot = b.ftot.p;
         wa.out = b.stream.p, wa.cap = b.stream_cap;
         hipLaunchKernelGGL(k_col_sizes, dim3(1), dim3(128), 0, st, wa, nwin);
+        c.timer.end(kStColumns, st);
         HIPCHK(hipGetLastError());
         uint64_t tot[5];
         HIPCHK(hipMemcpyAsync(tot, b.ftot.p, sizeof tot, hipMemcpyDeviceToHost, st));
@@ -711,7 +713,9 @@ int encode_columns(Ctx& c, uint32_t offset, uint32_t ws_abs, uint32_t n, ColumnC
             b.stream_cap = need;
             wa.out = b.stream.p, wa.cap = need;
         }
+        c.timer.begin(kStColumns, st);
         hipLaunchKernelGGL(k_col_write, dim3(nwin, 5), dim3(256), 0, st, wa);
+        c.timer.end(kStColumns, st);
         HIPCHK(hipGetLastError());
         for (int f = 0; f < 5; f++) {
             std::vector<uint8_t>& v = out.file[f];

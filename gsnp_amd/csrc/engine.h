@@ -66,6 +66,7 @@ enum Stage : int {
     kStRecycle,
     kStFused,
     kStAdjoint,
+    kStColumns,
     kNumStages
 };
 

@@ -276,6 +276,28 @@ int gsnp_job_call(gsnp_job* job, int batched, int emit_text, uint64_t* nbytes)
     return rc;
 }
 
+int gsnp_job_columns(gsnp_job* job, int batched, uint64_t* nbytes)
+{
+    if (!job || job->cur < 0) return gsnp::fail(GSNP_E_ARG, "no chromosome selected");
+    const gsnp::ReadBlock& b = job->blocks[job->cur];
+    const uint32_t N = (uint32_t)job->g.chr[b.chr].code.size();
+    uint64_t total = 0;
+    gsnp::ColumnChunks ch;
+    auto on_results = [&](uint32_t offset, uint32_t ws, uint32_t n) -> int {
+        if (n == 0) return GSNP_OK;
+        for (auto& f : ch.file) f.clear();
+        ch.starts.clear();
+        ch.sizes.clear();
+        int r = gsnp::encode_columns(*job->ctx, offset, ws, n, ch);
+        for (auto& f : ch.file) total += f.size();
+        return r;
+    };
+    auto on_text = [](uint64_t) -> int { return GSNP_OK; };
+    int rc = gsnp::call_chromosome(job->ctx, b.pos, N, batched, false, on_results, on_text);
+    if (nbytes) *nbytes = total;
+    return rc;
+}
+
 gsnp_ctx* gsnp_job_context(gsnp_job* job) { return job ? job->ctx : nullptr; }
 
 void gsnp_job_close(gsnp_job* job) { delete job; }

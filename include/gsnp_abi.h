@@ -197,7 +197,7 @@ int gsnp_sort_likelihood(gsnp_ctx* ctx, uint32_t* records, uint32_t pitch, const
  * stages site build, sort, likelihood, posterior, rank-sum, format, recycle,
  * fused, adjoint (9 entries), then turns recording on or off.  Replaces the cutil
  * timers of the reference (kernel/cuda_header.h:37-50). */
-#define GSNP_N_STAGES 9
+#define GSNP_N_STAGES 10
 int gsnp_profile(gsnp_ctx* ctx, int enable, double* stage_ms, uint64_t* stage_count, int n_stages);
 
 /* Observation records the likelihood stage has consumed since the last
@@ -221,6 +221,12 @@ int gsnp_job_select(gsnp_job* job, int index);
  * *nbytes sums their size.  Replaces the window loop of new_soap2cns,
  * cpp/call_genotype.cc:474-644. */
 int gsnp_job_call(gsnp_job* job, int batched, int emit_text, uint64_t* nbytes);
+/* the same pass with the -C column chunks of every window encoded on the
+ * device and copied to host memory, but not written to files (the encoder
+ * of gsnp_run with compress = 1); *nbytes sums the chunk bytes.  Replaces the
+ * extraction and compression of Call_win::call_cns2 with -C,
+ * cpp/compress.h:33-1312. */
+int gsnp_job_columns(gsnp_job* job, int batched, uint64_t* nbytes);
 gsnp_ctx* gsnp_job_context(gsnp_job* job);
 void gsnp_job_close(gsnp_job* job);
 

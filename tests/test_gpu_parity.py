@@ -172,6 +172,28 @@ def test_compressed_output(datasets, tmp_path, name, per):
     assert txt.read_bytes() == ref.read_bytes(), _first_diff(txt, ref)
 
 
+def test_compressed_long_span(built, tmp_path):
+    """A 5 Mb chromosome (20 windows, more than one encoder pass of 16 windows):
+    the -C files of one batched span equal those of the per-window path, and
+    decompress to the batched span's text."""
+    import gsnp_amd
+
+    pre = tmp_path / "c2h"
+    _, rl = gsnp_amd.synth(2, 7, 0.5, pre)
+    fa, so, snp = f"{pre}.fa", f"{pre}.soap", f"{pre}.snp"
+    flags = dict(read_length=rl, refine_mode=1)
+    print("[parity] -C long span", flush=True)
+    a, b, txt = tmp_path / "a", tmp_path / "b", tmp_path / "t.cns"
+    gsnp_amd.run(fa, so, a, dbsnp=snp, compress=True, batched=0, **flags)
+    gsnp_amd.run(fa, so, b, dbsnp=snp, compress=True, batched=64, **flags)
+    for sfx in SUFFIXES:
+        assert filecmp.cmp(str(a) + sfx, str(b) + sfx, shallow=False), f"{sfx} differs"
+    gsnp_amd.run(fa, so, txt, dbsnp=snp, batched=64, **flags)
+    dec = tmp_path / "b.dec.cns"
+    gsnp_amd.decompress(b, dec)
+    assert dec.read_bytes() == txt.read_bytes(), _first_diff(dec, txt)
+
+
 def test_sort_likelihood_unit(built):
     """Random record sets through the sort + likelihood kernels vs the oracle."""
     import gsnp_amd
