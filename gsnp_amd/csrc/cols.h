@@ -3,24 +3,28 @@
 
 #include <stdint.h>
 
-#include <vector>
+#include <functional>
 
 namespace gsnp {
 
 struct Ctx;
 
-// The chunks of a span's windows, in window order: the bytes each of the five
-// column files receives (.3_4_6, .5_7_8_9_14_16, .10, .11_12_13, .15_17) and
-// the start (1-based) and size of every window for the index file .1_2.
-struct ColumnChunks {
-    std::vector<uint8_t> file[5];
-    std::vector<uint64_t> starts;
-    std::vector<uint32_t> sizes;
+// One encoder pass (up to 16 windows) of a span: the bytes each of the five
+// column files receives (.3_4_6, .5_7_8_9_14_16, .10, .11_12_13, .15_17), in
+// window order, in pinned host memory that the next pass reuses, and the
+// start (1-based) and size of every window for the index file .1_2.
+struct ColumnPass {
+    const uint8_t* file[5];
+    uint64_t bytes[5];
+    const uint64_t* starts;
+    const uint32_t* sizes;
+    uint32_t n_windows;
 };
+using ColumnSink = std::function<int(const ColumnPass&)>;
 
 // Encodes the called sites [offset, offset+n) of the context, which hold the
 // chromosome positions [ws_abs, ws_abs+n) (ws_abs on a window boundary, n > 0),
-// one chunk per reference window, and appends them to `out`.
-int encode_columns(Ctx& c, uint32_t offset, uint32_t ws_abs, uint32_t n, ColumnChunks& out);
+// one chunk per reference window, and hands every pass to `sink`.
+int encode_columns(Ctx& c, uint32_t offset, uint32_t ws_abs, uint32_t n, const ColumnSink& sink);
 
 }  // namespace gsnp

@@ -23,6 +23,7 @@
 
 #include <algorithm>
 #include <string>
+#include <vector>
 
 #include "../../include/gsnp_abi.h"
 #include "cols.h"
@@ -310,15 +311,17 @@ __global__ __launch_bounds__(256) void k_col_emit(ColIn in, const uint32_t* boff
     }
 }
 
+// one thread per run: grid (runs / 256, column, window)
 __global__ __launch_bounds__(256) void k_col_runlen(uint32_t n, const uint32_t* tot, const uint32_t* rstart,
                                                     uint32_t* in32, size_t stride)
 {
-    const uint32_t w = blockIdx.x, c = blockIdx.y;
-    const uint32_t C = min(kWin, n - w * kWin);
+    const uint32_t c = blockIdx.y, w = blockIdx.z;
+    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t nrun = tot[(size_t)w * kNC + kCRun0 + c];
+    if (r >= nrun) return;
+    const uint32_t C = min(kWin, n - w * kWin);
     const uint32_t* rs = rstart + c * stride + (size_t)w * kWin;
-    uint32_t* len = in32 + (6 + c) * stride + (size_t)w * kWin;
-    for (uint32_t r = threadIdx.x; r < nrun; r += blockDim.x) len[r] = (r + 1 < nrun ? rs[r + 1] : C) - rs[r];
+    in32[(6 + c) * stride + (size_t)w * kWin + r] = (r + 1 < nrun ? rs[r + 1] : C) - rs[r];
 }
 
 struct DictArgs {
@@ -330,6 +333,7 @@ struct DictArgs {
     uint32_t* trank;
     uint64_t* dkeys;
     uint32_t* dslot;
+    uint32_t* eslot;  // the slot of each element
     uint32_t* words;
     uint32_t* dict32;
     uint64_t* dict64;
@@ -377,6 +381,7 @@ __global__ __launch_bounds__(kDictThreads) void k_col_dict(DictArgs a)
     uint32_t* trank = a.trank + wk * kTabStride;
     uint64_t* dk = a.dkeys + wk * kWin;
     uint32_t* ds = a.dslot + wk * kWin;
+    uint32_t* es = a.eslot + wk * kWin;
     if (t == 0) s_nd = 0;
     for (uint32_t h = t; h < cap; h += kDictThreads) tab[h] = kEmpty;
     __syncthreads();
@@ -398,6 +403,7 @@ __global__ __launch_bounds__(kDictThreads) void k_col_dict(DictArgs a)
             if (cur == key) break;
             h = (h + 1) & (cap - 1);
         }
+        es[e] = h;
     }
     __syncthreads();
     const uint32_t nd = s_nd;
@@ -437,11 +443,8 @@ __global__ __launch_bounds__(kDictThreads) void k_col_dict(DictArgs a)
         const uint32_t e0 = q * per, e1 = min(e0 + per, m);
         uint32_t acc = 0;
         for (uint32_t e = e0; e < e1; e++) {
-            const uint64_t key = key_at(e);
-            uint32_t h = slot_of(key, lg);
-            for (uint32_t probe = 0; probe < cap && tab_load(&tab[h]) != key; probe++) h = (h + 1) & (cap - 1);
             acc = bits >= 32 ? 0u : (acc << bits);
-            acc |= trank[h];
+            acc |= trank[es[e]];
         }
         wo[q] = acc;
     }
@@ -571,34 +574,43 @@ __global__ __launch_bounds__(128) void k_col_sizes(WriteArgs a, uint32_t nwin)
     }
 }
 
-// the whole block copies one piece after the other into its chunk
+constexpr uint32_t kWriteParts = 16;  // blocks per chunk
+
+// Block `part` of the chunk's kWriteParts blocks copies its slice of every
+// piece (slices start on 4-byte boundaries); part 0 also writes the headers.
 struct CopySink {
     uint8_t* dst;
+    uint32_t part;
     uint64_t* stage;  // LDS, 8 values
     __device__ void vals(const uint64_t* v, int count, uint64_t nbytes)
     {
-        if (threadIdx.x < (uint32_t)count) stage[threadIdx.x] = v[threadIdx.x];
-        __syncthreads();
-        const uint8_t* src = reinterpret_cast<const uint8_t*>(stage);
-        for (uint32_t b = threadIdx.x; b < nbytes; b += blockDim.x) dst[b] = src[b];
-        __syncthreads();
+        if (part == 0) {
+            if (threadIdx.x < (uint32_t)count) stage[threadIdx.x] = v[threadIdx.x];
+            __syncthreads();
+            const uint8_t* src = reinterpret_cast<const uint8_t*>(stage);
+            for (uint32_t b = threadIdx.x; b < nbytes; b += blockDim.x) dst[b] = src[b];
+            __syncthreads();
+        }
         dst += nbytes;
     }
     __device__ void bytes(const void* p, uint64_t nbytes)
     {
         const uint8_t* src = static_cast<const uint8_t*>(p);
+        const uint64_t lo = (nbytes * part / kWriteParts) & ~3ull;
+        const uint64_t hi = part + 1 == kWriteParts ? nbytes : (nbytes * (part + 1) / kWriteParts) & ~3ull;
         if (((reinterpret_cast<uintptr_t>(src) | reinterpret_cast<uintptr_t>(dst)) & 3) == 0) {
-            const uint64_t nw = nbytes / 4;
-            for (uint64_t q = threadIdx.x; q < nw; q += blockDim.x)
+            const uint64_t q0 = lo / 4, q1 = hi / 4;
+            for (uint64_t q = q0 + threadIdx.x; q < q1; q += blockDim.x)
                 reinterpret_cast<uint32_t*>(dst)[q] = reinterpret_cast<const uint32_t*>(src)[q];
-            for (uint64_t b = nw * 4 + threadIdx.x; b < nbytes; b += blockDim.x) dst[b] = src[b];
+            for (uint64_t b = q1 * 4 + threadIdx.x; b < hi; b += blockDim.x) dst[b] = src[b];
         } else {
-            for (uint64_t b = threadIdx.x; b < nbytes; b += blockDim.x) dst[b] = src[b];
+            for (uint64_t b = lo + threadIdx.x; b < hi; b += blockDim.x) dst[b] = src[b];
         }
         dst += nbytes;
     }
 };
 
+// grid (window, file, part)
 __global__ __launch_bounds__(256) void k_col_write(WriteArgs a)
 {
     __shared__ uint64_t stage[8];
@@ -606,6 +618,7 @@ __global__ __launch_bounds__(256) void k_col_write(WriteArgs a)
     const int f = (int)blockIdx.y;
     CopySink s;
     s.dst = a.out + (size_t)f * a.cap + a.foff[w * 5 + f];
+    s.part = blockIdx.z;
     s.stage = stage;
     chunk_pieces(a, w, f, s);
 }
@@ -643,6 +656,7 @@ static int alloc_cols(ColBufs& b, uint32_t win)
     HIPCHK(b.trank.alloc(wk * kTabStride));
     HIPCHK(b.dkeys.alloc(wk * kWin));
     HIPCHK(b.dslot.alloc(wk * kWin));
+    HIPCHK(b.eslot.alloc(wk * kWin));
     HIPCHK(b.words.alloc(wk * kWin));
     HIPCHK(b.dict32.alloc(wk * kWin));
     HIPCHK(b.dict64.alloc(S));
@@ -654,7 +668,7 @@ static int alloc_cols(ColBufs& b, uint32_t win)
     return GSNP_OK;
 }
 
-int encode_columns(Ctx& c, uint32_t offset, uint32_t ws_abs, uint32_t n, ColumnChunks& out)
+int encode_columns(Ctx& c, uint32_t offset, uint32_t ws_abs, uint32_t n, const ColumnSink& sink)
 {
     if (!n || ws_abs % kWin) return fail(GSNP_E_ARG, "column chunks start on a window boundary and hold sites");
     hipStream_t st = c.stream;
@@ -665,6 +679,8 @@ int encode_columns(Ctx& c, uint32_t offset, uint32_t ws_abs, uint32_t n, ColumnC
         int rc = alloc_cols(b, want);
         if (rc) return rc;
     }
+    std::vector<uint64_t> starts;
+    std::vector<uint32_t> sizes;
     DictArgs da{};
     for (int e = 0; e < 32; e++) da.pow2bits[e] = (uint8_t)dict_bits(1ull << e);
     const size_t S = (size_t)b.win * kWin;
@@ -688,9 +704,10 @@ int encode_columns(Ctx& c, uint32_t offset, uint32_t ws_abs, uint32_t n, ColumnC
         eo.rstart = b.rstart.p, eo.in32 = b.in32.p, eo.in64 = b.in64.p, eo.val15 = b.val15.p;
         eo.stride = S;
         hipLaunchKernelGGL(k_col_emit, dim3(nblk), dim3(kCB), 0, st, in, b.boff.p, eo);
-        hipLaunchKernelGGL(k_col_runlen, dim3(nwin, 6), dim3(256), 0, st, ns, b.tot.p, b.rstart.p, b.in32.p, S);
+        hipLaunchKernelGGL(k_col_runlen, dim3(kWinBlocks, 6, nwin), dim3(256), 0, st, ns, b.tot.p, b.rstart.p,
+                           b.in32.p, S);
         da.tot = b.tot.p, da.in32 = b.in32.p, da.in64 = b.in64.p, da.stride = S;
-        da.tab = b.tab.p, da.trank = b.trank.p, da.dkeys = b.dkeys.p, da.dslot = b.dslot.p;
+        da.tab = b.tab.p, da.trank = b.trank.p, da.dkeys = b.dkeys.p, da.dslot = b.dslot.p, da.eslot = b.eslot.p;
         da.words = b.words.p, da.dict32 = b.dict32.p, da.dict64 = b.dict64.p, da.nd = b.nd.p, da.nw = b.nw.p;
         hipLaunchKernelGGL(k_col_dict, dim3(nwin, kNDict), dim3(kDictThreads), 0, st, da);
         WriteArgs wa;
@@ -714,22 +731,39 @@ int encode_columns(Ctx& c, uint32_t offset, uint32_t ws_abs, uint32_t n, ColumnC
             wa.out = b.stream.p, wa.cap = need;
         }
         c.timer.begin(kStColumns, st);
-        hipLaunchKernelGGL(k_col_write, dim3(nwin, 5), dim3(256), 0, st, wa);
+        hipLaunchKernelGGL(k_col_write, dim3(nwin, 5, kWriteParts), dim3(256), 0, st, wa);
         c.timer.end(kStColumns, st);
         HIPCHK(hipGetLastError());
+        const uint64_t all = tot[0] + tot[1] + tot[2] + tot[3] + tot[4];
+        if (all > b.host_cap) {
+            if (b.host) HIPCHK(hipHostFree(b.host));
+            b.host = nullptr;
+            b.host_cap = 0;
+            HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&b.host), all, hipHostMallocDefault));
+            b.host_cap = all;
+        }
+        ColumnPass pass;
+        uint64_t at = 0;
         for (int f = 0; f < 5; f++) {
-            std::vector<uint8_t>& v = out.file[f];
-            const size_t at = v.size();
-            v.resize(at + tot[f]);
             if (tot[f])
-                HIPCHK(hipMemcpyAsync(v.data() + at, b.stream.p + (size_t)f * b.stream_cap, tot[f],
+                HIPCHK(hipMemcpyAsync(b.host + at, b.stream.p + (size_t)f * b.stream_cap, tot[f],
                                       hipMemcpyDeviceToHost, st));
+            pass.file[f] = b.host + at;
+            pass.bytes[f] = tot[f];
+            at += tot[f];
         }
         HIPCHK(hipStreamSynchronize(st));
+        starts.clear();
+        sizes.clear();
         for (uint32_t w = 0; w < nwin; w++) {
-            out.starts.push_back((uint64_t)ws_abs + s0 + (uint64_t)w * kWin + 1);
-            out.sizes.push_back(std::min<uint32_t>(kWin, ns - w * kWin));
+            starts.push_back((uint64_t)ws_abs + s0 + (uint64_t)w * kWin + 1);
+            sizes.push_back(std::min<uint32_t>(kWin, ns - w * kWin));
         }
+        pass.starts = starts.data();
+        pass.sizes = sizes.data();
+        pass.n_windows = nwin;
+        int rc = sink(pass);
+        if (rc) return rc;
     }
     return GSNP_OK;
 }

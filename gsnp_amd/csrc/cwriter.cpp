@@ -312,15 +312,15 @@ void CWriter::write_window(const std::string& name, uint32_t ws, const WindowCol
     }
 }
 
-void CWriter::write_chunks(const std::string& name, const ColumnChunks& ch)
+void CWriter::write_chunks(const std::string& name, const ColumnPass& ch)
 {
-    for (size_t i = 0; i < ch.sizes.size(); i++) {
+    for (uint32_t i = 0; i < ch.n_windows; i++) {
         names_.push_back(name);
         starts_.push_back(ch.starts[i]);
         sizes_.push_back(ch.sizes[i]);
     }
     FILE* fs[5] = {f346_, f5_, f10_, f11_, f15_};
-    for (int f = 0; f < 5; f++) put(fs[f], ch.file[f].data(), 1, ch.file[f].size());
+    for (int f = 0; f < 5; f++) put(fs[f], ch.file[f], 1, ch.bytes[f]);
 }
 
 int CWriter::close(std::string& err)

@@ -44,7 +44,7 @@ public:
     int open(const std::string& prefix, std::string& err);
     void write_window(const std::string& name, uint32_t ws, const WindowCols& w);
     // chunks encoded on the device (encode_columns), appended in window order
-    void write_chunks(const std::string& name, const ColumnChunks& ch);
+    void write_chunks(const std::string& name, const ColumnPass& ch);
     int close(std::string& err);
 
 private:

@@ -114,10 +114,19 @@ struct ColBufs {
     DevBuf<uint32_t> in32;    // the 15 dictionary-coded arrays (32-bit ones)
     DevBuf<uint64_t> in64;    // run values of column 16 (f64 bits)
     DevBuf<uint64_t> tab, dkeys;
-    DevBuf<uint32_t> trank, dslot, words, dict32, nd, nw;
+    DevBuf<uint32_t> trank, dslot, eslot, words, dict32, nd, nw;
     DevBuf<uint64_t> dict64, foff, ftot;
     DevBuf<uint8_t> stream;
     uint64_t stream_cap = 0;
+    uint8_t* host = nullptr;  // pinned copy of the five file streams of a pass
+    uint64_t host_cap = 0;
+    ColBufs() = default;
+    ColBufs(const ColBufs&) = delete;
+    ColBufs& operator=(const ColBufs&) = delete;
+    ~ColBufs()
+    {
+        if (host) (void)hipHostFree(host);
+    }
 };
 
 struct Ctx {

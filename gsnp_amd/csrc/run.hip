@@ -180,7 +180,6 @@ int run_pipeline(const char* fasta, const char* soap, const char* dbsnp, const c
     CWriter cw;
     if (compress && cw.open(out, err)) return fail(GSNP_E_IO, err);
     std::vector<char> host_text;
-    ColumnChunks chunks;
     gsnp_ctx* ctx = job->ctx;
     for (int bi = 0; bi < (int)job->blocks.size(); bi++) {
         rc = job_select(job.get(), bi);
@@ -195,13 +194,10 @@ int run_pipeline(const char* fasta, const char* soap, const char* dbsnp, const c
                 return GSNP_OK;
             }
             // one -C chunk per reference window of the span, encoded on the device
-            for (auto& f : chunks.file) f.clear();
-            chunks.starts.clear();
-            chunks.sizes.clear();
-            int r2 = encode_columns(*ctx, offset, ws, n, chunks);
-            if (r2) return r2;
-            cw.write_chunks(chr.name, chunks);
-            return GSNP_OK;
+            return encode_columns(*ctx, offset, ws, n, [&](const ColumnPass& pass) -> int {
+                cw.write_chunks(chr.name, pass);
+                return GSNP_OK;
+            });
         };
         auto on_text = [&](uint64_t nb) -> int {
             host_text.resize(nb);
@@ -282,15 +278,12 @@ int gsnp_job_columns(gsnp_job* job, int batched, uint64_t* nbytes)
     const gsnp::ReadBlock& b = job->blocks[job->cur];
     const uint32_t N = (uint32_t)job->g.chr[b.chr].code.size();
     uint64_t total = 0;
-    gsnp::ColumnChunks ch;
     auto on_results = [&](uint32_t offset, uint32_t ws, uint32_t n) -> int {
         if (n == 0) return GSNP_OK;
-        for (auto& f : ch.file) f.clear();
-        ch.starts.clear();
-        ch.sizes.clear();
-        int r = gsnp::encode_columns(*job->ctx, offset, ws, n, ch);
-        for (auto& f : ch.file) total += f.size();
-        return r;
+        return gsnp::encode_columns(*job->ctx, offset, ws, n, [&](const gsnp::ColumnPass& pass) -> int {
+            for (int f = 0; f < 5; f++) total += pass.bytes[f];
+            return GSNP_OK;
+        });
     };
     auto on_text = [](uint64_t) -> int { return GSNP_OK; };
     int rc = gsnp::call_chromosome(job->ctx, b.pos, N, batched, false, on_results, on_text);
