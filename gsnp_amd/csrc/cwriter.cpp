@@ -19,32 +19,8 @@ void put(FILE* f, const void* p, size_t sz, size_t n)
 
 bool get(FILE* f, void* p, size_t sz, size_t n) { return n == 0 || std::fread(p, sz, n, f) == n; }
 
-// 2-bit codes A0 C1 G2 T3, 16 per word, MSB first, last word right-aligned
-// (kernel/compress_kernel.cu:85-153); anything else encodes as A.
-uint32_t code2(char b)
-{
-    switch (b) {
-    case 'C': case 'c': return 1;
-    case 'G': case 'g': return 2;
-    case 'T': case 't': return 3;
-    default: return 0;
-    }
-}
-
-std::vector<uint32_t> encode2(const std::vector<char>& s)
-{
-    const uint64_t n = s.size();
-    const uint64_t nw = (uint64_t)std::ceil(n / 16.0);
-    std::vector<uint32_t> w(nw, 0);
-    for (uint64_t i = 0; i < nw; i++) {
-        const uint64_t m = (i == nw - 1) ? n - i * 16 : 16;
-        uint32_t x = 0;
-        for (uint64_t k = 0; k < m; k++) x = (x << 2) | code2(s[i * 16 + k]);
-        w[i] = x;
-    }
-    return w;
-}
-
+// 2-bit words, A0 C1 G2 T3, 16 per word, MSB first, the last word holding
+// the remainder right-aligned (kernel/compress_kernel.cu:104-153)
 void decode2(const uint32_t* w, uint64_t nw, uint64_t n, char* out)
 {
     static const char T[4] = {'A', 'C', 'G', 'T'};
@@ -53,39 +29,6 @@ void decode2(const uint32_t* w, uint64_t nw, uint64_t n, char* out)
         for (int k = 0; k < 16; k++) out[i * 16 + k] = T[(w[i] >> (30 - 2 * k)) & 3];
     const uint64_t left = n - (nw - 1) * 16;
     for (uint64_t k = 0; k < left; k++) out[(nw - 1) * 16 + k] = T[(w[nw - 1] >> (2 * (left - k - 1))) & 3];
-}
-
-template <class T>
-void rle(const std::vector<T>& v, std::vector<T>& val, std::vector<uint32_t>& len)
-{
-    val.clear();
-    len.clear();
-    for (size_t i = 0; i < v.size(); i++) {
-        if (!val.empty() && val.back() == v[i])
-            len.back()++;
-        else
-            val.push_back(v[i]), len.push_back(1);
-    }
-}
-
-template <class T>
-void write_rle_column(FILE* f, const std::vector<T>& col)
-{
-    std::vector<T> rv;
-    std::vector<uint32_t> rl;
-    rle(col, rv, rl);
-    std::vector<uint32_t> wv, wl;
-    std::vector<T> dv;
-    std::vector<uint32_t> dl;
-    const uint64_t nwv = dict_encode(rv.data(), rv.size(), wv, dv);
-    const uint64_t nwl = dict_encode(rl.data(), rl.size(), wl, dl);
-    const uint64_t param[6] = {rv.size(), nwv, nwl, dv.size(), dl.size(),
-                               4 * nwv + 4 * nwl + sizeof(T) * dv.size() + 4 * dl.size()};
-    put(f, param, 8, 6);
-    put(f, wv.data(), 4, nwv);
-    put(f, wl.data(), 4, nwl);
-    put(f, dv.data(), sizeof(T), dv.size());
-    put(f, dl.data(), 4, dl.size());
 }
 
 template <class T>
@@ -129,31 +72,6 @@ int dict_bits(uint64_t n)
     return b == 0 ? 1 : b;
 }
 
-template <class T>
-uint64_t dict_encode(const T* in, uint64_t n, std::vector<uint32_t>& words, std::vector<T>& dict)
-{
-    words.clear();
-    dict.assign(in, in + n);
-    if (!n) return 0;  // empty column: no words, empty dictionary
-    std::sort(dict.begin(), dict.end());
-    dict.erase(std::unique(dict.begin(), dict.end()), dict.end());
-    const int bits = dict_bits(dict.size());
-    const uint64_t per = (uint64_t)std::floor(32 / (double)bits);
-    const uint64_t nw = (uint64_t)std::ceil(n / (double)per);
-    words.assign(nw, 0);
-    for (uint64_t i = 0; i < n; i++) {
-        const uint32_t idx = (uint32_t)(std::lower_bound(dict.begin(), dict.end(), in[i]) - dict.begin());
-        uint32_t& w = words[i / per];
-        w = bits >= 32 ? 0u : (w << bits);
-        w |= idx;
-    }
-    return nw;
-}
-
-template uint64_t dict_encode<int32_t>(const int32_t*, uint64_t, std::vector<uint32_t>&, std::vector<int32_t>&);
-template uint64_t dict_encode<uint32_t>(const uint32_t*, uint64_t, std::vector<uint32_t>&, std::vector<uint32_t>&);
-template uint64_t dict_encode<double>(const double*, uint64_t, std::vector<uint32_t>&, std::vector<double>&);
-
 int CWriter::open(const std::string& prefix, std::string& err)
 {
     prefix_ = prefix;
@@ -169,147 +87,18 @@ int CWriter::open(const std::string& prefix, std::string& err)
     return 0;
 }
 
-void CWriter::write_window(const std::string& name, uint32_t ws, const WindowCols& w)
+// The chunk of an empty window (a chromosome whose length is a multiple of
+// the window ends with one, cpp/call_genotype.cc:636-644): an all-N chunk of
+// size 0 in every file (cpp/compress.h:151-1312).
+void CWriter::write_empty_window(const std::string& name, uint32_t ws)
 {
-    const uint32_t C = w.n;
     names_.push_back(name);
     starts_.push_back((uint64_t)ws + 1);
-    sizes_.push_back(C);
-    bool all_n = true;
-    for (uint32_t j = 0; j < C && all_n; j++) all_n = w.nflag(j);
-    // columns 3, 4, 6 (cpp/compress.h:187-409)
-    if (all_n) {
-        const uint64_t param[7] = {1, 0, C, 0, 0, 0, 0};
-        put(f346_, param, 8, 7);
-    } else {
-        std::vector<char> c3(C), c4(C), c6(C);
-        for (uint32_t j = 0; j < C; j++) {
-            if (w.nrow(j)) {
-                c3[j] = c4[j] = c6[j] = 'N';
-            } else {
-                c3[j] = kACTGN[w.ori[j] & 7];
-                c4[j] = kAbbv[w.type1[j]];
-                c6[j] = kACTGN[w.base1[j]];
-            }
-        }
-        std::vector<int32_t> id4, id6, idn;
-        std::vector<char> v4, v6;
-        for (uint32_t j = 0; j < C; j++) {
-            if (c4[j] != c3[j]) id4.push_back((int32_t)j), v4.push_back(c4[j]);
-            if (c6[j] != c3[j]) id6.push_back((int32_t)j), v6.push_back(c6[j]);
-            if (c3[j] == 'N') idn.push_back((int32_t)j), c3[j] = 'A';
-        }
-        std::vector<uint32_t> code = encode2(c3);
-        const uint64_t size = 5 * id4.size() + 5 * id6.size() + 4 * idn.size() + 4 * code.size();
-        const uint64_t param[7] = {0, size, C, code.size(), idn.size(), id4.size(), id6.size()};
-        put(f346_, param, 8, 7);
-        put(f346_, id4.data(), 4, id4.size());
-        put(f346_, v4.data(), 1, v4.size());
-        put(f346_, id6.data(), 4, id6.size());
-        put(f346_, v6.data(), 1, v6.size());
-        put(f346_, idn.data(), 4, idn.size());
-        put(f346_, code.data(), 4, code.size());
-    }
-    const int32_t nind = all_n ? 1 : 0;
-    // columns 5, 7, 8, 9, 14, 16: run lengths + dictionaries (cpp/compress.h:501-610)
-    put(f5_, &nind, 4, 1);
-    if (!all_n) {
-        std::vector<int32_t> col(C);
-        for (int which = 0; which < 5; which++) {
-            for (uint32_t j = 0; j < C; j++) {
-                int32_t v = 0;
-                if (!w.nrow(j)) {
-                    const uint32_t b = w.base1[j];
-                    const int qsb = w.qs[4ull * j + b], cub = w.cu[4ull * j + b];
-                    switch (which) {
-                    case 0: v = w.q_cns[j]; break;
-                    case 1: v = qsb == 0 ? 0 : qsb / cub; break;
-                    case 2: v = cub; break;
-                    case 3: v = w.ca[4ull * j + b]; break;
-                    default: v = w.depth[j]; break;
-                    }
-                }
-                col[j] = v;
-            }
-            write_rle_column(f5_, col);
-        }
-        std::vector<double> cn(C);
-        for (uint32_t j = 0; j < C; j++)
-            cn[j] = w.nrow(j) ? 255.0 : (w.depth[j] == 0 ? 255 : (double)w.repeat[j] / w.depth[j]);
-        write_rle_column(f5_, cn);
-    }
-    // column 10 (cpp/compress.h:786-897)
-    put(f10_, &nind, 4, 1);
-    if (!all_n) {
-        std::vector<char> c10(C);
-        std::vector<uint32_t> nidx;
-        for (uint32_t j = 0; j < C; j++) {
-            c10[j] = (!w.nrow(j) && w.base2[j] < 4) ? kACTGN[w.base2[j]] : 'N';
-            if (c10[j] == 'N') nidx.push_back(j), c10[j] = 'A';
-        }
-        std::vector<uint32_t> code = encode2(c10);
-        const uint64_t param[3] = {nidx.size(), code.size(), C};
-        put(f10_, param, 8, 3);
-        put(f10_, code.data(), 4, code.size());
-        put(f10_, nidx.data(), 4, nidx.size());
-    }
-    // columns 11, 12, 13 (cpp/compress.h:951-1175); col 13 is only kept
-    // through the common list or when col 11 and col 12 are both zero
-    put(f11_, &nind, 4, 1);
-    if (!all_n) {
-        std::vector<int32_t> cid, c11, c12, c13, x12i, x12v, x13i, x13v;
-        for (uint32_t j = 0; j < C; j++) {
-            int32_t a = 0, b = 0, d = 0;
-            if (!w.nrow(j) && w.base2[j] < 4) {
-                const uint32_t b2 = w.base2[j];
-                const int qsb = w.qs[4ull * j + b2];
-                b = w.cu[4ull * j + b2];
-                a = qsb == 0 ? 0 : qsb / b;
-                d = w.ca[4ull * j + b2];
-            }
-            if (a != 0) {
-                cid.push_back((int32_t)j), c11.push_back(a), c12.push_back(b), c13.push_back(d);
-            } else if (a != b) {
-                x12i.push_back((int32_t)j), x12v.push_back(b);
-            } else if (a != d) {
-                x13i.push_back((int32_t)j), x13v.push_back(d);
-            }
-        }
-        const uint64_t param[4] = {C, cid.size(), x12i.size(), x13i.size()};
-        put(f11_, param, 8, 4);
-        put(f11_, x12i.data(), 4, x12i.size());
-        put(f11_, x12v.data(), 4, x12v.size());
-        put(f11_, x13i.data(), 4, x13i.size());
-        put(f11_, x13v.data(), 4, x13v.size());
-        put(f11_, cid.data(), 4, cid.size());
-        for (const std::vector<int32_t>* col : {&c11, &c12, &c13}) {
-            std::vector<uint32_t> words;
-            std::vector<int32_t> dict;
-            const uint64_t nw = dict_encode(col->data(), col->size(), words, dict);
-            const uint64_t dp[2] = {nw, dict.size()};
-            put(f11_, dp, 8, 2);
-            put(f11_, words.data(), 4, nw);
-            put(f11_, dict.data(), 4, dict.size());
-        }
-    }
-    // columns 15 and 17 (cpp/compress.h:1178-1312)
-    put(f15_, &nind, 4, 1);
-    if (!all_n) {
-        std::vector<int32_t> id;
-        std::vector<double> val;
-        for (uint32_t j = 0; j < C; j++)
-            if (!w.nrow(j) && w.rank[j] != 1.0) id.push_back((int32_t)j), val.push_back(w.rank[j]);
-        int32_t n = (int32_t)id.size();
-        put(f15_, &n, 4, 1);
-        put(f15_, id.data(), 4, id.size());
-        put(f15_, val.data(), 8, val.size());
-        id.clear();
-        for (uint32_t j = 0; j < C; j++)
-            if (!w.nrow(j) && (w.ori[j] & 8)) id.push_back((int32_t)j);
-        n = (int32_t)id.size();
-        put(f15_, &n, 4, 1);
-        put(f15_, id.data(), 4, id.size());
-    }
+    sizes_.push_back(0);
+    const uint64_t param[7] = {1, 0, 0, 0, 0, 0, 0};
+    put(f346_, param, 8, 7);
+    const int32_t nind = 1;
+    for (FILE* f : {f5_, f10_, f11_, f15_}) put(f, &nind, 4, 1);
 }
 
 void CWriter::write_chunks(const std::string& name, const ColumnPass& ch)

@@ -190,7 +190,7 @@ int run_pipeline(const char* fasta, const char* soap, const char* dbsnp, const c
         auto on_results = [&](uint32_t offset, uint32_t ws, uint32_t n) -> int {
             if (!compress) return GSNP_OK;
             if (n == 0) {  // the empty window after a chromosome that fills its last window
-                cw.write_window(chr.name, ws, WindowCols());
+                cw.write_empty_window(chr.name, ws);
                 return GSNP_OK;
             }
             // one -C chunk per reference window of the span, encoded on the device

@@ -37,6 +37,7 @@ DATASETS = {
     "stress": (6, 13, 1.0, {"refine_mode": 1, "rank_sum_mode": 1, "transition_dominant": 1}),
     "c3s": (3, 14, 0.004, {"rank_sum_mode": 1}),        # 24 small contigs
     "c5s": (5, 15, 0.0012, {"rank_sum_mode": 1}),       # 150 bp reads
+    "c2x2": (2, 16, 0.0524288001, {"refine_mode": 1}),  # exactly 2 windows: an empty third window
 }
 
 

@@ -149,7 +149,8 @@ def test_parameter_variants(datasets, tmp_path, name, flags, batched):
     assert out.read_bytes() == ref.read_bytes(), _first_diff(out, ref)
 
 
-@pytest.mark.parametrize("name,per", [("stress", 0), ("c2w3", 0), ("stress", 4), ("c2w3", 4)])
+@pytest.mark.parametrize("name,per", [("stress", 0), ("c2w3", 0), ("stress", 4), ("c2w3", 4), ("c2x2", 0),
+                                      ("c2x2", 4)])
 def test_compressed_output(datasets, tmp_path, name, per):
     """-C files (encoded on the device, one chunk per reference window; with
     per > 0 the windows of a batched span are encoded together) are the
