@@ -22,6 +22,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <string>
 #include <vector>
 
@@ -41,6 +42,7 @@ constexpr int kNC = 16;                      // counters per block
 constexpr int kNDict = 15;                   // dictionary-coded arrays per window
 constexpr uint32_t kTabStride = 2 * kWin;    // hash slots per (window, array)
 constexpr uint32_t kDictThreads = 1024;
+constexpr uint32_t kLdsSlots = 4096;         // LDS hash set of a dictionary: up to kLdsSlots/2 distinct values
 constexpr uint32_t kSubWindows = 16;         // windows per encoder pass
 constexpr uint64_t kEmpty = ~0ull;           // never a key: int keys use 32 bits, f64 keys are not NaN
 
@@ -339,6 +341,7 @@ struct DictArgs {
     uint64_t* dict64;
     uint32_t* nd;
     uint32_t* nw;
+    uint32_t lds_max;      // distinct values kept in the LDS set before falling back to HBM (0: HBM only)
     uint8_t pow2bits[32];  // index bits of a dictionary of 2^e entries, as the host's libm gives them
 };
 
@@ -356,11 +359,16 @@ __device__ __forceinline__ uint64_t tab_load(const uint64_t* p)
 // hash set (load at most 1/2, so every probe sequence meets a free slot), each
 // distinct value's rank = the count of distinct values below it, then the
 // ranks packed floor(32/bits) per word, MSB first, the last word right-aligned
-// (dict_encode, cpp/dict_compression.h:116-166).
+// (dict_encode, cpp/dict_compression.h:116-166).  The set lives in LDS while
+// it holds at most lds_max values (the usual case: a window's arrays have a
+// few hundred distinct values among up to 262,144 elements); past that the
+// workgroup starts again with a set in HBM sized for every element.
 __global__ __launch_bounds__(kDictThreads) void k_col_dict(DictArgs a)
 {
-    __shared__ uint32_t s_nd;
-    __shared__ uint64_t tile[2048];
+    __shared__ uint32_t s_nd, s_over;
+    __shared__ unsigned long long ltab[kLdsSlots];
+    __shared__ uint32_t lrank[kLdsSlots];
+    __shared__ uint64_t tile[1024];
     const uint32_t w = blockIdx.x, k = blockIdx.y, t = threadIdx.x;
     const uint32_t* tw = a.tot + (size_t)w * kNC;
     const bool all_n = tw[kCNotN] == 0;
@@ -382,10 +390,46 @@ __global__ __launch_bounds__(kDictThreads) void k_col_dict(DictArgs a)
     uint64_t* dk = a.dkeys + wk * kWin;
     uint32_t* ds = a.dslot + wk * kWin;
     uint32_t* es = a.eslot + wk * kWin;
-    if (t == 0) s_nd = 0;
-    for (uint32_t h = t; h < cap; h += kDictThreads) tab[h] = kEmpty;
-    __syncthreads();
-    for (uint32_t e = t; e < m; e += kDictThreads) {
+    bool lds = a.lds_max > 0;
+    if (lds) {
+        const uint32_t limit = min(a.lds_max, kLdsSlots / 2);
+        if (t == 0) s_nd = s_over = 0;
+        for (uint32_t h = t; h < kLdsSlots; h += kDictThreads) ltab[h] = kEmpty;
+        __syncthreads();
+        const volatile uint32_t* over = &s_over;
+        for (uint32_t e = t; e < m && !*over; e += kDictThreads) {
+            const uint64_t key = key_at(e);
+            uint32_t h = slot_of(key, 12);
+            for (uint32_t probe = 0; probe < kLdsSlots; probe++) {
+                unsigned long long cur = ltab[h];
+                if (cur == kEmpty) {
+                    cur = atomicCAS(&ltab[h], (unsigned long long)kEmpty, (unsigned long long)key);
+                    if (cur == kEmpty) {
+                        const uint32_t p = atomicAdd(&s_nd, 1u);
+                        if (p < limit) {
+                            dk[p] = key;
+                            ds[p] = h;
+                        } else {
+                            s_over = 1;
+                        }
+                        break;
+                    }
+                }
+                if (cur == key) break;
+                h = (h + 1) & (kLdsSlots - 1);
+            }
+            es[e] = h;
+        }
+        __syncthreads();
+        lds = s_over == 0;
+    }
+    if (!lds) {
+        __syncthreads();  // every thread has read s_over
+        if (t == 0) s_nd = 0;
+        for (uint32_t h = t; h < cap; h += kDictThreads) tab[h] = kEmpty;
+        __syncthreads();
+    }
+    for (uint32_t e = t; e < m && !lds; e += kDictThreads) {
         const uint64_t key = key_at(e);
         uint32_t h = slot_of(key, lg);
         for (uint32_t probe = 0; probe < cap; probe++) {
@@ -411,8 +455,8 @@ __global__ __launch_bounds__(kDictThreads) void k_col_dict(DictArgs a)
         const uint32_t d = base + t;
         const uint64_t x = d < nd ? dk[d] : 0ull;
         uint32_t r = 0;
-        for (uint32_t t0 = 0; t0 < nd; t0 += 2048) {
-            const uint32_t len = min(2048u, nd - t0);
+        for (uint32_t t0 = 0; t0 < nd; t0 += 1024) {
+            const uint32_t len = min(1024u, nd - t0);
             for (uint32_t u = t; u < len; u += kDictThreads) tile[u] = dk[t0 + u];
             __syncthreads();
             if (d < nd) {
@@ -427,7 +471,10 @@ __global__ __launch_bounds__(kDictThreads) void k_col_dict(DictArgs a)
             __syncthreads();
         }
         if (d < nd) {
-            trank[ds[d]] = r;
+            if (lds)
+                lrank[ds[d]] = r;
+            else
+                trank[ds[d]] = r;
             if (dbl)
                 a.dict64[(size_t)w * kWin + r] = x;
             else
@@ -444,7 +491,7 @@ __global__ __launch_bounds__(kDictThreads) void k_col_dict(DictArgs a)
         uint32_t acc = 0;
         for (uint32_t e = e0; e < e1; e++) {
             acc = bits >= 32 ? 0u : (acc << bits);
-            acc |= trank[es[e]];
+            acc |= lds ? lrank[es[e]] : trank[es[e]];
         }
         wo[q] = acc;
     }
@@ -683,6 +730,10 @@ int encode_columns(Ctx& c, uint32_t offset, uint32_t ws_abs, uint32_t n, const C
     std::vector<uint32_t> sizes;
     DictArgs da{};
     for (int e = 0; e < 32; e++) da.pow2bits[e] = (uint8_t)dict_bits(1ull << e);
+    // GSNP_DICT_LDS_MAX: distinct values a dictionary keeps in LDS (at most
+    // 2048, the default; 0 builds every dictionary in HBM)
+    da.lds_max = kLdsSlots / 2;
+    if (const char* v = std::getenv("GSNP_DICT_LDS_MAX")) da.lds_max = (uint32_t)std::strtoul(v, nullptr, 10);
     const size_t S = (size_t)b.win * kWin;
     for (uint32_t s0 = 0; s0 < n; s0 += kSubWindows * kWin) {
         const uint32_t ns = std::min<uint32_t>(n - s0, kSubWindows * kWin);

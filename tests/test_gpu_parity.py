@@ -149,16 +149,21 @@ def test_parameter_variants(datasets, tmp_path, name, flags, batched):
     assert out.read_bytes() == ref.read_bytes(), _first_diff(out, ref)
 
 
-@pytest.mark.parametrize("name,per", [("stress", 0), ("c2w3", 0), ("stress", 4), ("c2w3", 4), ("c2x2", 0),
-                                      ("c2x2", 4)])
-def test_compressed_output(datasets, tmp_path, name, per):
+@pytest.mark.parametrize("name,per,lds", [("stress", 0, None), ("c2w3", 0, None), ("stress", 4, None),
+                                          ("c2w3", 4, None), ("c2x2", 0, None), ("c2x2", 4, None),
+                                          ("c2w3", 4, "0"), ("stress", 0, "0"), ("c2w3", 4, "8")])
+def test_compressed_output(datasets, tmp_path, monkeypatch, name, per, lds):
     """-C files (encoded on the device, one chunk per reference window; with
     per > 0 the windows of a batched span are encoded together) are the
-    oracle's bytes, and decompress to the oracle's text."""
+    oracle's bytes, and decompress to the oracle's text.  lds "0" builds every
+    dictionary's hash set in HBM, "8" makes most of them overflow the LDS set
+    and start again in HBM."""
     import gsnp_amd
 
+    if lds is not None:
+        monkeypatch.setenv("GSNP_DICT_LDS_MAX", lds)
     ds = datasets(name)
-    print(f"[parity] -C {name} batched={per}", flush=True)
+    print(f"[parity] -C {name} batched={per} lds={lds}", flush=True)
     gpu = tmp_path / f"{name}.gpu"
     _gpu_run(ds, gpu, compress=True, batched=per)
     cpu = tmp_path / f"{name}.cpu"
