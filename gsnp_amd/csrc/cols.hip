@@ -397,28 +397,40 @@ __global__ __launch_bounds__(kDictThreads) void k_col_dict(DictArgs a)
         for (uint32_t h = t; h < kLdsSlots; h += kDictThreads) ltab[h] = kEmpty;
         __syncthreads();
         const volatile uint32_t* over = &s_over;
-        for (uint32_t e = t; e < m && !*over; e += kDictThreads) {
-            const uint64_t key = key_at(e);
-            uint32_t h = slot_of(key, 12);
-            for (uint32_t probe = 0; probe < kLdsSlots; probe++) {
-                unsigned long long cur = ltab[h];
-                if (cur == kEmpty) {
-                    cur = atomicCAS(&ltab[h], (unsigned long long)kEmpty, (unsigned long long)key);
-                    if (cur == kEmpty) {
-                        const uint32_t p = atomicAdd(&s_nd, 1u);
-                        if (p < limit) {
-                            dk[p] = key;
-                            ds[p] = h;
-                        } else {
-                            s_over = 1;
-                        }
-                        break;
-                    }
-                }
-                if (cur == key) break;
-                h = (h + 1) & (kLdsSlots - 1);
+        // four elements per step, their keys loaded before any is inserted
+        for (uint32_t e0 = t; e0 < m && !*over; e0 += 4 * kDictThreads) {
+            uint64_t keys[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const uint32_t e = e0 + u * kDictThreads;
+                keys[u] = e < m ? key_at(e) : 0ull;
             }
-            es[e] = h;
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const uint32_t e = e0 + u * kDictThreads;
+                if (e >= m) break;
+                const uint64_t key = keys[u];
+                uint32_t h = slot_of(key, 12);
+                for (uint32_t probe = 0; probe < kLdsSlots; probe++) {
+                    unsigned long long cur = ltab[h];
+                    if (cur == kEmpty) {
+                        cur = atomicCAS(&ltab[h], (unsigned long long)kEmpty, (unsigned long long)key);
+                        if (cur == kEmpty) {
+                            const uint32_t p = atomicAdd(&s_nd, 1u);
+                            if (p < limit) {
+                                dk[p] = key;
+                                ds[p] = h;
+                            } else {
+                                s_over = 1;
+                            }
+                            break;
+                        }
+                    }
+                    if (cur == key) break;
+                    h = (h + 1) & (kLdsSlots - 1);
+                }
+                es[e] = h;
+            }
         }
         __syncthreads();
         lds = s_over == 0;
@@ -489,6 +501,7 @@ __global__ __launch_bounds__(kDictThreads) void k_col_dict(DictArgs a)
     for (uint32_t q = t; q < nwords; q += kDictThreads) {
         const uint32_t e0 = q * per, e1 = min(e0 + per, m);
         uint32_t acc = 0;
+#pragma unroll 8
         for (uint32_t e = e0; e < e1; e++) {
             acc = bits >= 32 ? 0u : (acc << bits);
             acc |= lds ? lrank[es[e]] : trank[es[e]];
