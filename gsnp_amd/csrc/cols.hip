@@ -739,8 +739,28 @@ int encode_columns(Ctx& c, uint32_t offset, uint32_t ws_abs, uint32_t n, const C
         int rc = alloc_cols(b, want);
         if (rc) return rc;
     }
-    std::vector<uint64_t> starts;
-    std::vector<uint32_t> sizes;
+    if (!b.copy) {
+        HIPCHK(hipStreamCreateWithFlags(&b.copy, hipStreamNonBlocking));
+        HIPCHK(hipEventCreateWithFlags(&b.ev_written, hipEventDisableTiming));
+        for (hipEvent_t& e : b.ev_copied) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    }
+    // a pass whose copy to host memory is in flight; handed to `sink` once the
+    // next pass has been queued (its buffers are reused two passes later)
+    struct Done {
+        ColumnPass pass;
+        int buf = 0;
+        std::vector<uint64_t> starts;
+        std::vector<uint32_t> sizes;
+    };
+    Done prev;
+    bool have_prev = false;
+    auto hand_over = [&](Done& d) -> int {
+        HIPCHK(hipEventSynchronize(b.ev_copied[d.buf]));
+        d.pass.starts = d.starts.data();
+        d.pass.sizes = d.sizes.data();
+        d.pass.n_windows = (uint32_t)d.sizes.size();
+        return sink(d.pass);
+    };
     DictArgs da{};
     for (int e = 0; e < 32; e++) da.pow2bits[e] = (uint8_t)dict_bits(1ull << e);
     // GSNP_DICT_LDS_MAX: distinct values a dictionary keeps in LDS (at most
@@ -748,6 +768,7 @@ int encode_columns(Ctx& c, uint32_t offset, uint32_t ws_abs, uint32_t n, const C
     da.lds_max = kLdsSlots / 2;
     if (const char* v = std::getenv("GSNP_DICT_LDS_MAX")) da.lds_max = (uint32_t)std::strtoul(v, nullptr, 10);
     const size_t S = (size_t)b.win * kWin;
+    int cur = 0;
     for (uint32_t s0 = 0; s0 < n; s0 += kSubWindows * kWin) {
         const uint32_t ns = std::min<uint32_t>(n - s0, kSubWindows * kWin);
         const uint32_t nwin = (ns + kWin - 1) / kWin;
@@ -781,55 +802,63 @@ int encode_columns(Ctx& c, uint32_t offset, uint32_t ws_abs, uint32_t n, const C
         wa.cid = b.cid.p, wa.id15 = b.id15.p, wa.id17 = b.id17.p, wa.v4 = b.v4.p, wa.v6 = b.v6.p;
         wa.code3 = b.code3.p, wa.code10 = b.code10.p, wa.words = b.words.p, wa.dict32 = b.dict32.p;
         wa.dict64 = b.dict64.p, wa.val15 = b.val15.p, wa.foff = b.foff.p, wa.ftot = b.ftot.p;
-        wa.out = b.stream.p, wa.cap = b.stream_cap;
+        wa.out = nullptr, wa.cap = 0;
         hipLaunchKernelGGL(k_col_sizes, dim3(1), dim3(128), 0, st, wa, nwin);
         c.timer.end(kStColumns, st);
         HIPCHK(hipGetLastError());
         uint64_t tot[5];
         HIPCHK(hipMemcpyAsync(tot, b.ftot.p, sizeof tot, hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
+        // buffer `cur` was last copied from two passes ago, and that pass has
+        // been handed over, so it is free
         const uint64_t need = (*std::max_element(tot, tot + 5) + 255) & ~255ull;  // files stay 256-byte aligned
-        if (need > b.stream_cap) {
-            HIPCHK(b.stream.alloc(5 * need));
-            b.stream_cap = need;
-            wa.out = b.stream.p, wa.cap = need;
+        if (need > b.stream_cap[cur]) {
+            HIPCHK(b.stream[cur].alloc(5 * need));
+            b.stream_cap[cur] = need;
         }
+        wa.out = b.stream[cur].p, wa.cap = b.stream_cap[cur];
         c.timer.begin(kStColumns, st);
         hipLaunchKernelGGL(k_col_write, dim3(nwin, 5, kWriteParts), dim3(256), 0, st, wa);
         c.timer.end(kStColumns, st);
         HIPCHK(hipGetLastError());
+        HIPCHK(hipEventRecord(b.ev_written, st));
+        HIPCHK(hipStreamWaitEvent(b.copy, b.ev_written, 0));
         const uint64_t all = tot[0] + tot[1] + tot[2] + tot[3] + tot[4];
-        if (all > b.host_cap) {
-            if (b.host) HIPCHK(hipHostFree(b.host));
-            b.host = nullptr;
-            b.host_cap = 0;
-            HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&b.host), all, hipHostMallocDefault));
-            b.host_cap = all;
+        if (all > b.host_cap[cur]) {
+            if (b.host[cur]) HIPCHK(hipHostFree(b.host[cur]));
+            b.host[cur] = nullptr;
+            b.host_cap[cur] = 0;
+            HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&b.host[cur]), all, hipHostMallocDefault));
+            b.host_cap[cur] = all;
         }
-        ColumnPass pass;
+        Done d;
+        d.buf = cur;
         uint64_t at = 0;
         for (int f = 0; f < 5; f++) {
             if (tot[f])
-                HIPCHK(hipMemcpyAsync(b.host + at, b.stream.p + (size_t)f * b.stream_cap, tot[f],
-                                      hipMemcpyDeviceToHost, st));
-            pass.file[f] = b.host + at;
-            pass.bytes[f] = tot[f];
+                HIPCHK(hipMemcpyAsync(b.host[cur] + at, b.stream[cur].p + (size_t)f * b.stream_cap[cur], tot[f],
+                                      hipMemcpyDeviceToHost, b.copy));
+            d.pass.file[f] = b.host[cur] + at;
+            d.pass.bytes[f] = tot[f];
             at += tot[f];
         }
-        HIPCHK(hipStreamSynchronize(st));
-        starts.clear();
-        sizes.clear();
+        HIPCHK(hipEventRecord(b.ev_copied[cur], b.copy));
         for (uint32_t w = 0; w < nwin; w++) {
-            starts.push_back((uint64_t)ws_abs + s0 + (uint64_t)w * kWin + 1);
-            sizes.push_back(std::min<uint32_t>(kWin, ns - w * kWin));
+            d.starts.push_back((uint64_t)ws_abs + s0 + (uint64_t)w * kWin + 1);
+            d.sizes.push_back(std::min<uint32_t>(kWin, ns - w * kWin));
         }
-        pass.starts = starts.data();
-        pass.sizes = sizes.data();
-        pass.n_windows = nwin;
-        int rc = sink(pass);
-        if (rc) return rc;
+        if (have_prev) {
+            int rc = hand_over(prev);
+            if (rc) {
+                (void)hipStreamSynchronize(b.copy);
+                return rc;
+            }
+        }
+        prev = std::move(d);
+        have_prev = true;
+        cur ^= 1;
     }
-    return GSNP_OK;
+    return have_prev ? hand_over(prev) : GSNP_OK;
 }
 
 }  // namespace gsnp

@@ -116,16 +116,25 @@ struct ColBufs {
     DevBuf<uint64_t> tab, dkeys;
     DevBuf<uint32_t> trank, dslot, eslot, words, dict32, nd, nw;
     DevBuf<uint64_t> dict64, foff, ftot;
-    DevBuf<uint8_t> stream;
-    uint64_t stream_cap = 0;
-    uint8_t* host = nullptr;  // pinned copy of the five file streams of a pass
-    uint64_t host_cap = 0;
+    // the five file streams of a pass, double-buffered: the copy of one pass
+    // to pinned host memory runs on `copy` while the next pass is encoded
+    DevBuf<uint8_t> stream[2];
+    uint64_t stream_cap[2] = {0, 0};
+    uint8_t* host[2] = {nullptr, nullptr};
+    uint64_t host_cap[2] = {0, 0};
+    hipStream_t copy = nullptr;
+    hipEvent_t ev_written = nullptr, ev_copied[2] = {nullptr, nullptr};
     ColBufs() = default;
     ColBufs(const ColBufs&) = delete;
     ColBufs& operator=(const ColBufs&) = delete;
     ~ColBufs()
     {
-        if (host) (void)hipHostFree(host);
+        if (copy) (void)hipStreamSynchronize(copy);
+        for (uint8_t* h : host)
+            if (h) (void)hipHostFree(h);
+        for (hipEvent_t e : {ev_written, ev_copied[0], ev_copied[1]})
+            if (e) (void)hipEventDestroy(e);
+        if (copy) (void)hipStreamDestroy(copy);
     }
 };
 
