@@ -77,8 +77,9 @@ struct ColSite {
 
 __device__ __forceinline__ int pick(int4 v, uint32_t b) { return b == 0 ? v.x : b == 1 ? v.y : b == 2 ? v.z : v.w; }
 
-// The column values of site j (Call_win::call_cns2's -C branch,
-// cpp/compress.h:187-1312, restated by the host writer in cwriter.cpp).
+// The column values of site j: extract_3_4_6 (cpp/compress.h:87-185),
+// extract_5/7/8/9/14/16 (:413-500), extract_10 (:899-913) and
+// extract_11_12_13 (:915-944), with the N row of call_cns2's -C branch.
 __device__ ColSite col_site(const ColIn& in, uint32_t j)
 {
     ColSite s;
@@ -529,8 +530,9 @@ struct WriteArgs {
     uint64_t cap;
 };
 
-// The pieces of window w's chunk in file f, in file order
-// (cpp/compress.h:151-1312; the host writer in cwriter.cpp writes the same).
+// The pieces of window w's chunk in file f, in file order (Column_3_4_6
+// cpp/compress.h:187-409, Column_5_7_8_9_14_16 :501-783, Column_10 :786-897,
+// Column_11_12_13 :951-1175, Column_15_17 :1178-1312).
 template <class Sink>
 __device__ void chunk_pieces(const WriteArgs& a, uint32_t w, int f, Sink& s)
 {
