@@ -109,16 +109,21 @@ def main() -> int:
         print(f"[bench] autotuned fused launch in {time.time() - t0:.1f}s: {tuned}", file=sys.stderr, flush=True)
 
     def timed(mode: str):
+        """Seconds per pass with the stage timers off (the batched span and its
+        formatting replay as captured graphs), then the same number of passes
+        with the stage timers on (eager launches) for the per-stage times."""
         batched = per if mode == "batched" else 0
         for _ in range(args.warmup):
             job.call(batched=batched)
-        job.records(reset=True)
-        job.profile(enable=True)
         t_start = time.perf_counter()
         nbytes = 0
         for _ in range(args.steps):
             nbytes = job.call(batched=batched)
         t_end = time.perf_counter()
+        job.records(reset=True)
+        job.profile(enable=True)
+        for _ in range(args.steps):
+            job.call(batched=batched)
         prof = job.profile(enable=False)
         recs = job.records(reset=True) // max(1, args.steps)
         secs = (t_end - t_start) / args.steps
