@@ -127,6 +127,8 @@ SIGNATURES = {
     "gsnp_job_select": (ctypes.c_int, [_P, ctypes.c_int]),
     "gsnp_job_call": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_uint64)]),
     "gsnp_job_columns": (ctypes.c_int, [_P, ctypes.c_int, ctypes.POINTER(ctypes.c_uint64)]),
+    "gsnp_set_matrix": (ctypes.c_int, [_P, _P]),
+    "gsnp_calib_count": (ctypes.c_int, [_P, _P]),
     "gsnp_job_context": (_P, [_P]),
     "gsnp_job_close": (None, [_P]),
     "gsnp_synth_write": (ctypes.c_longlong, [ctypes.c_int, ctypes.c_ulonglong, ctypes.c_double, ctypes.c_char_p,

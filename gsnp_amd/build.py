@@ -21,7 +21,7 @@ OUT = PKG / "_build"
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("GSNP_OFFLOAD_ARCH", "gfx950")
 
-LIB_SOURCES = ["tables.cpp", "inputs.cpp", "cwriter.cpp", "synth.cpp", "kernels.hip", "dict.hip", "cols.hip", "engine.hip", "run.hip",
+LIB_SOURCES = ["tables.cpp", "inputs.cpp", "cwriter.cpp", "synth.cpp", "kernels.hip", "dict.hip", "cols.hip", "calib.hip", "engine.hip", "run.hip",
                "adjoint.hip"]
 TOOLS = {"gsnp": "cli_main.cpp", "gsnp-decompress": "decompress_main.cpp"}
 

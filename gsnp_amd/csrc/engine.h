@@ -14,6 +14,8 @@ namespace gsnp {
 
 int fail(int code, const std::string& msg);
 struct Ctx;
+// calibration bins: (q - q_min) << 12 | cycle << 4 | ref << 2 | obs
+constexpr uint32_t kCalibBins = 64u << 12;
 int dict_encode_device(Ctx& c, const int32_t* d_in, uint64_t n, uint64_t* n_words, uint64_t* n_dict);
 
 template <class T>
@@ -214,7 +216,9 @@ struct Ctx {
     DevBuf<uint32_t> dict_bits, dict_wpre, dict_out;
     DevBuf<uint64_t> dict_bsum;
     ColBufs cols;  // -C column encoder
+    DevBuf<unsigned long long> calib;  // calibration sub-histograms
 
+    int set_matrix(const double* p_matrix);
     int alloc_window(uint32_t n_sites);
     SiteArrays site_view(uint32_t offset);
     Results result_view(uint32_t offset);
@@ -232,6 +236,8 @@ struct Ctx {
     int launch_captured(GraphSlot& slot, const std::vector<uint64_t>& key, F enqueue);
     FusedArgs fused_args(uint32_t start);
 };
+
+int create_ctx(const gsnp_params* p, const double* p_matrix, gsnp_ctx** out);
 
 }  // namespace gsnp
 

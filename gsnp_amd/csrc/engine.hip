@@ -280,6 +280,60 @@ int Ctx::check_err()
     return GSNP_OK;
 }
 
+// The score table, from a calibration matrix (256*256*16 doubles); every other
+// table is built at creation.
+int Ctx::set_matrix(const double* p_matrix)
+{
+    std::vector<double> sc = build_score_table(p_matrix);
+    HIPCHK(hipStreamSynchronize(stream));
+    if (score.n != sc.size()) HIPCHK(score.alloc(sc.size()));
+    HIPCHK(hipMemcpyAsync(score.p, sc.data(), sc.size() * 8, hipMemcpyHostToDevice, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+    return GSNP_OK;
+}
+
+// A context; with p_matrix null its score table is left for set_matrix (a job
+// that counts its calibration on the device creates the context first).
+int create_ctx(const gsnp_params* p, const double* p_matrix, gsnp_ctx** out)
+{
+    if (!p || !out) return fail(GSNP_E_ARG, "null argument");
+    if (p->read_length < 1 || p->read_length > 256) return fail(GSNP_E_ARG, "read length outside [1,256]");
+    if (p->q_min < 1 || p->q_max < p->q_min || p->q_max > 126 || p->q_max - p->q_min > 62)
+        return fail(GSNP_E_ARG, "quality characters out of range");
+    std::unique_ptr<gsnp_ctx> c(new gsnp_ctx());
+    c->p = *p;
+    HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    // adjusted qualities, priors, rank table (the score table: set_matrix)
+    QadjTable qt = build_qadj_table(p->pcr_dependency, p->global_dependency);
+    c->td = qt.td;
+    c->gd = qt.gd;
+    c->h_pprior.resize(128);
+    prior_gen(*p, c->h_pprior.data());
+    std::vector<double> lp = log_prior_rows(c->h_pprior.data());
+    c->h_prank = rank_table();
+    HIPCHK(c->qadj.alloc(qt.v.size()));
+    HIPCHK(c->lprior.alloc(lp.size()));
+    HIPCHK(c->prank.alloc(c->h_prank.size()));
+    HIPCHK(c->err.alloc(1));
+    HIPCHK(c->counts.alloc(8));
+    HIPCHK(c->job_count.alloc(1));
+    HIPCHK(c->fix_count.alloc(1));
+    HIPCHK(c->total.alloc(1));
+    HIPCHK(c->stat.alloc(4));
+    HIPCHK(hipMemsetAsync(c->stat.p, 0, 32, c->stream));
+    HIPCHK(hipMemcpyAsync(c->qadj.p, qt.v.data(), qt.v.size(), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->lprior.p, lp.data(), lp.size() * 8, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->prank.p, c->h_prank.data(), c->h_prank.size() * 8, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemsetAsync(c->err.p, 0, 4, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (p_matrix) {
+        int rc = c->set_matrix(p_matrix);
+        if (rc) return rc;
+    }
+    *out = c.release();
+    return GSNP_OK;
+}
+
 }  // namespace gsnp
 
 using namespace gsnp;
@@ -313,40 +367,13 @@ int gsnp_device_init(int device)
 int gsnp_create(const gsnp_params* p, const double* p_matrix, gsnp_ctx** out)
 {
     if (!p || !p_matrix || !out) return fail(GSNP_E_ARG, "null argument");
-    if (p->read_length < 1 || p->read_length > 256) return fail(GSNP_E_ARG, "read length outside [1,256]");
-    if (p->q_min < 1 || p->q_max < p->q_min || p->q_max > 126 || p->q_max - p->q_min > 62)
-        return fail(GSNP_E_ARG, "quality characters out of range");
-    std::unique_ptr<gsnp_ctx> c(new gsnp_ctx());
-    c->p = *p;
-    HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
-    // score table, adjusted qualities, priors, rank table
-    std::vector<double> sc = build_score_table(p_matrix);
-    QadjTable qt = build_qadj_table(p->pcr_dependency, p->global_dependency);
-    c->td = qt.td;
-    c->gd = qt.gd;
-    c->h_pprior.resize(128);
-    prior_gen(*p, c->h_pprior.data());
-    std::vector<double> lp = log_prior_rows(c->h_pprior.data());
-    c->h_prank = rank_table();
-    HIPCHK(c->score.alloc(sc.size()));
-    HIPCHK(c->qadj.alloc(qt.v.size()));
-    HIPCHK(c->lprior.alloc(lp.size()));
-    HIPCHK(c->prank.alloc(c->h_prank.size()));
-    HIPCHK(c->err.alloc(1));
-    HIPCHK(c->counts.alloc(8));
-    HIPCHK(c->job_count.alloc(1));
-    HIPCHK(c->fix_count.alloc(1));
-    HIPCHK(c->total.alloc(1));
-    HIPCHK(c->stat.alloc(4));
-    HIPCHK(hipMemsetAsync(c->stat.p, 0, 32, c->stream));
-    HIPCHK(hipMemcpyAsync(c->score.p, sc.data(), sc.size() * 8, hipMemcpyHostToDevice, c->stream));
-    HIPCHK(hipMemcpyAsync(c->qadj.p, qt.v.data(), qt.v.size(), hipMemcpyHostToDevice, c->stream));
-    HIPCHK(hipMemcpyAsync(c->lprior.p, lp.data(), lp.size() * 8, hipMemcpyHostToDevice, c->stream));
-    HIPCHK(hipMemcpyAsync(c->prank.p, c->h_prank.data(), c->h_prank.size() * 8, hipMemcpyHostToDevice, c->stream));
-    HIPCHK(hipMemsetAsync(c->err.p, 0, 4, c->stream));
-    HIPCHK(hipStreamSynchronize(c->stream));
-    *out = c.release();
-    return GSNP_OK;
+    return create_ctx(p, p_matrix, out);
+}
+
+int gsnp_set_matrix(gsnp_ctx* c, const double* p_matrix)
+{
+    if (!c || !p_matrix) return fail(GSNP_E_ARG, "null argument");
+    return c->set_matrix(p_matrix);
 }
 
 void gsnp_destroy(gsnp_ctx* c)

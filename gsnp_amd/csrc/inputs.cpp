@@ -325,32 +325,14 @@ int load_alignments(const char* soap, const gsnp_params& p, const Genome& g, std
     return GSNP_OK;
 }
 
-void matrix_gen(const std::vector<ReadBlock>& blocks, const Genome& g, const gsnp_params& p, double* pm)
+// The matrix from calibration counts (cpp/matrix.cc:151-205), cnt indexed
+// (qc - q_min) << 12 | cycle << 4 | ref << 2 | obs for quality characters qc.
+void matrix_from_counts(const uint64_t* cnt, const gsnp_params& p, double* pm)
 {
-    std::vector<uint64_t> cnt((size_t)256 * 256 * 16, 0);
-    for (const ReadBlock& b : blocks) {
-        const Chrom& c = g.chr[b.chr];
-        for (uint64_t i = 0; i < b.size(); i++) {
-            const uint32_t lh = (uint32_t)b.len_hit[i];
-            if ((lh & 0xffff) != 1) continue;
-            const int len = (int)(lh >> 16);
-            const char* sq = &b.seq[i * b.pitch];
-            const char* qu = &b.qual[i * b.pitch];
-            const bool fwd = b.strand[i] == '+';
-            for (int k = 0; k < len; k++) {
-                if (!acgt(sq[k])) continue;
-                const uint8_t ref = c.code[b.pos[i] + k];
-                if (ref & 12) continue;  // N or known SNP on the reference
-                const uint64_t cyc = fwd ? (uint64_t)k : (uint64_t)(len - 1 - k);
-                const uint64_t qc = (unsigned char)qu[k];
-                cnt[(qc << 12) | (cyc << 4) | ((uint64_t)(ref & 3) << 2) | (((unsigned)sq[k] >> 1) & 3)]++;
-            }
-        }
-    }
     for (size_t i = 0; i < (size_t)256 * 256 * 16; i++) pm[i] = 1.0;
     const uint64_t kPow = 10;  // "statistically powerful" count
     for (int qc = p.q_min; qc <= p.q_max; qc++) {
-        const uint64_t* row = &cnt[(size_t)qc << 12];
+        const uint64_t* row = &cnt[(size_t)(qc - p.q_min) << 12];
         uint64_t by_type[16] = {0}, by_ref[4] = {0}, total = 0, mism = 0;
         for (int c = 0; c < p.read_length; c++)
             for (int t = 0; t < 16; t++) {

@@ -45,7 +45,7 @@ int load_alignments(const char* soap, const gsnp_params& p, const Genome& g, std
                     std::string& err);
 
 // Counting-trained calibration matrix (cpp/matrix.cc:42-213) over the blocks.
-void matrix_gen(const std::vector<ReadBlock>& blocks, const Genome& g, const gsnp_params& p, double* pm);
+void matrix_from_counts(const uint64_t* cnt, const gsnp_params& p, double* pm);
 int matrix_read(const char* path, double* pm, std::string& err);
 int matrix_write(const char* path, const double* pm, const gsnp_params& p, std::string& err);
 

@@ -121,6 +121,8 @@ struct gsnp_job {
 
 namespace gsnp {
 
+int job_select(gsnp_job* job, int index);
+
 int job_open(const char* fasta, const char* soap, const char* dbsnp, const char* matrix_in, const char* matrix_out,
              const gsnp_params& p, std::unique_ptr<gsnp_job>& job)
 {
@@ -135,14 +137,28 @@ int job_open(const char* fasta, const char* soap, const char* dbsnp, const char*
     if (matrix_in && matrix_in[0]) {
         rc = matrix_read(matrix_in, job->pm.data(), err);
         if (rc) return fail(rc, err);
+        rc = create_ctx(&p, job->pm.data(), &job->ctx);
+        if (rc) return rc;
     } else {
-        matrix_gen(job->blocks, job->g, p, job->pm.data());
+        // pass 1 on the device: each chromosome's reads are uploaded and
+        // counted, the last one stays selected for calling
+        rc = create_ctx(&p, nullptr, &job->ctx);
+        if (rc) return rc;
+        std::vector<uint64_t> cnt(kCalibBins, 0);
+        for (int i = 0; i < (int)job->blocks.size(); i++) {
+            rc = job_select(job.get(), i);
+            if (!rc) rc = gsnp_calib_count(job->ctx, cnt.data());
+            if (rc) return rc;
+        }
+        matrix_from_counts(cnt.data(), p, job->pm.data());
+        rc = job->ctx->set_matrix(job->pm.data());
+        if (rc) return rc;
     }
     if (matrix_out && matrix_out[0]) {
         rc = matrix_write(matrix_out, job->pm.data(), p, err);
         if (rc) return fail(rc, err);
     }
-    return gsnp_create(&p, job->pm.data(), &job->ctx);
+    return GSNP_OK;
 }
 
 int job_select(gsnp_job* job, int index)
@@ -304,14 +320,11 @@ int gsnp_decompress(const char* prefix, const char* out)
 
 int gsnp_matrix_gen(const char* fasta, const char* soap, const char* dbsnp, const gsnp_params* p, double* pm)
 {
-    std::string err;
-    gsnp::Genome g;
-    int rc = gsnp::load_genome(fasta, dbsnp, g, err);
-    if (rc) return gsnp::fail(rc, err);
-    std::vector<gsnp::ReadBlock> blocks;
-    rc = gsnp::load_alignments(soap, *p, g, blocks, err);
-    if (rc) return gsnp::fail(rc, err);
-    gsnp::matrix_gen(blocks, g, *p, pm);
+    if (!fasta || !soap || !p || !pm) return gsnp::fail(GSNP_E_ARG, "null argument");
+    std::unique_ptr<gsnp_job> job;
+    int rc = gsnp::job_open(fasta, soap, dbsnp, nullptr, nullptr, *p, job);
+    if (rc) return rc;
+    std::copy(job->pm.begin(), job->pm.end(), pm);
     return GSNP_OK;
 }
 

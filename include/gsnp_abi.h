@@ -101,6 +101,19 @@ int gsnp_device_init(int device);
  * (kernel/gsnp_kernel.cu:24-33). */
 int gsnp_create(const gsnp_params* p, const double* p_matrix, gsnp_ctx** out);
 
+/* Replace the score table with one built from another calibration matrix
+ * (same layout as gsnp_create's).  Replaces the table half of
+ * GSNPKernelInit, kernel/gsnp_kernel.cu:24-33, for a context whose matrix is
+ * counted after its reads are on the device. */
+int gsnp_set_matrix(gsnp_ctx* ctx, const double* p_matrix);
+
+/* Calibration counts of the reads loaded into the context, added into
+ * counts[64 << 12], bin (q - q_min) << 12 | cycle << 4 | ref << 2 | obs:
+ * unique-hit bases observed at or above q_min on reference bases that are
+ * neither N nor a known SNP.  Replaces the counting loop of
+ * Prob_matrix::matrix_gen, cpp/matrix.cc:58-205. */
+int gsnp_calib_count(gsnp_ctx* ctx, uint64_t* counts);
+
 /* Replaces GSNPKernelFinalize, kernel/gsnp_kernel.cu:36-42. */
 void gsnp_destroy(gsnp_ctx* ctx);
 
@@ -175,8 +188,9 @@ int gsnp_run(const char* fasta, const char* soap, const char* dbsnp, const char*
 int gsnp_decompress(const char* prefix, const char* out);
 
 /* Counting-trained calibration matrix from a SOAP file, FASTA and optional
- * known-SNP file (whose sites are excluded from counting).  Replaces
- * Prob_matrix::matrix_gen, cpp/matrix.cc:42-213. */
+ * known-SNP file (whose sites are excluded from counting); the counts are
+ * taken on the device (gsnp_calib_count).  Replaces Prob_matrix::matrix_gen,
+ * cpp/matrix.cc:42-213. */
 int gsnp_matrix_gen(const char* fasta, const char* soap, const char* dbsnp, const gsnp_params* p, double* p_matrix);
 
 /* Dictionary encoding of a 32-bit integer column on the device: sorted

@@ -87,6 +87,11 @@ def orc() -> ctypes.CDLL:
             "orc_run": (i, [ctypes.c_char_p] * 5 + [ctypes.POINTER(OrcParams), ctypes.c_char_p, i, ctypes.c_int64, P]),
             "orc_decompress": (i, [ctypes.c_char_p, ctypes.c_char_p]),
             "orc_last_error": (ctypes.c_char_p, []),
+            "orc_genome_load": (i, [ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(P)]),
+            "orc_genome_free": (None, [P]),
+            "orc_reads_load": (i, [ctypes.c_char_p, ctypes.POINTER(OrcParams), P, ctypes.POINTER(P)]),
+            "orc_reads_free": (None, [P]),
+            "orc_matrix_gen": (i, [P, P, ctypes.POINTER(OrcParams), P]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -135,6 +140,31 @@ def run(fasta, soap, out, dbsnp=None, compress=False, max_sites=0, **kw) -> None
                        max_sites, None)
     if rc:
         raise RuntimeError(f"oracle error {rc}: {orc().orc_last_error().decode()}")
+
+
+def matrix_gen(fasta, soap, dbsnp=None, **kw):
+    """The oracle's counting-trained calibration matrix (256*256*16 doubles)."""
+    import numpy as np
+
+    L = orc()
+    p = params(**kw)
+    b = lambda s: None if s is None else os.fsencode(s)
+    g, r = ctypes.c_void_p(), ctypes.c_void_p()
+    rc = L.orc_genome_load(b(fasta), b(dbsnp), ctypes.byref(g))
+    if rc:
+        raise RuntimeError(f"oracle error {rc}: {L.orc_last_error().decode()}")
+    try:
+        rc = L.orc_reads_load(b(soap), ctypes.byref(p), g, ctypes.byref(r))
+        if rc:
+            raise RuntimeError(f"oracle error {rc}: {L.orc_last_error().decode()}")
+        pm = np.empty(256 * 256 * 16, dtype=np.float64)
+        rc = L.orc_matrix_gen(r, g, ctypes.byref(p), pm.ctypes.data)
+        L.orc_reads_free(r)
+        if rc:
+            raise RuntimeError(f"oracle error {rc}: {L.orc_last_error().decode()}")
+        return pm
+    finally:
+        L.orc_genome_free(g)
 
 
 def decompress(prefix, out) -> None:

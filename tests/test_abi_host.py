@@ -86,6 +86,7 @@ def test_matrix_gen_rejects_unknown_chromosome(built, tmp_path):
     assert rc == 4, built.lib().gsnp_last_error()
 
 
+@pytest.mark.gpu
 def test_matrix_gen_counts_a_clean_read(built, tmp_path):
     ref = "ACGT" * 100
     fa, so = _write_inputs(tmp_path, [_soap_line(f"r{i}", 1 + 4 * i, ref[4 * i:4 * i + 10]) for i in range(60)], ref)

@@ -200,6 +200,21 @@ def test_compressed_long_span(built, tmp_path):
     assert dec.read_bytes() == txt.read_bytes(), _first_diff(dec, txt)
 
 
+@pytest.mark.parametrize("name", ["c2w3", "stress", "c3s"])
+def test_calibration_matches_oracle(datasets, name):
+    """Pass-1 calibration counted on the device (gsnp_calib_count over every
+    chromosome's reads, then the matrix on the host) equals the oracle's
+    matrix_gen, double for double; c3s has 24 chromosomes, stress known SNPs."""
+    import gsnp_amd
+
+    ds = datasets(name)
+    print(f"[parity] calibration {name}", flush=True)
+    gpu = gsnp_amd.matrix_gen(ds.fasta, ds.soap, dbsnp=ds.dbsnp, **ds.flags)
+    ref = oracle_lib.matrix_gen(ds.fasta, ds.soap, dbsnp=ds.dbsnp, **ds.flags)
+    bad = np.flatnonzero(gpu.view(np.uint64) != ref.view(np.uint64))
+    assert bad.size == 0, f"{bad.size} entries differ, first at {bad[0]}: {gpu[bad[0]]!r} vs {ref[bad[0]]!r}"
+
+
 def test_sort_likelihood_unit(built):
     """Random record sets through the sort + likelihood kernels vs the oracle."""
     import gsnp_amd
