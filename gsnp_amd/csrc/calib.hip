@@ -1,5 +1,5 @@
 // calib.hip -- pass-1 calibration counts on the device (the counting loop of
-// Prob_matrix::matrix_gen, cpp/matrix.cc:58-205): every base of a unique-hit
+// Prob_matrix::matrix_gen, cpp/matrix.cc:58-121): every base of a unique-hit
 // read that was observed (A, C, G or T at or above the quality floor) and
 // lies on a reference base that is neither N nor a known SNP counts once in
 // the bin (quality, cycle, reference base, observed base).  The counts are

@@ -325,7 +325,7 @@ int load_alignments(const char* soap, const gsnp_params& p, const Genome& g, std
     return GSNP_OK;
 }
 
-// The matrix from calibration counts (cpp/matrix.cc:151-205), cnt indexed
+// The matrix from calibration counts (cpp/matrix.cc:129-205), cnt indexed
 // (qc - q_min) << 12 | cycle << 4 | ref << 2 | obs for quality characters qc.
 void matrix_from_counts(const uint64_t* cnt, const gsnp_params& p, double* pm)
 {

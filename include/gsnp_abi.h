@@ -111,7 +111,7 @@ int gsnp_set_matrix(gsnp_ctx* ctx, const double* p_matrix);
  * counts[64 << 12], bin (q - q_min) << 12 | cycle << 4 | ref << 2 | obs:
  * unique-hit bases observed at or above q_min on reference bases that are
  * neither N nor a known SNP.  Replaces the counting loop of
- * Prob_matrix::matrix_gen, cpp/matrix.cc:58-205. */
+ * Prob_matrix::matrix_gen, cpp/matrix.cc:58-121. */
 int gsnp_calib_count(gsnp_ctx* ctx, uint64_t* counts);
 
 /* Replaces GSNPKernelFinalize, kernel/gsnp_kernel.cu:36-42. */
