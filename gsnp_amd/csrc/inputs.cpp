@@ -9,6 +9,12 @@
 #include <cstdlib>
 #include <cstring>
 #include <memory>
+#include <thread>
+
+#include <fcntl.h>
+#include <sys/mman.h>
+#include <sys/stat.h>
+#include <unistd.h>
 
 namespace gsnp {
 
@@ -206,24 +212,37 @@ int load_genome(const char* fasta, const char* dbsnp, Genome& g, std::string& er
     return GSNP_OK;
 }
 
-int load_alignments(const char* soap, const gsnp_params& p, const Genome& g, std::vector<ReadBlock>& blocks,
-                    std::string& err)
+namespace {
+
+// One SOAP line that passed the per-line checks, in file order.
+struct ParsedRead {
+    int32_t ci, pos, len_hit;
+    char strand;
+    bool beyond;  // ends at or past the chromosome end: kept out of the reads
+};
+
+// The lines of one slice of the file: its reads (seq/qual `L` bytes each)
+// up to its first per-line error, and that error.
+struct ParsedSlice {
+    std::vector<ParsedRead> reads;
+    std::vector<char> seq, qual;
+    int err = GSNP_OK;
+    std::string msg;
+};
+
+// Per-line part of the SOAP reader (cpp/soap_snp.h:132-169): fields, indel
+// edits, chromosome lookup and range checks.  What depends on earlier lines
+// (chromosome grouping, the order check) is left to the merge.
+void parse_slice(const char* b, const char* e, const gsnp_params& p, const Genome& g, ParsedSlice& out)
 {
-    FilePtr f(std::fopen(soap, "r"));
-    if (!f) {
-        err = std::string("cannot open ") + soap;
-        return GSNP_E_IO;
-    }
     const int L = p.read_length;
-    LineReader lr(f.get());
-    const char* s;
-    size_t n;
     std::vector<char> rd(1024), ql(1024);
-    std::vector<char> seen(g.chr.size(), 0);
-    int cur = -1;
-    int32_t last = 0;
-    while (lr.next(s, n)) {
-        Scan sc{s, s + n};
+    std::string name;
+    while (b < e) {
+        const char* nl = static_cast<const char*>(std::memchr(b, '\n', (size_t)(e - b)));
+        const char* le = nl ? nl : e;
+        Scan sc{b, le};
+        b = nl ? nl + 1 : e;
         const char *id, *r0, *q0, *chr;
         int nid, nr, nq, nchr, hit, rlen, pos, mm;
         char ab, strand;
@@ -243,8 +262,9 @@ int load_alignments(const char* soap, const gsnp_params& p, const Genome& g, std
             bool ins_n = mm > 200;
             bool bad = ins_n ? (ip < 0 || ip > nr || ip > nq || il > 1024) : (ip < 0 || ip + il > nr || ip + il > nq);
             if (bad) {
-                err = "indel position out of range";
-                return GSNP_E_FORMAT;
+                out.err = GSNP_E_FORMAT;
+                out.msg = "indel position out of range";
+                return;
             }
             int skip = ins_n ? 0 : il;
             int tail = rlen - ip - skip;
@@ -267,18 +287,22 @@ int load_alignments(const char* soap, const gsnp_params& p, const Genome& g, std
         }
         pos -= 1;
         if (pos < 0) continue;
-        auto it = g.index.find(std::string(chr, nchr));
+        name.assign(chr, nchr);
+        auto it = g.index.find(name);
         if (it == g.index.end()) {
-            err = "Chromosome: !" + std::string(chr, nchr) + "! NOT found";
-            return GSNP_E_CHROM;
+            out.err = GSNP_E_CHROM;
+            out.msg = "Chromosome: !" + name + "! NOT found";
+            return;
         }
         if (rlen < 1 || rlen > L) {
-            err = "read length " + std::to_string(rlen) + " outside [1," + std::to_string(L) + "]";
-            return GSNP_E_RANGE;
+            out.err = GSNP_E_RANGE;
+            out.msg = "read length " + std::to_string(rlen) + " outside [1," + std::to_string(L) + "]";
+            return;
         }
         if (hit < 0 || hit > 65535) {
-            err = "hit count " + std::to_string(hit) + " outside [0,65535]";
-            return GSNP_E_RANGE;
+            out.err = GSNP_E_RANGE;
+            out.msg = "hit count " + std::to_string(hit) + " outside [0,65535]";
+            return;
         }
         if (hit == 1) {
             for (int k = 0; k < rlen && k < elen; k++) {
@@ -286,41 +310,168 @@ int load_alignments(const char* soap, const gsnp_params& p, const Genome& g, std
                 if (!acgt(es[k]) || qc < p.q_min) continue;
                 int q = qc - p.q_min;
                 if (q < 1 || q > 63) {
-                    err = "quality " + std::to_string(q) + " outside [1,63] on a unique-hit base";
-                    return GSNP_E_QUAL;
+                    out.err = GSNP_E_QUAL;
+                    out.msg = "quality " + std::to_string(q) + " outside [1,63] on a unique-hit base";
+                    return;
                 }
             }
         }
         const int ci = it->second;
-        if (ci != cur) {
-            if (seen[ci]) {
-                err = "reads of chromosome " + g.chr[ci].name + " are not contiguous";
+        ParsedRead r;
+        r.ci = ci;
+        r.pos = pos;
+        r.len_hit = (int32_t)(((uint32_t)rlen << 16) | (uint32_t)hit);
+        r.strand = strand;
+        r.beyond = (int64_t)pos + rlen >= (int64_t)g.chr[ci].code.size();
+        out.reads.push_back(r);
+        if (r.beyond) continue;
+        const size_t at = out.seq.size();
+        out.seq.resize(at + L);
+        out.qual.resize(at + L);
+        for (int k = 0; k < L; k++) {
+            out.seq[at + k] = k < elen ? es[k] : 'N';
+            out.qual[at + k] = k < eqlen ? eq[k] : 0;
+        }
+    }
+}
+
+// The file's bytes, memory-mapped (or read, where mapping is refused).
+struct FileBytes {
+    const char* p = nullptr;
+    size_t n = 0;
+    void* map = nullptr;
+    std::vector<char> copy;
+    ~FileBytes()
+    {
+        if (map) munmap(map, n);
+    }
+    bool open(const char* path)
+    {
+        const int fd = ::open(path, O_RDONLY);
+        if (fd < 0) return false;
+        struct stat st;
+        if (fstat(fd, &st) == 0 && S_ISREG(st.st_mode) && st.st_size > 0) {
+            void* m = mmap(nullptr, (size_t)st.st_size, PROT_READ, MAP_PRIVATE, fd, 0);
+            if (m != MAP_FAILED) {
+                map = m;
+                n = (size_t)st.st_size;
+                p = static_cast<const char*>(m);
+                ::close(fd);
+                return true;
+            }
+        }
+        char buf[1 << 16];
+        ssize_t k;
+        while ((k = ::read(fd, buf, sizeof buf)) > 0) copy.insert(copy.end(), buf, buf + k);
+        ::close(fd);
+        if (k < 0) return false;
+        p = copy.data();
+        n = copy.size();
+        return true;
+    }
+};
+
+}  // namespace
+
+int load_alignments(const char* soap, const gsnp_params& p, const Genome& g, std::vector<ReadBlock>& blocks,
+                    std::string& err)
+{
+    FileBytes fb;
+    if (!fb.open(soap)) {
+        err = std::string("cannot open ") + soap;
+        return GSNP_E_IO;
+    }
+    const int L = p.read_length;
+    // slices of whole lines, parsed on their own threads (up to 16, one per
+    // 4 MB; GSNP_PARSE_THREADS sets the number of slices)
+    const size_t hw = std::max(1u, std::thread::hardware_concurrency());
+    size_t T = std::max<size_t>(1, std::min<size_t>({hw, 16, fb.n / (4u << 20) + 1}));
+    if (const char* v = std::getenv("GSNP_PARSE_THREADS")) T = std::max<size_t>(1, std::min<size_t>(64, std::strtoul(v, nullptr, 10)));
+    std::vector<size_t> cut(T + 1, fb.n);
+    cut[0] = 0;
+    for (size_t i = 1; i < T; i++) {
+        size_t c = std::max(cut[i - 1], fb.n * i / T);
+        const char* nl = c < fb.n ? static_cast<const char*>(std::memchr(fb.p + c, '\n', fb.n - c)) : nullptr;
+        cut[i] = nl ? (size_t)(nl - fb.p) + 1 : fb.n;
+    }
+    std::vector<ParsedSlice> sl(T);
+    {
+        std::vector<std::thread> th;
+        for (size_t i = 1; i < T; i++)
+            th.emplace_back([&, i] { parse_slice(fb.p + cut[i], fb.p + cut[i + 1], p, g, sl[i]); });
+        parse_slice(fb.p + cut[0], fb.p + cut[1], p, g, sl[0]);
+        for (auto& t : th) t.join();
+    }
+    // kept reads per block, so that each block is allocated once
+    std::vector<uint64_t> per_block;
+    {
+        int c = -1;
+        for (const ParsedSlice& s : sl)
+            for (const ParsedRead& r : s.reads) {
+                if (r.ci != c) per_block.push_back(0), c = r.ci;
+                per_block.back() += r.beyond ? 0 : 1;
+            }
+    }
+    // merge in file order: chromosome grouping, the order check
+    // (cpp/call_genotype.cc:495-519), the first error in file order
+    std::vector<char> seen(g.chr.size(), 0);
+    int cur = -1;
+    int32_t last = 0;
+    for (size_t i = 0; i < T; i++) {
+        ParsedSlice& s = sl[i];
+        // the slice's kept reads so far (k) and the first of them not yet
+        // copied into the current block (run): runs are copied whole
+        size_t k = 0, run = 0;
+        auto flush = [&] {
+            if (k > run) {
+                ReadBlock& b = blocks.back();
+                b.seq.insert(b.seq.end(), s.seq.begin() + run * L, s.seq.begin() + k * L);
+                b.qual.insert(b.qual.end(), s.qual.begin() + run * L, s.qual.begin() + k * L);
+            }
+            run = k;
+        };
+        for (const ParsedRead& r : s.reads) {
+            if (r.ci != cur) {
+                flush();
+                if (seen[r.ci]) {
+                    err = "reads of chromosome " + g.chr[r.ci].name + " are not contiguous";
+                    return GSNP_E_UNSORTED;
+                }
+                seen[r.ci] = 1;
+                blocks.emplace_back();
+                ReadBlock& nb = blocks.back();
+                nb.chr = r.ci;
+                nb.pitch = (uint32_t)L;
+                const uint64_t want = per_block[blocks.size() - 1];
+                nb.pos.reserve(want);
+                nb.len_hit.reserve(want);
+                nb.strand.reserve(want);
+                nb.seq.reserve(want * L);
+                nb.qual.reserve(want * L);
+                cur = r.ci;
+                last = 0;
+            }
+            if (r.beyond) continue;
+            if (r.pos < last) {
+                flush();
+                err = "Errors in sorting:" + std::to_string(r.pos) + "<" + std::to_string(last);
                 return GSNP_E_UNSORTED;
             }
-            seen[ci] = 1;
-            blocks.emplace_back();
-            blocks.back().chr = ci;
-            blocks.back().pitch = (uint32_t)L;
-            cur = ci;
-            last = 0;
+            last = r.pos;
+            ReadBlock& b = blocks.back();
+            b.pos.push_back(r.pos);
+            b.len_hit.push_back(r.len_hit);
+            b.strand.push_back(r.strand);
+            k++;
         }
-        if ((int64_t)pos + rlen >= (int64_t)g.chr[ci].code.size()) continue;
-        if (pos < last) {
-            err = "Errors in sorting:" + std::to_string(pos) + "<" + std::to_string(last);
-            return GSNP_E_UNSORTED;
+        flush();
+        if (s.err) {
+            err = s.msg;
+            return s.err;
         }
-        last = pos;
-        ReadBlock& b = blocks.back();
-        b.pos.push_back(pos);
-        b.len_hit.push_back((int32_t)(((uint32_t)rlen << 16) | (uint32_t)hit));
-        b.strand.push_back(strand);
-        size_t at = b.seq.size();
-        b.seq.resize(at + L);
-        b.qual.resize(at + L);
-        for (int k = 0; k < L; k++) {
-            b.seq[at + k] = k < elen ? es[k] : 'N';
-            b.qual[at + k] = k < eqlen ? eq[k] : 0;
-        }
+        std::vector<ParsedRead>().swap(s.reads);  // release the slice as the merge passes it
+        std::vector<char>().swap(s.seq);
+        std::vector<char>().swap(s.qual);
     }
     return GSNP_OK;
 }

@@ -86,6 +86,36 @@ def test_matrix_gen_rejects_unknown_chromosome(built, tmp_path):
     assert rc == 4, built.lib().gsnp_last_error()
 
 
+@pytest.mark.parametrize("case", ["unsorted", "unknown", "noncontiguous", "unknown_then_unsorted"])
+def test_threaded_reader_reports_the_first_error(built, tmp_path, monkeypatch, case):
+    """The SOAP reader parses slices of the file on threads and merges them in
+    file order: with one slice or seven, the same first error is reported."""
+    ref = "ACGT" * 2500
+    lines = [_soap_line(f"r{i}", 1 + i, ref[i:i + 10]) for i in range(3000)]
+    if case == "unsorted":
+        lines[2500], lines[2501] = lines[2501], lines[2500]
+    elif case == "unknown":
+        lines[1700] = _soap_line("x", 1701, ref[1700:1710], chrom="chrZ")
+    elif case == "noncontiguous":
+        lines = [_soap_line("c2", 5, ref[4:14], chrom="chr2")] + lines + [_soap_line("c2b", 9, ref[8:18], chrom="chr2")]
+    else:
+        lines[400] = _soap_line("x", 401, ref[400:410], chrom="chrZ")
+        lines[2800], lines[2801] = lines[2801], lines[2800]
+    fa = tmp_path / "r.fa"
+    fa.write_text(">chr1\n" + ref + "\n>chr2\n" + ref + "\n")
+    so = tmp_path / "r.soap"
+    so.write_text("".join(lines))
+    got = []
+    for slices in ("1", "7"):
+        monkeypatch.setenv("GSNP_PARSE_THREADS", slices)
+        pm = np.zeros(256 * 256 * 16)
+        p = built.params(read_length=10)
+        rc = built.lib().gsnp_matrix_gen(os.fsencode(fa), os.fsencode(so), None, ctypes.byref(p), pm.ctypes.data)
+        got.append((rc, built.lib().gsnp_last_error()))
+    assert got[0] == got[1]
+    assert got[0][0] in (3, 4), got[0]
+
+
 @pytest.mark.gpu
 def test_matrix_gen_counts_a_clean_read(built, tmp_path):
     ref = "ACGT" * 100
