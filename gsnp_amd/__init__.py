@@ -128,6 +128,9 @@ SIGNATURES = {
     "gsnp_job_call": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_uint64)]),
     "gsnp_job_columns": (ctypes.c_int, [_P, ctypes.c_int, ctypes.POINTER(ctypes.c_uint64)]),
     "gsnp_set_matrix": (ctypes.c_int, [_P, _P]),
+    "gsnp_cns_open": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(_P)]),
+    "gsnp_cns_next": (ctypes.c_int, [_P, _P, ctypes.POINTER(ctypes.c_int)]),
+    "gsnp_cns_close": (None, [_P]),
     "gsnp_calib_count": (ctypes.c_int, [_P, _P]),
     "gsnp_job_context": (_P, [_P]),
     "gsnp_job_close": (None, [_P]),
@@ -192,6 +195,50 @@ def run(fasta, soap, out, dbsnp=None, matrix_in=None, matrix_out=None, compress=
 
 def decompress(prefix, out) -> None:
     check(lib().gsnp_decompress(_b(prefix), _b(out)))
+
+
+class CnsWindow(ctypes.Structure):
+    """One window of -C output as its 17 .cns columns (gsnp_cns_window)."""
+
+    _c, _i, _d = ctypes.POINTER(ctypes.c_char), ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_double)
+    _fields_ = [("chr", ctypes.c_char_p), ("start", ctypes.c_uint64), ("n", ctypes.c_uint64),
+                ("ref", _c), ("cns", _c), ("q_cns", _i), ("base1", _c), ("avg_q1", _i), ("count_uni1", _i),
+                ("count_all1", _i), ("base2", _c), ("avg_q2", _i), ("count_uni2", _i), ("count_all2", _i),
+                ("depth", _i), ("rank_sum", _d), ("copy_num", _d), ("dbsnp", _i)]
+
+
+def cns_windows(prefix):
+    """The windows of -C files, read in memory one at a time (gsnp_cns_open /
+    gsnp_cns_next): yields (chr, start, columns), the columns a dict of numpy
+    arrays named after gsnp_cns_window's fields (copies)."""
+    import numpy as np
+
+    L = lib()
+    r = ctypes.c_void_p()
+    check(L.gsnp_cns_open(_b(prefix), ctypes.byref(r)))
+    try:
+        w = CnsWindow()
+        has = ctypes.c_int()
+        while True:
+            check(L.gsnp_cns_next(r, ctypes.byref(w), ctypes.byref(has)))
+            if not has.value:
+                return
+            n = int(w.n)
+            cols = {}
+            for name, ctype, _ in (("ref", ctypes.c_char, 0), ("cns", ctypes.c_char, 0), ("q_cns", ctypes.c_int32, 0),
+                                   ("base1", ctypes.c_char, 0), ("avg_q1", ctypes.c_int32, 0),
+                                   ("count_uni1", ctypes.c_int32, 0), ("count_all1", ctypes.c_int32, 0),
+                                   ("base2", ctypes.c_char, 0), ("avg_q2", ctypes.c_int32, 0),
+                                   ("count_uni2", ctypes.c_int32, 0), ("count_all2", ctypes.c_int32, 0),
+                                   ("depth", ctypes.c_int32, 0), ("rank_sum", ctypes.c_double, 0),
+                                   ("copy_num", ctypes.c_double, 0), ("dbsnp", ctypes.c_int32, 0)):
+                ptr = getattr(w, name)
+                dt = {ctypes.c_char: np.uint8, ctypes.c_int32: np.int32, ctypes.c_double: np.float64}[ctype]
+                cols[name] = np.ctypeslib.as_array(ctypes.cast(ptr, ctypes.POINTER(ctype * n)).contents).view(dt).copy() \
+                    if n else np.zeros(0, dt)
+            yield w.chr.decode(), int(w.start), cols
+    finally:
+        L.gsnp_cns_close(r)
 
 
 def synth(config: int, seed: int, scale: float, prefix) -> tuple[int, int]:

@@ -129,138 +129,166 @@ int CWriter::close(std::string& err)
     return 0;
 }
 
-int decompress_files(const std::string& prefix, const std::string& out, std::string& err)
+ColumnReader::~ColumnReader()
 {
-    FILE* f12 = std::fopen((prefix + ".1_2").c_str(), "r");
-    if (!f12) {
+    for (FILE* f : {f12_, f346_, f5_, f10_, f11_, f15_})
+        if (f) std::fclose(f);
+}
+
+int ColumnReader::open(const std::string& prefix, std::string& err)
+{
+    f12_ = std::fopen((prefix + ".1_2").c_str(), "r");
+    if (!f12_) {
         err = "cannot open " + prefix + ".1_2";
         return 1;
     }
-    unsigned long nwin = 0;
-    if (std::fscanf(f12, "%lu", &nwin) != 1) {
-        std::fclose(f12);
+    if (std::fscanf(f12_, "%lu", &nwin_) != 1) {
         err = "bad window index";
         return 2;
     }
-    FILE* f346 = std::fopen((prefix + ".3_4_6").c_str(), "rb");
-    FILE* f5 = std::fopen((prefix + ".5_7_8_9_14_16").c_str(), "rb");
-    FILE* f10 = std::fopen((prefix + ".10").c_str(), "rb");
-    FILE* f11 = std::fopen((prefix + ".11_12_13").c_str(), "rb");
-    FILE* f15 = std::fopen((prefix + ".15_17").c_str(), "rb");
-    FILE* fo = std::fopen(out.c_str(), "w");
-    int rc = 0;
-    if (!f346 || !f5 || !f10 || !f11 || !f15 || !fo) {
+    f346_ = std::fopen((prefix + ".3_4_6").c_str(), "rb");
+    f5_ = std::fopen((prefix + ".5_7_8_9_14_16").c_str(), "rb");
+    f10_ = std::fopen((prefix + ".10").c_str(), "rb");
+    f11_ = std::fopen((prefix + ".11_12_13").c_str(), "rb");
+    f15_ = std::fopen((prefix + ".15_17").c_str(), "rb");
+    if (!f346_ || !f5_ || !f10_ || !f11_ || !f15_) {
         err = "cannot open the -C files";
-        rc = 1;
+        return 1;
     }
+    return 0;
+}
+
+// One window's chunks back to its 17 columns (the read_decompress calls of
+// decompressAll, cpp/decompression.cpp:130-255).
+int ColumnReader::next(CnsWindow& w, bool& got, std::string& err)
+{
+    got = false;
+    if (wi_ >= nwin_) return 0;
     char name[1024];
-    std::vector<char> c3, c4, c6, c10;
-    std::vector<int32_t> c5, c7, c8, c9, c14, c11, c12, c13, c17;
-    std::vector<double> c15, c16;
-    std::vector<char> line;
-    for (unsigned long wi = 0; !rc && wi < nwin; wi++) {
-        unsigned long ps, C;
-        if (std::fscanf(f12, "%1023s\t%lu\t%lu\n", name, &ps, &C) != 3) {
-            err = "bad window index row";
-            rc = 2;
-            break;
-        }
-        c3.assign(C, 'N'), c4.assign(C, 'N'), c6.assign(C, 'N'), c10.assign(C, 'N');
-        c5.assign(C, 0), c7.assign(C, 0), c8.assign(C, 0), c9.assign(C, 0), c14.assign(C, 0);
-        c11.assign(C, 0), c12.assign(C, 0), c13.assign(C, 0), c17.assign(C, 0);
-        c15.assign(C, 1.0), c16.assign(C, 255.0);
-        uint64_t pa[7];
-        bool ok = get(f346, pa, 8, 7);
-        if (ok && pa[0] != 1) {
-            std::vector<int32_t> id4(pa[5]), id6(pa[6]), idn(pa[4]);
-            std::vector<char> v4(pa[5]), v6(pa[6]);
-            std::vector<uint32_t> code(pa[3]);
-            ok = get(f346, id4.data(), 4, pa[5]) && get(f346, v4.data(), 1, pa[5]) && get(f346, id6.data(), 4, pa[6]) &&
-                 get(f346, v6.data(), 1, pa[6]) && get(f346, idn.data(), 4, pa[4]) && get(f346, code.data(), 4, pa[3]);
-            if (ok) {
-                decode2(code.data(), pa[3], C, c3.data());
-                for (int32_t i : idn) c3[i] = 'N';
-                c4 = c3;
-                for (size_t i = 0; i < id4.size(); i++) c4[id4[i]] = v4[i];
-                c6 = c3;
-                for (size_t i = 0; i < id6.size(); i++) c6[id6[i]] = v6[i];
-            }
-        }
-        int32_t nind = 1;
-        ok = ok && get(f5, &nind, 4, 1);
-        if (ok && nind != 1)
-            ok = read_rle_column(f5, C, c5.data()) && read_rle_column(f5, C, c7.data()) &&
-                 read_rle_column(f5, C, c8.data()) && read_rle_column(f5, C, c9.data()) &&
-                 read_rle_column(f5, C, c14.data()) && read_rle_column(f5, C, c16.data());
-        ok = ok && get(f10, &nind, 4, 1);
-        if (ok && nind != 1) {
-            uint64_t p3[3];
-            ok = get(f10, p3, 8, 3);
-            std::vector<uint32_t> code(ok ? p3[1] : 0), ni(ok ? p3[0] : 0);
-            ok = ok && get(f10, code.data(), 4, p3[1]) && get(f10, ni.data(), 4, p3[0]);
-            if (ok) {
-                decode2(code.data(), p3[1], C, c10.data());
-                for (uint32_t i : ni) c10[i] = 'N';
-            }
-        }
-        ok = ok && get(f11, &nind, 4, 1);
-        if (ok && nind != 1) {
-            uint64_t p4[4];
-            ok = get(f11, p4, 8, 4);
-            const uint64_t nc = ok ? p4[1] : 0, n12 = ok ? p4[2] : 0, n13 = ok ? p4[3] : 0;
-            std::vector<int32_t> x12i(n12), x12v(n12), x13i(n13), x13v(n13), cid(nc);
-            ok = ok && get(f11, x12i.data(), 4, n12) && get(f11, x12v.data(), 4, n12) && get(f11, x13i.data(), 4, n13) &&
-                 get(f11, x13v.data(), 4, n13) && get(f11, cid.data(), 4, nc);
-            std::vector<int32_t> vals[3];
-            for (int k = 0; k < 3 && ok; k++) {
-                uint64_t dp[2];
-                ok = get(f11, dp, 8, 2);
-                std::vector<uint32_t> words(ok ? dp[0] : 0);
-                std::vector<int32_t> dict(ok ? dp[1] : 0);
-                ok = ok && get(f11, words.data(), 4, dp[0]) && get(f11, dict.data(), 4, dp[1]);
-                vals[k].assign(nc, 0);
-                if (ok) dict_decode(words.data(), dp[0], dict.data(), dp[1], nc, vals[k].data());
-            }
-            if (ok) {
-                for (uint64_t i = 0; i < nc; i++) c11[cid[i]] = vals[0][i], c12[cid[i]] = vals[1][i], c13[cid[i]] = vals[2][i];
-                for (uint64_t i = 0; i < n12; i++) c12[x12i[i]] = x12v[i];
-                for (uint64_t i = 0; i < n13; i++) c13[x13i[i]] = x13v[i];
-            }
-        }
-        ok = ok && get(f15, &nind, 4, 1);
-        if (ok && nind != 1) {
-            int32_t n = 0;
-            ok = get(f15, &n, 4, 1);
-            std::vector<int32_t> id(ok ? n : 0);
-            std::vector<double> val(ok ? n : 0);
-            ok = ok && get(f15, id.data(), 4, n) && get(f15, val.data(), 8, n);
-            if (ok)
-                for (int32_t i = 0; i < n; i++) c15[id[i]] = val[i];
-            ok = ok && get(f15, &n, 4, 1);
-            if (ok) {
-                id.assign(n, 0);
-                ok = get(f15, id.data(), 4, n);
-                if (ok)
-                    for (int32_t i = 0; i < n; i++) c17[id[i]] = 1;
-            }
-        }
-        if (!ok) {
-            err = "truncated -C column file";
-            rc = 2;
-            break;
-        }
-        for (unsigned long i = 0; i < C; i++) {
-            if (c3[i] == 'N' && c4[i] == 'N' && c6[i] == 'N')
-                std::fprintf(fo, "%s\t%lu\tN\tN\t0\tN\t0\t0\t0\tN\t0\t0\t0\t0\t1.000\t255.000\t0\n", name, ps + i);
-            else
-                std::fprintf(fo, "%s\t%lu\t%c\t%c\t%d\t%c\t%d\t%d\t%d\t%c\t%d\t%d\t%d\t%d\t%#g\t%#g\t%d\n", name, ps + i,
-                             c3[i], c4[i], c5[i], c6[i], c7[i], c8[i], c9[i], c10[i], c11[i], c12[i], c13[i], c14[i],
-                             c15[i], c16[i], c17[i]);
+    unsigned long ps, C;
+    if (std::fscanf(f12_, "%1023s\t%lu\t%lu\n", name, &ps, &C) != 3) {
+        err = "bad window index row";
+        return 2;
+    }
+    wi_++;
+    w.chr = name;
+    w.start = ps;
+    w.n = C;
+    w.c3.assign(C, 'N'), w.c4.assign(C, 'N'), w.c6.assign(C, 'N'), w.c10.assign(C, 'N');
+    w.c5.assign(C, 0), w.c7.assign(C, 0), w.c8.assign(C, 0), w.c9.assign(C, 0), w.c14.assign(C, 0);
+    w.c11.assign(C, 0), w.c12.assign(C, 0), w.c13.assign(C, 0), w.c17.assign(C, 0);
+    w.c15.assign(C, 1.0), w.c16.assign(C, 255.0);
+    uint64_t pa[7];
+    bool ok = get(f346_, pa, 8, 7);
+    if (ok && pa[0] != 1) {
+        std::vector<int32_t> id4(pa[5]), id6(pa[6]), idn(pa[4]);
+        std::vector<char> v4(pa[5]), v6(pa[6]);
+        std::vector<uint32_t> code(pa[3]);
+        ok = get(f346_, id4.data(), 4, pa[5]) && get(f346_, v4.data(), 1, pa[5]) && get(f346_, id6.data(), 4, pa[6]) &&
+             get(f346_, v6.data(), 1, pa[6]) && get(f346_, idn.data(), 4, pa[4]) && get(f346_, code.data(), 4, pa[3]);
+        if (ok) {
+            decode2(code.data(), pa[3], C, w.c3.data());
+            for (int32_t i : idn) w.c3[i] = 'N';
+            w.c4 = w.c3;
+            for (size_t i = 0; i < id4.size(); i++) w.c4[id4[i]] = v4[i];
+            w.c6 = w.c3;
+            for (size_t i = 0; i < id6.size(); i++) w.c6[id6[i]] = v6[i];
         }
     }
-    std::fclose(f12);
-    for (FILE* f : {f346, f5, f10, f11, f15, fo})
-        if (f) std::fclose(f);
+    int32_t nind = 1;
+    ok = ok && get(f5_, &nind, 4, 1);
+    if (ok && nind != 1)
+        ok = read_rle_column(f5_, C, w.c5.data()) && read_rle_column(f5_, C, w.c7.data()) &&
+             read_rle_column(f5_, C, w.c8.data()) && read_rle_column(f5_, C, w.c9.data()) &&
+             read_rle_column(f5_, C, w.c14.data()) && read_rle_column(f5_, C, w.c16.data());
+    ok = ok && get(f10_, &nind, 4, 1);
+    if (ok && nind != 1) {
+        uint64_t p3[3];
+        ok = get(f10_, p3, 8, 3);
+        std::vector<uint32_t> code(ok ? p3[1] : 0), ni(ok ? p3[0] : 0);
+        ok = ok && get(f10_, code.data(), 4, p3[1]) && get(f10_, ni.data(), 4, p3[0]);
+        if (ok) {
+            decode2(code.data(), p3[1], C, w.c10.data());
+            for (uint32_t i : ni) w.c10[i] = 'N';
+        }
+    }
+    ok = ok && get(f11_, &nind, 4, 1);
+    if (ok && nind != 1) {
+        uint64_t p4[4];
+        ok = get(f11_, p4, 8, 4);
+        const uint64_t nc = ok ? p4[1] : 0, n12 = ok ? p4[2] : 0, n13 = ok ? p4[3] : 0;
+        std::vector<int32_t> x12i(n12), x12v(n12), x13i(n13), x13v(n13), cid(nc);
+        ok = ok && get(f11_, x12i.data(), 4, n12) && get(f11_, x12v.data(), 4, n12) &&
+             get(f11_, x13i.data(), 4, n13) && get(f11_, x13v.data(), 4, n13) && get(f11_, cid.data(), 4, nc);
+        std::vector<int32_t> vals[3];
+        for (int k = 0; k < 3 && ok; k++) {
+            uint64_t dp[2];
+            ok = get(f11_, dp, 8, 2);
+            std::vector<uint32_t> words(ok ? dp[0] : 0);
+            std::vector<int32_t> dict(ok ? dp[1] : 0);
+            ok = ok && get(f11_, words.data(), 4, dp[0]) && get(f11_, dict.data(), 4, dp[1]);
+            vals[k].assign(nc, 0);
+            if (ok) dict_decode(words.data(), dp[0], dict.data(), dp[1], nc, vals[k].data());
+        }
+        if (ok) {
+            for (uint64_t i = 0; i < nc; i++)
+                w.c11[cid[i]] = vals[0][i], w.c12[cid[i]] = vals[1][i], w.c13[cid[i]] = vals[2][i];
+            for (uint64_t i = 0; i < n12; i++) w.c12[x12i[i]] = x12v[i];
+            for (uint64_t i = 0; i < n13; i++) w.c13[x13i[i]] = x13v[i];
+        }
+    }
+    ok = ok && get(f15_, &nind, 4, 1);
+    if (ok && nind != 1) {
+        int32_t n = 0;
+        ok = get(f15_, &n, 4, 1);
+        std::vector<int32_t> id(ok ? n : 0);
+        std::vector<double> val(ok ? n : 0);
+        ok = ok && get(f15_, id.data(), 4, n) && get(f15_, val.data(), 8, n);
+        if (ok)
+            for (int32_t i = 0; i < n; i++) w.c15[id[i]] = val[i];
+        ok = ok && get(f15_, &n, 4, 1);
+        if (ok) {
+            id.assign(n, 0);
+            ok = get(f15_, id.data(), 4, n);
+            if (ok)
+                for (int32_t i = 0; i < n; i++) w.c17[id[i]] = 1;
+        }
+    }
+    if (!ok) {
+        err = "truncated -C column file";
+        return 2;
+    }
+    got = true;
+    return 0;
+}
+
+int decompress_files(const std::string& prefix, const std::string& out, std::string& err)
+{
+    ColumnReader rd;
+    int rc = rd.open(prefix, err);
+    if (rc) return rc;
+    FILE* fo = std::fopen(out.c_str(), "w");
+    if (!fo) {
+        err = "cannot create " + out;
+        return 1;
+    }
+    CnsWindow w;
+    bool got = true;
+    while (!rc) {
+        rc = rd.next(w, got, err);
+        if (rc || !got) break;
+        const char* name = w.chr.c_str();
+        for (uint64_t i = 0; i < w.n; i++) {
+            const unsigned long pos = (unsigned long)(w.start + i);
+            if (w.c3[i] == 'N' && w.c4[i] == 'N' && w.c6[i] == 'N')
+                std::fprintf(fo, "%s\t%lu\tN\tN\t0\tN\t0\t0\t0\tN\t0\t0\t0\t0\t1.000\t255.000\t0\n", name, pos);
+            else
+                std::fprintf(fo, "%s\t%lu\t%c\t%c\t%d\t%c\t%d\t%d\t%d\t%c\t%d\t%d\t%d\t%d\t%#g\t%#g\t%d\n", name, pos,
+                             w.c3[i], w.c4[i], w.c5[i], w.c6[i], w.c7[i], w.c8[i], w.c9[i], w.c10[i], w.c11[i], w.c12[i],
+                             w.c13[i], w.c14[i], w.c15[i], w.c16[i], w.c17[i]);
+        }
+    }
+    std::fclose(fo);
     return rc;
 }
 

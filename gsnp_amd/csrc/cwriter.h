@@ -32,6 +32,31 @@ private:
     std::vector<uint64_t> starts_, sizes_;
 };
 
+// One window of -C output as its 17 columns (CNSDataWindow,
+// cpp/decompression.cpp:7-128): chr, 1-based start, rows; columns 3, 4, 6,
+// 10 as characters, 15 and 16 as doubles, the rest as integers.
+struct CnsWindow {
+    std::string chr;
+    uint64_t start = 0, n = 0;
+    std::vector<char> c3, c4, c6, c10;
+    std::vector<int32_t> c5, c7, c8, c9, c11, c12, c13, c14, c17;
+    std::vector<double> c15, c16;
+};
+
+// Reads -C files one window at a time (decompressAll's loop,
+// cpp/decompression.cpp:130-255).
+class ColumnReader {
+public:
+    ~ColumnReader();
+    int open(const std::string& prefix, std::string& err);
+    // got = false at the end of the files
+    int next(CnsWindow& w, bool& got, std::string& err);
+
+private:
+    FILE *f12_ = nullptr, *f346_ = nullptr, *f5_ = nullptr, *f10_ = nullptr, *f11_ = nullptr, *f15_ = nullptr;
+    unsigned long nwin_ = 0, wi_ = 0;
+};
+
 // -C files back to .cns text (cpp/decompression.cpp:130-255).
 int decompress_files(const std::string& prefix, const std::string& out, std::string& err);
 

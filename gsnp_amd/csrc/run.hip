@@ -106,6 +106,11 @@ int call_chromosome(gsnp_ctx* ctx, const std::vector<int32_t>& pos, uint32_t N, 
 
 }  // namespace gsnp
 
+struct gsnp_cns_reader {
+    gsnp::ColumnReader rd;
+    gsnp::CnsWindow w;
+};
+
 struct gsnp_job {
     gsnp_params p{};
     gsnp::Genome g;
@@ -317,6 +322,39 @@ int gsnp_decompress(const char* prefix, const char* out)
     int rc = gsnp::decompress_files(prefix, out, err);
     return rc ? gsnp::fail(rc == 1 ? GSNP_E_IO : GSNP_E_FORMAT, err) : GSNP_OK;
 }
+
+int gsnp_cns_open(const char* prefix, gsnp_cns_reader** out)
+{
+    if (!prefix || !out) return gsnp::fail(GSNP_E_ARG, "null argument");
+    std::unique_ptr<gsnp_cns_reader> r(new gsnp_cns_reader());
+    std::string err;
+    int rc = r->rd.open(prefix, err);
+    if (rc) return gsnp::fail(rc == 1 ? GSNP_E_IO : GSNP_E_FORMAT, err);
+    *out = r.release();
+    return GSNP_OK;
+}
+
+int gsnp_cns_next(gsnp_cns_reader* r, gsnp_cns_window* w, int* has_window)
+{
+    if (!r || !w || !has_window) return gsnp::fail(GSNP_E_ARG, "null argument");
+    std::string err;
+    bool got = false;
+    int rc = r->rd.next(r->w, got, err);
+    *has_window = got ? 1 : 0;
+    if (rc) return gsnp::fail(GSNP_E_FORMAT, err);
+    if (!got) return GSNP_OK;
+    const gsnp::CnsWindow& c = r->w;
+    w->chr = c.chr.c_str();
+    w->start = c.start;
+    w->n = c.n;
+    w->ref = c.c3.data(), w->cns = c.c4.data(), w->q_cns = c.c5.data(), w->base1 = c.c6.data();
+    w->avg_q1 = c.c7.data(), w->count_uni1 = c.c8.data(), w->count_all1 = c.c9.data(), w->base2 = c.c10.data();
+    w->avg_q2 = c.c11.data(), w->count_uni2 = c.c12.data(), w->count_all2 = c.c13.data(), w->depth = c.c14.data();
+    w->rank_sum = c.c15.data(), w->copy_num = c.c16.data(), w->dbsnp = c.c17.data();
+    return GSNP_OK;
+}
+
+void gsnp_cns_close(gsnp_cns_reader* r) { delete r; }
 
 int gsnp_matrix_gen(const char* fasta, const char* soap, const char* dbsnp, const gsnp_params* p, double* pm)
 {

@@ -187,6 +187,32 @@ int gsnp_run(const char* fasta, const char* soap, const char* dbsnp, const char*
 /* -C files back to .cns text.  Replaces decompressAll, cpp/decompression.cpp:130-255. */
 int gsnp_decompress(const char* prefix, const char* out);
 
+/* -C files read in memory one window at a time, as the 17 .cns columns.
+ * Replaces CNSDataWindow and the read_decompress loop of decompressAll,
+ * cpp/decompression.cpp:7-255 (the in-memory extraction the reference's
+ * README describes).  The arrays belong to the reader and stay valid until
+ * its next gsnp_cns_next or gsnp_cns_close. */
+typedef struct gsnp_cns_reader gsnp_cns_reader;
+typedef struct {
+    const char* chr;       /* column 1 */
+    uint64_t start;        /* column 2 of row 0 (1-based) */
+    uint64_t n;            /* rows */
+    const char* ref;       /* column 3 */
+    const char* cns;       /* column 4 */
+    const int32_t* q_cns;  /* column 5 */
+    const char* base1;     /* column 6 */
+    const int32_t* avg_q1, *count_uni1, *count_all1;  /* columns 7, 8, 9 */
+    const char* base2;                                 /* column 10 */
+    const int32_t* avg_q2, *count_uni2, *count_all2;  /* columns 11, 12, 13 */
+    const int32_t* depth;                              /* column 14 */
+    const double* rank_sum, *copy_num;                 /* columns 15, 16 */
+    const int32_t* dbsnp;                              /* column 17 */
+} gsnp_cns_window;
+int gsnp_cns_open(const char* prefix, gsnp_cns_reader** out);
+/* *has_window = 0 once every window has been read */
+int gsnp_cns_next(gsnp_cns_reader* reader, gsnp_cns_window* w, int* has_window);
+void gsnp_cns_close(gsnp_cns_reader* reader);
+
 /* Counting-trained calibration matrix from a SOAP file, FASTA and optional
  * known-SNP file (whose sites are excluded from counting); the counts are
  * taken on the device (gsnp_calib_count).  Replaces Prob_matrix::matrix_gen,

@@ -131,3 +131,36 @@ def test_matrix_gen_counts_a_clean_read(built, tmp_path):
 def test_decompressor_reports_missing_files(built, tmp_path):
     rc = built.lib().gsnp_decompress(os.fsencode(tmp_path / "nothing"), os.fsencode(tmp_path / "out.cns"))
     assert rc == 1
+
+
+@pytest.mark.parametrize("name", ["c1", "stress"])
+def test_cns_reader_gives_the_oracle_rows(built, datasets, tmp_path, name):
+    """-C files written by the oracle, read in memory window by window
+    (gsnp_cns_open / gsnp_cns_next), give the oracle's .cns rows column for
+    column; stress has three chromosomes and known SNPs."""
+    import oracle_lib
+
+    ds = datasets(name)
+    prefix = tmp_path / name
+    oracle_lib.run(ds.fasta, ds.soap, prefix, dbsnp=ds.dbsnp, compress=True, **ds.flags)
+    rows = []
+    for chrom, start, c in built.cns_windows(prefix):
+        for i in range(len(c["ref"])):
+            ref, cns, b1 = chr(c["ref"][i]), chr(c["cns"][i]), chr(c["base1"][i])
+            if ref == cns == b1 == "N":
+                rows.append(f"{chrom}\t{start + i}\tN\tN\t0\tN\t0\t0\t0\tN\t0\t0\t0\t0\t1.000\t255.000\t0\n")
+                continue
+            rows.append("\t".join([chrom, str(start + i), ref, cns, str(c["q_cns"][i]), b1, str(c["avg_q1"][i]),
+                                   str(c["count_uni1"][i]), str(c["count_all1"][i]), chr(c["base2"][i]),
+                                   str(c["avg_q2"][i]), str(c["count_uni2"][i]), str(c["count_all2"][i]),
+                                   str(c["depth"][i]), "%#g" % c["rank_sum"][i], "%#g" % c["copy_num"][i],
+                                   str(c["dbsnp"][i])]) + "\n")
+    want = ds.oracle_text().read_text().splitlines(keepends=True)
+    assert len(rows) == len(want)
+    bad = [i for i in range(len(rows)) if rows[i] != want[i]]
+    assert not bad, f"{len(bad)} rows differ, first {bad[0]}: {rows[bad[0]]!r} vs {want[bad[0]]!r}"
+
+
+def test_cns_reader_reports_missing_files(built, tmp_path):
+    r = ctypes.c_void_p()
+    assert built.lib().gsnp_cns_open(os.fsencode(tmp_path / "nothing"), ctypes.byref(r)) == 1
