@@ -178,6 +178,36 @@ def test_compressed_output(datasets, tmp_path, monkeypatch, name, per, lds):
     assert txt.read_bytes() == ref.read_bytes(), _first_diff(txt, ref)
 
 
+@pytest.mark.parametrize("per", [0, 4])
+def test_compressed_many_distinct_values(built, tmp_path, per):
+    """Reads with random hit counts give a window tens of thousands of
+    distinct copy numbers (column 16): the dictionary's hash set outgrows LDS
+    and restarts in HBM, and the ranks take many tiles.  The -C files are the
+    oracle's bytes."""
+    import random
+
+    import gsnp_amd
+
+    rng = random.Random(7)
+    G, L = 300000, 50
+    ref = "".join(rng.choice("ACGT") for _ in range(G))
+    fa, so = tmp_path / "g.fa", tmp_path / "d.soap"
+    fa.write_text(">chrD\n" + "\n".join(ref[i:i + 80] for i in range(0, G, 80)) + "\n")
+    lines = []
+    for i, p in enumerate(sorted(rng.randrange(1, G - L - 1) for _ in range(G * 20 // L))):
+        seq = [b if rng.random() >= 0.01 else rng.choice("ACGT") for b in ref[p - 1:p - 1 + L]]
+        q = "".join(chr(64 + rng.randint(5, 40)) for _ in range(L))
+        hits = 1 if rng.random() < 0.4 else rng.randint(2, 400)
+        lines.append(f"r{i}\t{''.join(seq)}\t{q}\t{hits}\ta\t{L}\t{rng.choice('+-')}\tchrD\t{p}\t0\n")
+    so.write_text("".join(lines))
+    print(f"[parity] -C many distinct values batched={per}", flush=True)
+    gpu, cpu = tmp_path / "gpu", tmp_path / "cpu"
+    gsnp_amd.run(fa, so, gpu, compress=True, batched=per, read_length=L)
+    oracle_lib.run(fa, so, cpu, compress=True, read_length=L)
+    for sfx in SUFFIXES:
+        assert filecmp.cmp(str(gpu) + sfx, str(cpu) + sfx, shallow=False), f"{sfx} differs"
+
+
 def test_compressed_long_span(built, tmp_path):
     """A 5 Mb chromosome (20 windows, more than one encoder pass of 16 windows):
     the -C files of one batched span equal those of the per-window path, and
