@@ -206,6 +206,11 @@ def test_compressed_many_distinct_values(built, tmp_path, per):
     oracle_lib.run(fa, so, cpu, compress=True, read_length=L)
     for sfx in SUFFIXES:
         assert filecmp.cmp(str(gpu) + sfx, str(cpu) + sfx, shallow=False), f"{sfx} differs"
+    # the same calls as text: thousands of distinct %#g copy numbers
+    gt, ct = tmp_path / "gpu.cns", tmp_path / "cpu.cns"
+    gsnp_amd.run(fa, so, gt, batched=per, read_length=L)
+    oracle_lib.run(fa, so, ct, read_length=L)
+    assert gt.read_bytes() == ct.read_bytes(), _first_diff(gt, ct)
 
 
 def test_compressed_long_span(built, tmp_path):
