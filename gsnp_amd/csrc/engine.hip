@@ -7,6 +7,7 @@
 #include <cstring>
 #include <memory>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/gsnp_abi.h"
@@ -446,35 +447,68 @@ int gsnp_load_reads(gsnp_ctx* c, const gsnp_read_batch* b)
     const uint32_t L = (uint32_t)c->p.read_length;
     if (b->n && (b->pitch < 1 || !b->pos || !b->len_hit || !b->strand || !b->seq || !b->qual))
         return fail(GSNP_E_ARG, "incomplete read batch");
-    // validate and pack: byte = (q << 2) | base, 0 when the base is not observed
+    // validate and pack: byte = (q << 2) | base, 0 when the base is not observed;
+    // ranges of reads on threads, the error of the lowest read index reported
     std::vector<uint32_t> meta(b->n);
-    std::vector<uint8_t> seq((size_t)b->n * L, 0);
-    int32_t last = 0;
-    for (uint64_t i = 0; i < b->n; i++) {
-        const uint32_t lh = (uint32_t)b->len_hit[i];
-        const uint32_t len = lh >> 16, hit = lh & 0xffff;
-        const int32_t pos = b->pos[i];
-        if (len < 1 || len > L || len > b->pitch) return fail(GSNP_E_RANGE, "read length outside [1, read_length]");
-        if (pos < 0 || (uint64_t)pos + len >= c->chr_len)
-            return fail(GSNP_E_RANGE, "read outside the chromosome (filter pos >= 0 and pos+len < length first)");
-        if (pos < last) return fail(GSNP_E_UNSORTED, "Errors in sorting:" + std::to_string(pos) + "<" + std::to_string(last));
-        last = pos;
-        const bool rev = b->strand[i] != '+';
-        meta[i] = hit | (len << 16) | (rev ? 0x80000000u : 0u);
-        const char* s = b->seq + i * b->pitch;
-        const char* q = b->qual + i * b->pitch;
-        uint8_t* o = &seq[i * L];
-        for (uint32_t k = 0; k < len; k++) {
-            const int qc = (unsigned char)q[k];
-            if (!is_acgt(s[k]) || qc < c->p.q_min) continue;
-            int qq = qc - c->p.q_min;
-            if (hit == 1) {
-                if (qq < 1 || qq > 63) return fail(GSNP_E_QUAL, "quality outside [1,63] on a unique-hit base");
-            } else {
-                qq = 1;  // repeats only count the base; any non-zero quality marks it observed
+    std::unique_ptr<uint8_t[]> seq(new uint8_t[(size_t)b->n * L + 1]);
+    struct PackErr {
+        uint64_t at = UINT64_MAX;
+        int code = 0;
+        std::string msg;
+    };
+    const uint32_t q_min = (uint32_t)c->p.q_min;
+    const uint64_t chr_len = c->chr_len;
+    auto pack = [&](uint64_t i0, uint64_t i1, PackErr& e) {
+        int32_t last = i0 ? b->pos[i0 - 1] : 0;  // what the reads before i0 left, when they pass
+        for (uint64_t i = i0; i < i1; i++) {
+            const uint32_t lh = (uint32_t)b->len_hit[i];
+            const uint32_t len = lh >> 16, hit = lh & 0xffff;
+            const int32_t pos = b->pos[i];
+            if (len < 1 || len > L || len > b->pitch) {
+                e = {i, GSNP_E_RANGE, "read length outside [1, read_length]"};
+                return;
             }
-            o[k] = (uint8_t)((qq << 2) | (((unsigned)s[k] >> 1) & 3));
+            if (pos < 0 || (uint64_t)pos + len >= chr_len) {
+                e = {i, GSNP_E_RANGE, "read outside the chromosome (filter pos >= 0 and pos+len < length first)"};
+                return;
+            }
+            if (pos < last) {
+                e = {i, GSNP_E_UNSORTED, "Errors in sorting:" + std::to_string(pos) + "<" + std::to_string(last)};
+                return;
+            }
+            last = pos;
+            const bool rev = b->strand[i] != '+';
+            meta[i] = hit | (len << 16) | (rev ? 0x80000000u : 0u);
+            const char* sq = b->seq + i * b->pitch;
+            const char* q = b->qual + i * b->pitch;
+            uint8_t* o = &seq[i * L];
+            std::memset(o, 0, L);
+            for (uint32_t k = 0; k < len; k++) {
+                const uint32_t qc = (unsigned char)q[k];
+                if (!is_acgt(sq[k]) || qc < q_min) continue;
+                uint32_t qq = qc - q_min;
+                if (hit == 1) {
+                    if (qq < 1 || qq > 63) {
+                        e = {i, GSNP_E_QUAL, "quality outside [1,63] on a unique-hit base"};
+                        return;
+                    }
+                } else {
+                    qq = 1;  // repeats only count the base; any non-zero quality marks it observed
+                }
+                o[k] = (uint8_t)((qq << 2) | (((unsigned)sq[k] >> 1) & 3));
+            }
         }
+    };
+    {
+        const uint64_t hw = std::max(1u, std::thread::hardware_concurrency());
+        const uint64_t T = std::max<uint64_t>(1, std::min<uint64_t>({hw, 16, b->n / 65536 + 1}));
+        std::vector<PackErr> errs(T);
+        std::vector<std::thread> th;
+        for (uint64_t t = 1; t < T; t++) th.emplace_back(pack, b->n * t / T, b->n * (t + 1) / T, std::ref(errs[t]));
+        pack(0, b->n / T, errs[0]);
+        for (auto& x : th) x.join();
+        for (const PackErr& e : errs)  // ranges are in read order: the first error found is the lowest
+            if (e.code) return fail(e.code, e.msg);
     }
     HIPCHK(hipStreamSynchronize(c->stream));
     c->n_reads = b->n;
@@ -483,11 +517,12 @@ int gsnp_load_reads(gsnp_ctx* c, const gsnp_read_batch* b)
     c->h_rpos.assign(b->pos, b->pos + b->n);
     HIPCHK(c->rpos.alloc(b->n + 1));
     HIPCHK(c->rmeta.alloc(b->n + 1));
-    HIPCHK(c->rseq.alloc(seq.size() + 64));
+    const size_t nseq = (size_t)b->n * L;
+    HIPCHK(c->rseq.alloc(nseq + 64));
     if (b->n) {
         HIPCHK(hipMemcpyAsync(c->rpos.p, b->pos, 4 * b->n, hipMemcpyHostToDevice, c->stream));
         HIPCHK(hipMemcpyAsync(c->rmeta.p, meta.data(), 4 * b->n, hipMemcpyHostToDevice, c->stream));
-        HIPCHK(hipMemcpyAsync(c->rseq.p, seq.data(), seq.size(), hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipMemcpyAsync(c->rseq.p, seq.get(), nseq, hipMemcpyHostToDevice, c->stream));
     }
     HIPCHK(hipStreamSynchronize(c->stream));
     return GSNP_OK;

@@ -343,3 +343,46 @@ def test_device_dictionary_matches_reference_codec(built):
                 assert np.array_equal(w[:nw3], w3[:nw3]) and np.array_equal(d[:nd3.value], d3[:nd3.value])
     finally:
         L.gsnp_destroy(ctx)
+
+
+@pytest.mark.parametrize("case", ["sort_then_quality", "quality_then_sort", "length"])
+def test_load_reads_reports_the_first_read_error(built, case):
+    """gsnp_load_reads validates and packs ranges of reads on threads: the
+    error reported is the one of the lowest read index, as a read-by-read
+    loop finds it."""
+    import gsnp_amd
+
+    L = gsnp_amd.lib()
+    n, rl = 400000, 10
+    p = gsnp_amd.params(read_length=rl)
+    pm = np.full(256 * 256 * 16, 0.25)
+    ctx = ctypes.c_void_p()
+    gsnp_amd.check(L.gsnp_device_init(0))
+    gsnp_amd.check(L.gsnp_create(ctypes.byref(p), pm.ctypes.data, ctypes.byref(ctx)))
+    try:
+        G = n + 100
+        codes = np.zeros(G, dtype=np.uint8)
+        gsnp_amd.check(L.gsnp_set_chromosome(ctx, b"chr1", codes.ctypes.data, G, None, None, 0))
+        pos = np.arange(n, dtype=np.int32)
+        len_hit = np.full(n, (rl << 16) | 1, dtype=np.int32)
+        strand = np.full(n, ord("+"), dtype=np.uint8)
+        seq = np.full(n * rl, ord("A"), dtype=np.uint8)
+        qual = np.full(n * rl, ord("I"), dtype=np.uint8)
+        if case == "sort_then_quality":
+            pos[100000] = 5
+            qual[300000 * rl + 3] = ord("@")  # q - q_min = 0 on a unique-hit base
+            want = (3, b"Errors in sorting:5<99999")
+        elif case == "quality_then_sort":
+            qual[100000 * rl + 3] = ord("@")
+            pos[300000] = 5
+            want = (5, b"quality outside [1,63] on a unique-hit base")
+        else:
+            len_hit[350000] = ((rl + 1) << 16) | 1
+            qual[360000 * rl] = ord("@")
+            want = (6, b"read length outside [1, read_length]")
+        b = gsnp_amd.ReadBatch(n, rl, pos.ctypes.data, len_hit.ctypes.data, strand.ctypes.data, seq.ctypes.data,
+                               qual.ctypes.data)
+        rc = L.gsnp_load_reads(ctx, ctypes.byref(b))
+        assert (rc, L.gsnp_last_error()) == want
+    finally:
+        L.gsnp_destroy(ctx)
