@@ -32,6 +32,22 @@ STAGE_KERNELS = {"fused": ("k_bucket", "k_fused<"), "site_build": ("k_read_range
                  "format": ("k_rowlen", "k_gtext", "k_scan_blocks", "k_format")}
 
 
+ISSUE_PROFILE = ROOT / "profiles" / "calling_kernels_10Mb_issue.md"
+
+
+def measured_issue() -> dict | None:
+    """Share of SIMD cycles the fused kernels spend issuing VALU on the 10 Mb
+    workload, from the committed SQ counter summary (its last column)."""
+    if not ISSUE_PROFILE.exists():
+        return None
+    out = {}
+    for line in ISSUE_PROFILE.read_text().splitlines():
+        cols = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cols) == 7 and cols[0].startswith("`k_fused<") and cols[6].endswith("%"):
+            out[cols[0].strip("`")] = float(cols[6].rstrip(" %")) / 100.0
+    return out or None
+
+
 def measured_traffic(stage: str):
     """HBM bytes per pass of a stage on the 10 Mb workload (config 2, scale 1),
     from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary (measured MB
@@ -170,6 +186,10 @@ def main() -> int:
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "traffic_source": TRAFFIC_PROFILE.relative_to(ROOT).as_posix() if traffic else None,
                 "bytes_per_launch": int(a_bytes), "ms_per_launch": round(t_launch * 1e3, 4)}
+    # what bounds the dominant stage instead: VALU issue (committed counters)
+    issue = measured_issue() if (dom == "fused" and args.config == 2 and args.scale == 1.0) else None
+    compute_bound = {"bound": "valu_issue", "valu_issue_frac": issue,
+                     "source": ISSUE_PROFILE.relative_to(ROOT).as_posix()} if issue else None
     # the calling kernels together at B_site = 4n + 80 per called site
     call_ms = sum(stage_ms.get(k, 0) for k in ("sort", "likelihood", "posterior", "rank"))
     call_gbs = (4 * recs + 80 * sites) / (call_ms / 1e3) / 1e9 if call_ms else None
@@ -231,6 +251,7 @@ def main() -> int:
                    "windows_per_span": per if args.mode == "batched" else 1, "text_bytes": text_bytes,
                    "autotuned": tuned},
         "roofline": roofline,
+        "compute_bound": compute_bound,
         "call_kernels_gbs": round(call_gbs, 2) if call_gbs else None,
         "stage_ms_per_step": {k: round(v, 4) for k, v in stage_ms.items()},
         "cpu_baseline": cpu,
