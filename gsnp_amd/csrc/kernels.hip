@@ -247,6 +247,9 @@ __global__ __launch_bounds__(256) void k_bucket(const int32_t* nrec, uint32_t n_
     int cls[kBucketPer];
     uint32_t loc[kBucketPer];
     unsigned long long recs = 0;
+    // places within the wave's share of each class (wave-uniform running
+    // counts), then one LDS reservation per class per wave
+    uint32_t run[6] = {0, 0, 0, 0, 0, 0};
 #pragma unroll
     for (uint32_t it = 0; it < kBucketPer; it++) {
         const uint32_t s = s0 + it * 256u + threadIdx.x;
@@ -259,15 +262,27 @@ __global__ __launch_bounds__(256) void k_bucket(const int32_t* nrec, uint32_t n_
         }
         cls[it] = b;
         loc[it] = 0;
+#pragma unroll
         for (int k = 0; k < 6; k++) {
             const uint64_t mask = __ballot(b == k);
-            if (!mask) continue;
-            const uint32_t leader = (uint32_t)__ffsll((long long)mask) - 1;
-            uint32_t off = 0;
-            if (lane == leader) off = atomicAdd(&cnt[k], (uint32_t)__popcll(mask));
-            off = __shfl(off, (int)leader);
-            if (b == k) loc[it] = off + (uint32_t)__popcll(mask & below);
+            if (b == k) loc[it] = run[k] + (uint32_t)__popcll(mask & below);
+            run[k] += (uint32_t)__popcll(mask);
         }
+    }
+    uint32_t wbase[6];
+#pragma unroll
+    for (int k = 0; k < 6; k++) {
+        uint32_t off = 0;
+        if (lane == 0 && run[k]) off = atomicAdd(&cnt[k], run[k]);
+        wbase[k] = (uint32_t)__shfl((int)off, 0);
+    }
+#pragma unroll
+    for (uint32_t it = 0; it < kBucketPer; it++) {
+        const int b = cls[it];
+        uint32_t wb = 0;
+#pragma unroll
+        for (int k = 0; k < 6; k++) wb = b == k ? wbase[k] : wb;
+        loc[it] += wb;
     }
     for (int off = 32; off > 0; off >>= 1) recs += __shfl_xor(recs, off);
     if (lane == 0) rsum[threadIdx.x >> 6] = recs;
