@@ -7,6 +7,7 @@
 #include <cstring>
 #include <memory>
 #include <string>
+#include <system_error>
 #include <thread>
 #include <vector>
 
@@ -504,9 +505,18 @@ int gsnp_load_reads(gsnp_ctx* c, const gsnp_read_batch* b)
         const uint64_t T = std::max<uint64_t>(1, std::min<uint64_t>({hw, 16, b->n / 65536 + 1}));
         std::vector<PackErr> errs(T);
         std::vector<std::thread> th;
-        for (uint64_t t = 1; t < T; t++) th.emplace_back(pack, b->n * t / T, b->n * (t + 1) / T, std::ref(errs[t]));
+        std::vector<char> started(T, 0);  // a range whose thread cannot be started is packed here afterwards
+        for (uint64_t t = 1; t < T; t++) {
+            try {
+                th.emplace_back(pack, b->n * t / T, b->n * (t + 1) / T, std::ref(errs[t]));
+                started[t] = 1;
+            } catch (const std::system_error&) {
+            }
+        }
         pack(0, b->n / T, errs[0]);
         for (auto& x : th) x.join();
+        for (uint64_t t = 1; t < T; t++)
+            if (!started[t]) pack(b->n * t / T, b->n * (t + 1) / T, errs[t]);
         for (const PackErr& e : errs)  // ranges are in read order: the first error found is the lowest
             if (e.code) return fail(e.code, e.msg);
     }

@@ -9,6 +9,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <memory>
+#include <system_error>
 #include <thread>
 
 #include <fcntl.h>
@@ -396,11 +397,20 @@ int load_alignments(const char* soap, const gsnp_params& p, const Genome& g, std
     }
     std::vector<ParsedSlice> sl(T);
     {
+        // a slice whose thread cannot be started is parsed here afterwards
         std::vector<std::thread> th;
-        for (size_t i = 1; i < T; i++)
-            th.emplace_back([&, i] { parse_slice(fb.p + cut[i], fb.p + cut[i + 1], p, g, sl[i]); });
+        std::vector<char> started(T, 0);
+        for (size_t i = 1; i < T; i++) {
+            try {
+                th.emplace_back([&, i] { parse_slice(fb.p + cut[i], fb.p + cut[i + 1], p, g, sl[i]); });
+                started[i] = 1;
+            } catch (const std::system_error&) {
+            }
+        }
         parse_slice(fb.p + cut[0], fb.p + cut[1], p, g, sl[0]);
         for (auto& t : th) t.join();
+        for (size_t i = 1; i < T; i++)
+            if (!started[i]) parse_slice(fb.p + cut[i], fb.p + cut[i + 1], p, g, sl[i]);
     }
     // kept reads per block, so that each block is allocated once
     std::vector<uint64_t> per_block;
