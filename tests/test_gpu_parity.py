@@ -386,3 +386,21 @@ def test_load_reads_reports_the_first_read_error(built, case):
         assert (rc, L.gsnp_last_error()) == want
     finally:
         L.gsnp_destroy(ctx)
+
+
+def test_full_size_batched_equals_per_window(built, tmp_path):
+    """The 10 Mb workload (30x, 3 M reads, 291 M records): one batched span of
+    39 windows writes the same .cns bytes as the per-window path (a
+    size-independent check at the benchmark's full size, where the oracle is
+    too slow to run)."""
+    import gsnp_amd
+
+    pre = tmp_path / "c2"
+    _, rl = gsnp_amd.synth(2, 2024, 1.0, pre)
+    fa, so, snp = f"{pre}.fa", f"{pre}.soap", f"{pre}.snp"
+    print("[parity] 10 Mb batched vs per-window", flush=True)
+    a, b = tmp_path / "a.cns", tmp_path / "b.cns"
+    gsnp_amd.run(fa, so, a, dbsnp=snp, batched=64, read_length=rl, refine_mode=1)
+    gsnp_amd.run(fa, so, b, dbsnp=snp, batched=0, read_length=rl, refine_mode=1)
+    assert a.stat().st_size > 500_000_000
+    assert filecmp.cmp(a, b, shallow=False), _first_diff(a, b)
