@@ -90,7 +90,8 @@ def main() -> int:
     args = ap.parse_args()
 
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (multi-GPU sharding is design-only)"}))
+        print(json.dumps({"error": "the calling path runs on one GPU; --gpus must be 1 (sharding across GPUs is "
+                                   "designed, not built)"}))
         return 2
 
     import gsnp_amd
