@@ -1,19 +1,30 @@
 #!/usr/bin/env python3
 """Called sites per second of the SOAPsnp calling path on one MI355X.
 
-Workload: the 10 Mb plain-GPU configuration (seeded reference, 30x 100 bp
-pairs, 3% two-hit reads, 1% known-SNP prior file, refine mode on), generated
-by gsnp_amd's seeded generator.  A step is one calling pass over the whole
-chromosome: site matrix -> per-site sort -> FP64 likelihood -> posterior ->
-.cns rows formatted in device memory, with the reads already resident in HBM.
-The timed region excludes parsing, calibration and the host copy of the text.
+Workload (default): the uneven-depth configuration -- 24 seeded contigs with
+lengths log-uniform in 1-20 Mb (184 Mb at seed 2024), 100 bp pairs, depth per
+10 kb block ~ Gamma(4, 7.5), 1% repeat-like blocks at depth 200-255 with 1-3
+hits per read, 2% uncovered blocks -- generated in memory by gsnp_amd's seeded
+generator and calibrated on the device.  Every contig's reads stay resident in
+HBM.
 
-Prints one JSON line (metric, value, roofline of the dominant kernel from HIP
-events, and the CPU oracle timed on a bounded sample of the same workload).
+A step is one calling pass over every contig: site matrix -> depth classes ->
+fused sort + FP64 likelihood + posterior -> .cns rows formatted on the device
+and copied to pinned host memory, each span's copy running beside the next
+span's kernels.  The timed region starts with the reads in HBM and ends with
+every row in host memory.  Beside it, in the same run: the per-window (plain)
+form with one synchronous copy per window, the device-only time (rows left in
+HBM), the -C chunks instead of rows, per-stage HIP-event times with the
+roofline of the dominant stage, the CPU oracle on a sample of the same
+workload (one thread, and window-parallel on the host cores), and the adjoint
+leg on the 2 Mb miscalibrated workload.
+
+Prints one JSON line.
 """
 from __future__ import annotations
 
 import argparse
+import ctypes
 import json
 import os
 import subprocess
@@ -26,45 +37,116 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, 8 TB/s
-TRAFFIC_PROFILE = ROOT / "profiles" / "calling_path_10Mb_hbm_traffic.md"
-# kernels of each stage as named in the traffic profile
-STAGE_KERNELS = {"fused": ("k_bucket", "k_fused<"), "site_build": ("k_read_ranges", "k_site_build"),
-                 "format": ("k_rowlen", "k_gtext", "k_scan_blocks", "k_format")}
+WORKLOADS = {
+    2: "10 Mb reference, 30x 100 bp pairs (insert 500+-50), 3% 2-hit reads, 1% known SNPs, refine mode",
+    3: "24 contigs log-uniform 1-20 Mb, 100 bp pairs, depth per 10 kb ~ Gamma(4, 7.5), 1% repeat-like blocks "
+       "at depth 200-255 with 1-3 hits, 2% uncovered blocks",
+    5: "248 Mb reference, 30x 150 bp pairs (insert 400+-40), cycle-decaying qualities, rank-sum test",
+}
+TRAFFIC_PROFILE = {3: ROOT / "profiles" / "fused_stage_c3_hbm_traffic.md"}
 
 
-ISSUE_PROFILE = ROOT / "profiles" / "calling_kernels_10Mb_issue.md"
+def host_share() -> int:
+    """Host threads this run may use: the box's CPU share (OMP_NUM_THREADS)
+    when set, else the machine's."""
+    n = os.cpu_count() or 1
+    try:
+        omp = int(os.environ.get("OMP_NUM_THREADS", "0"))
+    except ValueError:
+        omp = 0
+    return max(1, min(n, omp) if omp > 0 else n)
 
 
-def measured_issue() -> dict | None:
-    """Share of SIMD cycles the fused kernels spend issuing VALU on the 10 Mb
-    workload, from the committed SQ counter summary (its last column)."""
-    if not ISSUE_PROFILE.exists():
-        return None
-    out = {}
-    for line in ISSUE_PROFILE.read_text().splitlines():
-        cols = [c.strip() for c in line.strip().strip("|").split("|")]
-        if len(cols) == 7 and cols[0].startswith("`k_fused<") and cols[6].endswith("%"):
-            out[cols[0].strip("`")] = float(cols[6].rstrip(" %")) / 100.0
-    return out or None
+def cpu_model() -> str:
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
-def measured_traffic(stage: str):
-    """HBM bytes per pass of a stage on the 10 Mb workload (config 2, scale 1),
-    from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary (measured MB
-    read + written, copy-calibrated); None when the summary is absent."""
-    if stage not in STAGE_KERNELS or not TRAFFIC_PROFILE.exists():
-        return None
-    total = 0.0
-    found = False
-    for line in TRAFFIC_PROFILE.read_text().splitlines():
-        cols = [c.strip() for c in line.strip().strip("|").split("|")]
-        if len(cols) != 5 or not cols[0].startswith("`"):
-            continue
-        name = cols[0].strip("`")
-        if any(name == k or (k.endswith("<") and name.startswith(k)) for k in STAGE_KERNELS[stage]):
-            total += (float(cols[3]) + float(cols[4])) * 1e6
-            found = True
-    return int(total) if found else None
+def device_sync() -> None:
+    """hipDeviceSynchronize through the HIP runtime libgsnp.so loaded."""
+    hip = ctypes.CDLL("libamdhip64.so.7")
+    rc = hip.hipDeviceSynchronize()
+    if rc != 0:
+        raise RuntimeError(f"hipDeviceSynchronize: {rc}")
+
+
+def measured_traffic(config: int, scale: float):
+    """HBM bytes per pass of the fused stage on this workload, from the
+    committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary (copy-calibrated);
+    counters cannot be read inside the timed run."""
+    prof = TRAFFIC_PROFILE.get(config)
+    if prof is None or scale != 1.0 or not prof.exists():
+        return None, None
+    for line in prof.read_text().splitlines():
+        if line.startswith("| fused stage total"):
+            cols = [c.strip() for c in line.strip().strip("|").split("|")]
+            return int(float(cols[1]) * 1e6), prof.relative_to(ROOT).as_posix()
+    return None, None
+
+
+def timed_passes(job, steps: int, warmup: int, per: int, mode: int):
+    for _ in range(warmup):
+        job.run_pass(batched=per, emit_text=mode)
+    device_sync()
+    t0 = time.perf_counter()
+    nb = ns = 0
+    for _ in range(steps):
+        nb, ns = job.run_pass(batched=per, emit_text=mode)
+    device_sync()
+    return (time.perf_counter() - t0) / steps, nb, ns
+
+
+def cpu_leg(args, work: Path, lens: list[int], sites_total: int) -> dict:
+    """The CPU oracle on a sample of the same workload: a few whole contigs
+    written as files; one thread over its first windows (about 10 s), and the
+    window-parallel run over the whole sample on the host cores."""
+    import gsnp_amd
+
+    sys.path.insert(0, str(ROOT / "tests"))
+    import oracle_lib
+
+    oracle_lib.ensure_built()
+    order = sorted(range(len(lens)), key=lambda i: lens[i])
+    pick, tot = [], 0
+    for i in order:  # the smallest contigs until the sample holds args.cpu_mb Mb
+        if tot >= args.cpu_mb * 1e6:
+            break
+        pick.append(i)
+        tot += lens[i]
+    pick.sort()
+    pre = work / "cpu_sample"
+    t0 = time.time()
+    gsnp_amd.synth_subset(args.config, args.seed, args.scale, pick, pre)
+    print(f"[bench] cpu sample: contigs {pick} ({tot / 1e6:.1f} Mb) written in {time.time() - t0:.1f}s",
+          file=sys.stderr, flush=True)
+    base = [str(oracle_lib.ORACLE_CLI), "-i", f"{pre}.soap", "-d", f"{pre}.fa", "-L", str(args.read_len), "-R"]
+    if args.config == 5:
+        base += ["-u"]
+    snp = Path(f"{pre}.snp")
+    if snp.exists():
+        base += ["-s", str(snp), "-2"]
+    res = subprocess.run(base + ["-o", str(work / "cpu1.cns"), "-X", str(args.cpu_sites)], capture_output=True,
+                         text=True, check=True)
+    one = json.loads(res.stdout.strip().splitlines()[-1])
+    threads = host_share()
+    res = subprocess.run(base + ["-o", str(work / "cpuN.cns"), "-J", str(threads)], capture_output=True, text=True,
+                         check=True)
+    many = json.loads(res.stdout.strip().splitlines()[-1])
+    (work / "cpu1.cns").unlink(missing_ok=True)
+    (work / "cpuN.cns").unlink(missing_ok=True)
+    names = ", ".join(f"chr{i + 1}" for i in pick)
+    return {"value": round(many["sites"] / many["seconds_call"], 1), "unit": "sites/s", "cores": threads,
+            "kind": "port", "cpu_model": cpu_model(),
+            "sample": f"contigs {names} ({many['sites']} sites, {many['windows']} windows) of the same workload as "
+                      f"files; the CPU oracle calling windows on {threads} threads, .cns text written; call stage "
+                      f"only (parsing excluded)",
+            "single_thread": {"value": round(one["sites"] / one["seconds_call"], 1), "unit": "sites/s", "cores": 1,
+                              "sample": f"first {one['sites']} sites ({one['windows']} windows) of the same sample"}}
 
 
 def main() -> int:
@@ -72,17 +154,19 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--config", type=int, default=2)
+    ap.add_argument("--config", type=int, default=3, choices=[1, 2, 3, 4, 5])
     ap.add_argument("--scale", type=float, default=1.0)
     ap.add_argument("--seed", type=int, default=2024)
-    ap.add_argument("--mode", choices=["batched", "plain"], default="batched")
-    ap.add_argument("--per", type=int, default=0, help="windows per batched span (0: tuned default)")
-    ap.add_argument("--compare", action="store_true", help="also time the other mode")
-    ap.add_argument("--cpu-sites", type=int, default=2 * 262144, help="sites in the CPU oracle sample")
+    ap.add_argument("--per", type=int, default=32, help="windows per batched span (-1: one span per contig)")
+    ap.add_argument("--plain-steps", type=int, default=1, help="timed passes of the per-window form")
+    ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-mb", type=float, default=12.0, help="Mb of the workload in the CPU sample")
+    ap.add_argument("--cpu-sites", type=int, default=16 * 262144, help="sites of the single-thread CPU run")
     ap.add_argument("--no-tune", action="store_true", help="keep the default fused launch shape")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint leg (2 Mb miscalibrated workload)")
-    ap.add_argument("--no-columns", action="store_true", help="skip the -C leg (column chunks instead of text)")
+    ap.add_argument("--no-columns", action="store_true", help="skip the -C leg (column chunks instead of rows)")
+    ap.add_argument("--no-stages", action="store_true", help="skip the per-stage timing passes")
     ap.add_argument("--adjoint-evals", type=int, default=10)
     ap.add_argument("--fd-entries", type=int, default=256)
     ap.add_argument("--refit-steps", type=int, default=50)
@@ -90,8 +174,8 @@ def main() -> int:
     args = ap.parse_args()
 
     if args.gpus != 1:
-        print(json.dumps({"error": "the calling path runs on one GPU; --gpus must be 1 (sharding across GPUs is "
-                                   "designed, not built)"}))
+        print(json.dumps({"error": "this build runs on one GPU: --gpus must be 1 (sharding across GPUs is designed "
+                                   "in the documentation, not built)"}))
         return 2
 
     import gsnp_amd
@@ -103,139 +187,93 @@ def main() -> int:
 
     work = Path(args.workdir) if args.workdir else Path(tempfile.mkdtemp(prefix="gsnp_bench_"))
     work.mkdir(parents=True, exist_ok=True)
-    pre = work / f"c{args.config}"
+    flags = {"rank_sum_mode": 1} if args.config == 5 else {}
+    if args.config == 2:
+        flags["refine_mode"] = 1
     t0 = time.time()
-    n_reads, rl = gsnp_amd.synth(args.config, args.seed, args.scale, pre)
-    fa, so = f"{pre}.fa", f"{pre}.soap"
-    snp = f"{pre}.snp" if Path(f"{pre}.snp").exists() else None
-    flags = dict(read_length=rl, refine_mode=1 if snp else 0)
-    print(f"[bench] generated config {args.config}: {n_reads} reads in {time.time() - t0:.1f}s", file=sys.stderr,
-          flush=True)
-    t0 = time.time()
-    job = gsnp_amd.Job(fa, so, dbsnp=snp, **flags)
-    name, chr_len, chr_reads = job.info(0)
-    job.select(0)
-    print(f"[bench] parsed + calibrated + uploaded {name} ({chr_len} bp, {chr_reads} reads) in "
-          f"{time.time() - t0:.1f}s", file=sys.stderr, flush=True)
+    job = gsnp_amd.Job(synth=(args.config, args.seed, args.scale), **flags)
+    args.read_len = job.read_len
+    contigs = [job.info(i) for i in range(len(job))]
+    sites = sum(c[1] for c in contigs)
+    reads = sum(c[2] for c in contigs)
+    lens = gsnp_amd.synth_contigs(args.config, args.seed, args.scale)
+    print(f"[bench] config {args.config} x{args.scale}: {len(contigs)} contigs, {sites} sites, {reads} reads "
+          f"generated, calibrated and uploaded in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
 
-    per = args.per or int(os.environ.get("GSNP_BATCH_WINDOWS", "0")) or (chr_len // gsnp_amd.WIN + 1)
+    per = args.per
     tuned = None
-    if args.mode == "batched" and not args.no_tune:
+    if not args.no_tune:
         t0 = time.time()
-        tuned = job.autotune(0, min(chr_len, per * gsnp_amd.WIN))
-        print(f"[bench] autotuned fused launch in {time.time() - t0:.1f}s: {tuned}", file=sys.stderr, flush=True)
+        big = max(range(len(contigs)), key=lambda i: contigs[i][1])
+        job.select(big)
+        span = contigs[big][1] if per < 0 else min(contigs[big][1], per * gsnp_amd.WIN)
+        tuned = job.autotune(0, span)
+        print(f"[bench] autotuned fused launch on {contigs[big][0]} in {time.time() - t0:.1f}s: {tuned}",
+              file=sys.stderr, flush=True)
 
-    def timed(mode: str):
-        """Seconds per pass with the stage timers off (the batched span and its
-        formatting replay as captured graphs), then the same number of passes
-        with the stage timers on (eager launches) for the per-stage times."""
-        batched = per if mode == "batched" else 0
-        for _ in range(args.warmup):
-            job.call(batched=batched)
-        t_start = time.perf_counter()
-        nbytes = 0
-        for _ in range(args.steps):
-            nbytes = job.call(batched=batched)
-        t_end = time.perf_counter()
-        job.records(reset=True)
+    # the headline: rows in pinned host memory
+    job.counters(reset=True)
+    secs, text_bytes, ns = timed_passes(job, args.steps, args.warmup, per, 2)
+    assert ns == sites
+    value = sites / secs
+    cnt = job.counters(reset=True)
+    recs = cnt["records"] // max(1, args.steps + args.warmup)
+    fixups = {"host_fixups_per_pass": cnt["host_fixups"] // max(1, args.steps + args.warmup)}
+    print(f"[bench] batched, rows to host: {secs * 1e3:.1f} ms/pass, {value:.4g} sites/s", file=sys.stderr,
+          flush=True)
+    d_secs, _, _ = timed_passes(job, args.steps, 1, per, 1)
+    print(f"[bench] batched, rows left in HBM: {d_secs * 1e3:.1f} ms/pass", file=sys.stderr, flush=True)
+
+    plain = None
+    if not args.no_plain:
+        p_secs, p_bytes, _ = timed_passes(job, args.plain_steps, 0, 0, 2)
+        assert p_bytes == text_bytes, (p_bytes, text_bytes)
+        plain = {"mode": "plain (per-window path, one synchronous copy of each window's rows)",
+                 "value": round(sites / p_secs, 1), "ms_per_step": round(p_secs * 1e3, 3), "steps": args.plain_steps,
+                 "batched_over_plain": round(p_secs / secs, 3)}
+        print(f"[bench] plain: {p_secs * 1e3:.1f} ms/pass", file=sys.stderr, flush=True)
+
+    # per-stage HIP-event times (eager launches, rows left in HBM)
+    stage_ms, launches = {}, {}
+    if not args.no_stages:
         job.profile(enable=True)
         for _ in range(args.steps):
-            job.call(batched=batched)
+            job.run_pass(batched=per, emit_text=1)
         prof = job.profile(enable=False)
-        recs = job.records(reset=True) // max(1, args.steps)
-        secs = (t_end - t_start) / args.steps
-        return secs, prof, recs, nbytes
+        stage_ms = {k: v[0] / args.steps for k, v in prof.items() if v[1]}
+        launches = {k: v[1] / args.steps for k, v in prof.items() if v[1]}
+    # roofline of the dominant stage: the fused sort + likelihood + posterior
+    # (4 B per record read once, 64 B of per-site counters and reference in,
+    # 16 B of calls out: 4n + 80 per site)
+    fused_bytes = 4 * recs + 80 * sites
+    roofline = None
+    if stage_ms.get("fused"):
+        t_f = stage_ms["fused"] / 1e3
+        achieved = fused_bytes / t_f / 1e9
+        traffic, src = measured_traffic(args.config, args.scale)
+        t_call_gbs = fused_bytes / secs / 1e9
+        roofline = {"bound": "hbm", "kernel": "fused stage (k_bucket + k_fused<S> per depth class)",
+                    "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "traffic_source": src,
+                    "bytes_per_pass": int(fused_bytes), "ms_per_pass": round(t_f * 1e3, 4),
+                    "launches_per_pass": launches.get("fused"),
+                    "t_call": {"achieved": round(t_call_gbs, 2), "frac": round(t_call_gbs / HBM_PEAK_GBS, 4),
+                               "note": "the same 4n + 80 bytes per site over the whole timed pass, rows to host"}}
 
-    secs, prof, recs, text_bytes = timed(args.mode)
-    sites_per_s = chr_len / secs
-    other = None
-    if args.compare:
-        o_mode = "plain" if args.mode == "batched" else "batched"
-        o_secs, _, _, _ = timed(o_mode)
-        other = {"mode": o_mode, "value": chr_len / o_secs, "ms_per_step": o_secs * 1e3}
-
-    # dominant stage and its algorithmic bytes per launch
-    steps = args.steps
-    stage_ms = {k: v[0] / steps for k, v in prof.items() if v[1]}
-    launches = {k: v[1] / steps for k, v in prof.items() if v[1]}
-    dom = max(stage_ms, key=stage_ms.get)
-    sites = chr_len
-    packed = chr_reads * rl  # one packed byte per read base
-    algo = {
-        # reads (packed bases + 8 B position/metadata) in, records + 60 B of counters per site out
-        "site_build": packed + 8 * chr_reads + 4 * recs + 60 * sites,
-        # records read and written back, record counts read
-        "sort": 8 * recs + 4 * sites,
-        # records and count in, 10 FP64 log-likelihoods out (4n + 84 per site)
-        "likelihood": 4 * recs + 84 * sites,
-        # counters, reference code and likelihoods in; calls out
-        "posterior": 121 * sites + 15 * sites,
-        "rank": 0,
-        # call columns and counters in, .cns text out
-        "format": 72 * sites + text_bytes,
-        "recycle": 0,
-        "fused": 4 * recs + 80 * sites,
-    }
-    a_bytes = algo.get(dom, 0) / max(1e-9, launches[dom])
-    t_launch = stage_ms[dom] / 1e3 / max(1e-9, launches[dom])
-    achieved = a_bytes / t_launch / 1e9
-    # traffic: measured HBM bytes per pass of the stage, from the committed
-    # counter runs over this same workload (default launch shape); counters
-    # cannot be read inside the timed run
-    traffic = measured_traffic(dom) if (args.config == 2 and args.scale == 1.0 and args.mode == "batched") else None
-    roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "traffic_source": TRAFFIC_PROFILE.relative_to(ROOT).as_posix() if traffic else None,
-                "bytes_per_launch": int(a_bytes), "ms_per_launch": round(t_launch * 1e3, 4)}
-    # what bounds the dominant stage instead: VALU issue (committed counters)
-    issue = measured_issue() if (dom == "fused" and args.config == 2 and args.scale == 1.0) else None
-    compute_bound = {"bound": "valu_issue", "valu_issue_frac": issue,
-                     "source": ISSUE_PROFILE.relative_to(ROOT).as_posix()} if issue else None
-    # the calling kernels together at B_site = 4n + 80 per called site
-    call_ms = sum(stage_ms.get(k, 0) for k in ("sort", "likelihood", "posterior", "rank"))
-    call_gbs = (4 * recs + 80 * sites) / (call_ms / 1e3) / 1e9 if call_ms else None
-
-    # -C leg: the same pass with each window's column chunks encoded on the
-    # device and copied to host memory (no file writes) instead of the text
     columns = None
     if not args.no_columns:
-        batched = per if args.mode == "batched" else 0
-        job.columns(batched=batched)
-        job.profile(enable=True)
-        t0 = time.perf_counter()
-        c_bytes = 0
-        for _ in range(args.steps):
-            c_bytes = job.columns(batched=batched)
-        c_secs = (time.perf_counter() - t0) / args.steps
-        c_prof = job.profile(enable=False)
-        enc_ms = c_prof["columns"][0] / args.steps
-        columns = {"metric": "called sites/sec with -C column chunks", "value": round(chr_len / c_secs, 1),
-                   "unit": "sites/s", "ms_per_step": round(c_secs * 1e3, 3), "encoder_ms_per_step": round(enc_ms, 4),
-                   "chunk_bytes": c_bytes,
-                   "note": "encoder = the device kernels of the chunks; the step also copies them to host memory"}
+        c_secs, c_bytes, _ = timed_passes(job, args.steps, 1, per, 3)
+        columns = {"metric": "called sites/sec with -C column chunks in host memory", "value": round(sites / c_secs, 1),
+                   "unit": "sites/s", "ms_per_step": round(c_secs * 1e3, 3), "chunk_bytes": c_bytes}
+        print(f"[bench] -C: {c_secs * 1e3:.1f} ms/pass", file=sys.stderr, flush=True)
 
-    cpu = None
-    if not args.no_cpu:
-        sys.path.insert(0, str(ROOT / "tests"))
-        import oracle_lib
-
-        oracle_lib.ensure_built()
-        out = work / "cpu.cns"
-        cmd = [str(oracle_lib.ORACLE_CLI), "-i", so, "-d", fa, "-o", str(out), "-L", str(rl), "-X", str(args.cpu_sites),
-               "-R"]
-        if snp:
-            cmd += ["-s", snp, "-2"]
-        res = subprocess.run(cmd, capture_output=True, text=True, check=True)
-        st = json.loads(res.stdout.strip().splitlines()[-1])
-        cpu = {"value": round(st["sites"] / st["seconds_call"], 1), "unit": "sites/s", "cores": 1, "kind": "port",
-               "sample": f"first {st['sites']} sites ({st['windows']} windows) of the same workload, "
-                         f"single-threaded CPU oracle, call stage only"}
-
+    cpu = None if args.no_cpu else cpu_leg(args, work, lens, sites)
+    job.close()
     adjoint = None if args.no_adjoint else adjoint_leg(args, work)
 
     line = {
         "metric": "called sites/sec at bit-exact .cns",
-        "value": round(sites_per_s, 1),
+        "value": round(value, 1),
         "unit": "sites/s",
         "n_gpus": 1,
         "steps": args.steps,
@@ -245,39 +283,40 @@ def main() -> int:
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic (seeded generator, config %d)" % args.config,
-        "config": {"workload": "10 Mb reference, 30x 100 bp pairs, 3% 2-hit, 1% known SNPs, refine on"
-                   if args.config == 2 else f"config {args.config} x{args.scale}",
-                   "sites": chr_len, "reads": chr_reads, "records": recs, "mode": args.mode,
-                   "windows_per_span": per if args.mode == "batched" else 1, "text_bytes": text_bytes,
+        "data": f"synthetic (seeded generator, config {args.config}, seed {args.seed}, scale {args.scale}), in memory",
+        "config": {"workload": WORKLOADS.get(args.config, f"config {args.config}") +
+                   ("" if args.scale == 1.0 else f", scaled x{args.scale}"),
+                   "contigs": len(contigs), "sites": sites, "reads": reads, "records": recs,
+                   "mode": "batched", "windows_per_span": per, "text_bytes": text_bytes,
+                   "timed_region": "reads resident in HBM -> .cns rows in pinned host memory", "flags": flags,
                    "autotuned": tuned},
+        "device_only": {"value": round(sites / d_secs, 1), "ms_per_step": round(d_secs * 1e3, 3),
+                        "note": "the same pass with the rows left in HBM"},
+        "other_mode": plain,
         "roofline": roofline,
-        "compute_bound": compute_bound,
-        "call_kernels_gbs": round(call_gbs, 2) if call_gbs else None,
         "stage_ms_per_step": {k: round(v, 4) for k, v in stage_ms.items()},
+        "rank_sum": fixups,
         "cpu_baseline": cpu,
     }
-    if other:
-        line["other_mode"] = other
     if columns:
         line["columns"] = columns
     if adjoint:
         line["adjoint"] = adjoint
     print(json.dumps(line), flush=True)
-    job.close()
     return 0
 
 
 def adjoint_leg(args, work: Path) -> dict:
     """Gradient evaluations per second of the adjoint on the 2 Mb miscalibrated
     workload (30x, 100 bp), its central-difference check on random table
-    entries, and the refit judged by the plain calling path."""
+    entries, and the refit judged by the plain calling path and against the
+    miscalibration the generator used."""
     import math
 
     import numpy as np
 
     import gsnp_amd
-    from gsnp_amd.adjoint import Adjoint, calls_changed, central_differences, refit, write_matrix
+    from gsnp_amd.adjoint import Adjoint, calls_changed, central_differences, refit, truth_distance, write_matrix
 
     pre = work / "c4"
     t0 = time.time()
@@ -321,10 +360,27 @@ def adjoint_leg(args, work: Path) -> dict:
             gsnp_amd.run(fa, so, c1, matrix_in=m1, read_length=rl)
             ng, nq = calls_changed(c0, c1)
             fit = {"steps": args.refit_steps, "objective_start": trace[0], "objective_end": trace[-1],
-                   "genotype_calls_changed_vs_plain": ng, "consensus_quality_changed_vs_plain": nq}
+                   "genotype_calls_changed_vs_plain": ng, "consensus_quality_changed_vs_plain": nq,
+                   "distance_to_generator_truth": {"counting_table": truth_distance(p0, rl),
+                                                   "refit_table": truth_distance(p1, rl)},
+                   "truth_metric": "RMS over q 10-40 and every cycle of log10(row error rate) - log10 of the "
+                                   "generator's 10^(-(q - delta)/10)"}
         recs, sites = adj.n_records, adj.n_sites
     finally:
         job.close()
+    if fit is not None:
+        # the same refit where the calling model matches the generator (no
+        # dependency adjustment: -p 1 -g 1, independent errors)
+        job = gsnp_amd.Job(fa, so, read_length=rl, pcr_dependency=1.0, global_dependency=1.0)
+        try:
+            adj = Adjoint(job)
+            p1i, trace_i = refit(adj, p0, math.log(job._p.het_novel_r), steps=args.refit_steps)
+            fit["independent_errors_model"] = {
+                "objective_start": trace_i[0], "objective_end": trace_i[-1],
+                "distance_to_generator_truth": {"counting_table": truth_distance(p0, rl),
+                                                "refit_table": truth_distance(p1i, rl)}}
+        finally:
+            job.close()
     # algorithmic bytes per evaluation: record keys read twice, four keyed FP64
     # contributions written and read back plus a rank, 80 B per site, 2 MB gradient
     algo = 2 * 4 * recs + 48 * recs + 80 * sites + 64 * 256 * 16 * 8

@@ -119,6 +119,7 @@ SIGNATURES = {
     "gsnp_format_g": (ctypes.c_int, [ctypes.c_double, ctypes.c_char_p]),
     "gsnp_profile": (ctypes.c_int, [_P, ctypes.c_int, _P, _P, ctypes.c_int]),
     "gsnp_stats": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_uint64), ctypes.c_int]),
+    "gsnp_counters": (ctypes.c_int, [_P, _P, ctypes.c_int, ctypes.c_int]),
     "gsnp_job_open": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p,
                                      ctypes.POINTER(Params), ctypes.POINTER(_P)]),
     "gsnp_job_count": (ctypes.c_int, [_P]),
@@ -127,6 +128,13 @@ SIGNATURES = {
     "gsnp_job_select": (ctypes.c_int, [_P, ctypes.c_int]),
     "gsnp_job_call": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_uint64)]),
     "gsnp_job_columns": (ctypes.c_int, [_P, ctypes.c_int, ctypes.POINTER(ctypes.c_uint64)]),
+    "gsnp_job_write": (ctypes.c_int, [_P, ctypes.c_char_p, ctypes.c_int, ctypes.c_int]),
+    "gsnp_job_set_regions": (ctypes.c_int, [_P, ctypes.c_char_p]),
+    "gsnp_job_write_matrix": (ctypes.c_int, [_P, ctypes.c_char_p]),
+    "gsnp_job_pass": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_uint64),
+                                     ctypes.POINTER(ctypes.c_uint64)]),
+    "gsnp_job_synth": (ctypes.c_int, [ctypes.c_int, ctypes.c_ulonglong, ctypes.c_double, ctypes.POINTER(Params),
+                                      ctypes.c_char_p, ctypes.POINTER(_P), ctypes.POINTER(ctypes.c_int)]),
     "gsnp_set_matrix": (ctypes.c_int, [_P, _P]),
     "gsnp_cns_open": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(_P)]),
     "gsnp_cns_next": (ctypes.c_int, [_P, _P, ctypes.POINTER(ctypes.c_int)]),
@@ -136,6 +144,9 @@ SIGNATURES = {
     "gsnp_job_close": (None, [_P]),
     "gsnp_synth_write": (ctypes.c_longlong, [ctypes.c_int, ctypes.c_ulonglong, ctypes.c_double, ctypes.c_char_p,
                                              ctypes.POINTER(ctypes.c_int)]),
+    "gsnp_synth_write_contigs": (ctypes.c_longlong, [ctypes.c_int, ctypes.c_ulonglong, ctypes.c_double,
+                                                     ctypes.c_char_p, _P, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
+    "gsnp_synth_contigs": (ctypes.c_int, [ctypes.c_int, ctypes.c_ulonglong, ctypes.c_double, _P, ctypes.c_int]),
     "gsnp_autotune": (ctypes.c_int, [_P, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, _P]),
     "gsnp_adjoint_prepare": (ctypes.c_int, [_P, ctypes.c_uint32, ctypes.c_uint32]),
     "gsnp_adjoint_eval": (ctypes.c_int, [_P, _P, ctypes.c_double, ctypes.POINTER(ctypes.c_double), _P,
@@ -250,6 +261,24 @@ def synth(config: int, seed: int, scale: float, prefix) -> tuple[int, int]:
     return int(n), rl.value
 
 
+def synth_contigs(config: int, seed: int, scale: float) -> list[int]:
+    """Contig lengths of a configuration's data set."""
+    n = lib().gsnp_synth_contigs(config, seed, scale, None, 0)
+    arr = (ctypes.c_uint64 * max(1, n))()
+    lib().gsnp_synth_contigs(config, seed, scale, arr, n)
+    return [int(arr[i]) for i in range(n)]
+
+
+def synth_subset(config: int, seed: int, scale: float, contigs, prefix) -> tuple[int, int]:
+    """The files of the listed contigs only; returns (alignment lines, read length)."""
+    rl = ctypes.c_int()
+    idx = (ctypes.c_int * max(1, len(contigs)))(*contigs)
+    n = lib().gsnp_synth_write_contigs(config, seed, scale, _b(prefix), idx, len(contigs), ctypes.byref(rl))
+    if n < 0:
+        raise GsnpError(1, f"cannot write {prefix}.*")
+    return int(n), rl.value
+
+
 def matrix_gen(fasta, soap, dbsnp=None, **kw):
     """Counting-trained calibration matrix (gsnp_matrix_gen), 256*256*16 doubles
     indexed q<<12 | cycle<<4 | ref<<2 | obs, as a numpy array."""
@@ -267,19 +296,32 @@ def format_g(v: float) -> str:
     return buf.raw[:n].decode()
 
 
-STAGES = ["site_build", "sort", "likelihood", "posterior", "rank", "format", "recycle", "fused", "adjoint", "columns"]
+STAGES = ["site_build", "sort", "likelihood", "posterior", "rank", "format", "recycle", "fused", "adjoint", "columns",
+          "calib"]
 
 
 class Job:
     """A calling session over input files (gsnp_job_*): parse and calibrate
     once, then repeat calling passes on device-resident inputs."""
 
-    def __init__(self, fasta, soap, dbsnp=None, matrix_in=None, **kw):
+    def __init__(self, fasta=None, soap=None, dbsnp=None, matrix_in=None, synth=None, **kw):
+        """From files, or with synth=(config, seed, scale) from a seeded data
+        set generated in memory (gsnp_job_synth)."""
         L = lib()
         check(L.gsnp_device_init(0))
+        self.h = None
         self._p = params(**kw)
         h = ctypes.c_void_p()
-        check(L.gsnp_job_open(_b(fasta), _b(soap), _b(dbsnp), _b(matrix_in), ctypes.byref(self._p), ctypes.byref(h)))
+        if synth is not None:
+            rl = ctypes.c_int()
+            cfg, seed, scale = synth
+            check(L.gsnp_job_synth(int(cfg), int(seed), float(scale), ctypes.byref(self._p), _b(matrix_in),
+                                   ctypes.byref(h), ctypes.byref(rl)))
+            self.read_len = rl.value
+        else:
+            check(L.gsnp_job_open(_b(fasta), _b(soap), _b(dbsnp), _b(matrix_in), ctypes.byref(self._p),
+                                  ctypes.byref(h)))
+            self.read_len = self._p.read_length
         self.h = h
 
     def __len__(self) -> int:
@@ -296,10 +338,27 @@ class Job:
         check(lib().gsnp_job_select(self.h, i))
         self._length = self.info(i)[1]
 
-    def call(self, batched: int = 0, emit_text: bool = True) -> int:
+    def call(self, batched: int = 0, emit_text: int = 1) -> int:
+        """A calling pass over the selected chromosome; emit_text 0 no rows,
+        1 rows in device memory, 2 rows copied to host memory.  Returns the row bytes."""
         nb = ctypes.c_uint64()
-        check(lib().gsnp_job_call(self.h, int(batched), int(bool(emit_text)), ctypes.byref(nb)))
+        check(lib().gsnp_job_call(self.h, int(batched), int(emit_text), ctypes.byref(nb)))
         return nb.value
+
+    def set_regions(self, path) -> None:
+        """-T target regions for later calls and writes (gsnp_job_set_regions); None clears them."""
+        check(lib().gsnp_job_set_regions(self.h, _b(path)))
+
+    def write(self, out, compress: bool = False, batched: int = 0) -> None:
+        """Every chromosome called and written as .cns text or -C files (gsnp_job_write)."""
+        check(lib().gsnp_job_write(self.h, _b(out), int(bool(compress)), int(batched)))
+
+    def run_pass(self, batched: int = -1, emit_text: int = 2) -> tuple[int, int]:
+        """A calling pass over every chromosome (gsnp_job_pass); batched < 0:
+        one span per chromosome, 0: the per-window path.  Returns (row bytes, sites)."""
+        nb, ns = ctypes.c_uint64(), ctypes.c_uint64()
+        check(lib().gsnp_job_pass(self.h, int(batched), int(emit_text), ctypes.byref(nb), ctypes.byref(ns)))
+        return nb.value, ns.value
 
     def columns(self, batched: int = 0) -> int:
         """A calling pass whose -C chunks are encoded on the device and copied to
@@ -327,6 +386,12 @@ class Job:
         check(lib().gsnp_autotune(self.ctx, start, n_sites, reps, cfg))
         return {"smallest_network": 8 << cfg[0], "waves_per_block": list(cfg[1:7]), "grid_cap": list(cfg[7:13])}
 
+    def counters(self, reset: bool = True) -> dict:
+        """Records called and sites finished on the host since the last reset (gsnp_counters)."""
+        v = (ctypes.c_uint64 * 3)()
+        check(lib().gsnp_counters(self.ctx, v, 3, int(reset)))
+        return {"records": v[0], "host_fixups": v[1], "host_fixups_normal": v[2]}
+
     def records(self, reset: bool = True) -> int:
         v = ctypes.c_uint64()
         check(lib().gsnp_stats(self.ctx, ctypes.byref(v), int(reset)))
@@ -338,8 +403,7 @@ class Job:
             self.h = None
 
     def __del__(self):
-        try:
-            self.close()
-        except Exception:
-            pass
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.gsnp_job_close(self.h)
+            self.h = None
 

@@ -193,6 +193,22 @@ def refit(adj: Adjoint, p0: np.ndarray, log_het: float, steps: int = 50, lr: flo
     return out.reshape(-1), trace
 
 
+def truth_distance(p_matrix: np.ndarray, read_length: int, q_lo: int = 10, q_hi: int = 40) -> float:
+    """Distance of a table from the error rates the miscalibrated workload was
+    generated with: the RMS over rows (q, cycle), q_lo <= q <= q_hi and cycle
+    < read_length, of log10 of the row's error rate (1 - p[q][cycle][t][t],
+    averaged over the four reference bases t) minus log10 of the true
+    10^(-(q - delta)/10), delta = 3 [q > 30] + 2 cycle / 99."""
+    pm = np.asarray(p_matrix, dtype=np.float64).reshape(PM_SHAPE)
+    d = []
+    for q in range(q_lo, q_hi + 1):
+        for c in range(read_length):
+            err = float(np.mean([1.0 - pm[q, c, t, t] for t in range(4)]))
+            true = 10.0 ** (-(q - (3.0 * (q > 30) + 2.0 * c / 99.0)) / 10.0)
+            d.append(math.log10(max(err, 1e-300)) - math.log10(true))
+    return float(np.sqrt(np.mean(np.square(d))))
+
+
 def calls_changed(cns_a, cns_b) -> tuple[int, int]:
     """(rows whose genotype call differs, rows whose consensus quality differs)
     between two .cns files of the same sites."""

@@ -3,7 +3,11 @@
 
 #include <hip/hip_runtime.h>
 
+#include <exception>
+#include <functional>
 #include <memory>
+#include <utility>
+#include <new>
 #include <string>
 #include <vector>
 
@@ -13,6 +17,19 @@
 namespace gsnp {
 
 int fail(int code, const std::string& msg);
+// Runs an entry point's body; a C++ exception never crosses the C ABI: host
+// allocation failures become GSNP_E_NOMEM, anything else GSNP_E_ARG.
+template <class F>
+int guarded(F body)
+{
+    try {
+        return body();
+    } catch (const std::bad_alloc&) {
+        return fail(GSNP_E_NOMEM, "out of host memory");
+    } catch (const std::exception& e) {
+        return fail(GSNP_E_ARG, e.what());
+    }
+}
 struct Ctx;
 // calibration bins: (q - q_min) << 12 | cycle << 4 | ref << 2 | obs
 constexpr uint32_t kCalibBins = 64u << 12;
@@ -25,7 +42,28 @@ struct DevBuf {
     DevBuf() = default;
     DevBuf(const DevBuf&) = delete;
     DevBuf& operator=(const DevBuf&) = delete;
+    DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n)
+    {
+        o.p = nullptr;
+        o.n = 0;
+    }
+    DevBuf& operator=(DevBuf&& o) noexcept
+    {
+        if (this != &o) {
+            release();
+            p = o.p;
+            n = o.n;
+            o.p = nullptr;
+            o.n = 0;
+        }
+        return *this;
+    }
     ~DevBuf() { release(); }
+    void swap(DevBuf& o)
+    {
+        std::swap(p, o.p);
+        std::swap(n, o.n);
+    }
     hipError_t alloc(size_t count)
     {
         release();
@@ -69,6 +107,7 @@ enum Stage : int {
     kStFused,
     kStAdjoint,
     kStColumns,
+    kStCalib,
     kNumStages
 };
 
@@ -140,6 +179,57 @@ struct ColBufs {
     }
 };
 
+// The device inputs of one chromosome: reference codes, known SNPs and the
+// packed read store.  The context holds the selected chromosome's in its own
+// fields; a job that keeps every chromosome resident parks the others in
+// ChromStates and swaps them in (Ctx::swap_chrom) without an upload.
+struct ChromState {
+    uint64_t chrom_id = 0;
+    std::string chr_name;
+    uint32_t chr_len = 0;
+    DevBuf<uint8_t> ref;
+    DevBuf<uint32_t> snp_pos, snp_first;
+    DevBuf<double> snp_lp;
+    uint32_t n_snp = 0;
+    DevBuf<char> d_name;
+    uint64_t n_reads = 0;
+    uint32_t pitch = 0;
+    DevBuf<int32_t> rpos;
+    DevBuf<uint32_t> rmeta;
+    DevBuf<uint8_t> rseq;
+    std::vector<int32_t> h_rpos;
+    std::vector<gsnp_snp> h_snps;
+    std::vector<uint8_t> h_snp_ref;
+};
+
+// .cns rows on their way to host memory: the rows of a span are formatted
+// into one of two device buffers while the copy of the previous span's rows
+// to pinned host memory runs on a stream of its own.
+struct TextPipe {
+    DevBuf<char> dev[2];
+    char* host[2] = {nullptr, nullptr};
+    uint64_t host_cap[2] = {0, 0};
+    uint64_t len[2] = {0, 0};
+    bool copying[2] = {false, false};  // a copy into host[i] is queued and not yet handed on
+    bool recorded[2] = {false, false};  // ev_done[i] has been recorded
+    int cur = 0;                        // buffer the next span is formatted into
+    hipStream_t copy = nullptr;
+    hipEvent_t ev_done[2] = {nullptr, nullptr};  // [i]: host[i] holds its span's rows
+    hipEvent_t ev_fmt = nullptr;
+    TextPipe() = default;
+    TextPipe(const TextPipe&) = delete;
+    TextPipe& operator=(const TextPipe&) = delete;
+    ~TextPipe()
+    {
+        if (copy) (void)hipStreamSynchronize(copy);
+        for (char* h : host)
+            if (h) (void)hipHostFree(h);
+        for (hipEvent_t e : {ev_done[0], ev_done[1], ev_fmt})
+            if (e) (void)hipEventDestroy(e);
+        if (copy) (void)hipStreamDestroy(copy);
+    }
+};
+
 struct Ctx {
     gsnp_params p{};
     hipStream_t stream = nullptr;
@@ -149,7 +239,9 @@ struct Ctx {
     DevBuf<uint8_t> qadj;
     int td = 1, gd = 1;
     std::vector<double> h_prank, h_pprior;
-    // chromosome
+    // chromosome (the fields of ChromState; chrom_id names the uploaded
+    // inputs in the keys of captured graphs)
+    uint64_t chrom_id = 0;
     std::string chr_name;
     uint32_t chr_len = 0;
     DevBuf<uint8_t> ref;
@@ -180,6 +272,9 @@ struct Ctx {
     DevBuf<uint32_t> fix_count;
     uint32_t fix_offset = 0;
     uint32_t last_fixups = 0;
+    // sites finished on the host since the last reset: every one, and those
+    // whose rank-sum test took the normal approximation (n1 + n2 >= 64)
+    uint64_t fixups_total = 0, fixups_normal = 0;
     // text
     DevBuf<uint32_t> rowlen;
     DevBuf<uint64_t> blocksum, total;
@@ -195,8 +290,9 @@ struct Ctx {
     uint64_t buf_gen = 0, graphs_gen = 0;
     std::vector<std::unique_ptr<GraphSlot>> graphs;  // captured spans, at most kMaxGraphs
     GraphSlot& graph_slot(const std::vector<uint64_t>& key);
-    DevBuf<char> text;
+    DevBuf<char> text;  // rows of the per-window path and of spans formatted for gsnp_fetch_text
     uint64_t text_cap = 0, text_len = 0;
+    TextPipe pipe;      // rows of batched spans copied to host memory (gsnp_job_call, gsnp_run)
     DevBuf<uint32_t> err;
     DevBuf<unsigned long long> stat;  // [0] records called
     DevBuf<int4> head_qs;  // q_sum of window-boundary sites before their own reads (batched span)
@@ -219,6 +315,19 @@ struct Ctx {
     DevBuf<unsigned long long> calib;  // calibration sub-histograms
 
     int set_matrix(const double* p_matrix);
+    void swap_chrom(ChromState& s);
+    // rows of the span just called (gsnp_batched_call without text) formatted
+    // into the pipe's device buffer and their copy to pinned host memory queued
+    // on the copy stream; then the rows of the previous span, landed in host
+    // memory, are handed to consume(ptr, len).  finish_text hands on the last.
+    using TextSink = std::function<int(const char*, uint64_t)>;
+    int text_to_host(uint32_t start, uint32_t n, uint64_t* nbytes, const TextSink& consume);
+    int finish_text(const TextSink& consume);
+    // drops rows still in the pipe (a pass that ended early) and waits for its copies
+    int reset_text();
+    // the rows of the last gsnp_window_text copied to pinned host memory and
+    // handed to consume, synchronously (the per-window path)
+    int window_text_to_host(const TextSink& consume);
     int alloc_window(uint32_t n_sites);
     SiteArrays site_view(uint32_t offset);
     Results result_view(uint32_t offset);
@@ -228,7 +337,7 @@ struct Ctx {
     int host_fixups();
     int check_err();
     int format_text(uint32_t offset, uint32_t ws_abs, uint32_t n_sites, uint32_t pos_base, uint64_t* nbytes,
-                    bool graph = false);
+                    bool graph = false, char* out = nullptr);
     int span_build(uint32_t start, uint32_t n);
     int span_prepare(uint32_t n);
     void span_enqueue(uint32_t start, uint32_t n);

@@ -2,6 +2,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -246,8 +247,10 @@ int Ctx::host_fixups()
     HIPCHK(hipStreamSynchronize(stream));
     std::vector<double> r(nfix);
     std::vector<int32_t> q(nfix);
+    fixups_total += nfix;
     for (uint32_t i = 0; i < nfix; i++) {
         const FixJob& f = fx[i];
+        fixups_normal += f.normal ? 1 : 0;
         double rk = f.normal ? host_normal_test(f.n1, f.n2, f.T1, f.T2)
                              : host_table_test(h_prank.data(), f.n1, f.n2, f.T1, f.T2);
         int qc = rk == 0.0 ? 0 : trunc_x86(10 * f.dtp + 10 * std::log10(rk));
@@ -280,6 +283,35 @@ int Ctx::check_err()
     HIPCHK(hipStreamSynchronize(stream));
     if (e & kErrQadjRange) return fail(GSNP_E_RANGE, "adjusted quality exceeds the score table");
     return GSNP_OK;
+}
+
+void Ctx::swap_chrom(ChromState& s)
+{
+    std::swap(chrom_id, s.chrom_id);
+    std::swap(chr_name, s.chr_name);
+    std::swap(chr_len, s.chr_len);
+    ref.swap(s.ref);
+    snp_pos.swap(s.snp_pos);
+    snp_first.swap(s.snp_first);
+    snp_lp.swap(s.snp_lp);
+    std::swap(n_snp, s.n_snp);
+    d_name.swap(s.d_name);
+    std::swap(n_reads, s.n_reads);
+    std::swap(pitch, s.pitch);
+    rpos.swap(s.rpos);
+    rmeta.swap(s.rmeta);
+    rseq.swap(s.rseq);
+    h_rpos.swap(s.h_rpos);
+    h_snps.swap(s.h_snps);
+    h_snp_ref.swap(s.h_snp_ref);
+    built = false;
+    adj_n = 0;  // a prepared adjoint span belongs to the chromosome it was built on
+}
+
+static uint64_t next_chrom_id()
+{
+    static std::atomic<uint64_t> id{0};
+    return ++id;
 }
 
 // The score table, from a calibration matrix (256*256*16 doubles); every other
@@ -391,25 +423,31 @@ int gsnp_set_chromosome(gsnp_ctx* c, const char* name, const uint8_t* codes, uin
                         const gsnp_snp* snps, uint32_t n_snp)
 {
     if (!c || !name || (!codes && len)) return fail(GSNP_E_ARG, "null argument");
+    if (n_snp && (!snp_pos || !snps)) return fail(GSNP_E_ARG, "null known-SNP arrays");
+    // everything is checked before the context changes: a rejected call leaves
+    // the previous chromosome in place
+    for (uint32_t i = 0; i < n_snp; i++)
+        if (snp_pos[i] >= len || (i && snp_pos[i] <= snp_pos[i - 1]))
+            return fail(GSNP_E_ARG, "known-SNP positions must be sorted, unique and inside the chromosome");
     HIPCHK(hipStreamSynchronize(c->stream));
-    c->chr_name = name;
-    c->chr_len = len;
+    // until every upload below has succeeded the context holds no chromosome
+    // (length 0, no known SNPs, no reads): a failure leaves nothing stale to call
+    c->chr_len = 0;
+    c->n_snp = 0;
+    c->n_reads = 0;
     c->buf_gen++;
+    c->chrom_id = next_chrom_id();
+    c->chr_name = name;
     HIPCHK(c->ref.alloc((size_t)len + GSNP_WIN + 512));
     HIPCHK(hipMemsetAsync(c->ref.p, 4, (size_t)len + GSNP_WIN + 512, c->stream));  // beyond the end reads as N
     if (len) HIPCHK(hipMemcpyAsync(c->ref.p, codes, len, hipMemcpyHostToDevice, c->stream));
-    if (n_snp && (!snp_pos || !snps)) return fail(GSNP_E_ARG, "null known-SNP arrays");
-    c->n_snp = n_snp;
     c->h_snps.assign(snps, snps + n_snp);
     c->h_snp_ref.resize(n_snp);
     for (uint32_t i = 0; i < n_snp; i++) c->h_snp_ref[i] = snp_pos[i] < len ? codes[snp_pos[i]] : 0;
     if (n_snp) {
         std::vector<double> lp((size_t)n_snp * 10);
-        for (uint32_t i = 0; i < n_snp; i++) {
-            if (snp_pos[i] >= len || (i && snp_pos[i] <= snp_pos[i - 1]))
-                return fail(GSNP_E_ARG, "known-SNP positions must be sorted, unique and inside the chromosome");
+        for (uint32_t i = 0; i < n_snp; i++)
             snp_log_prior(c->p, c->h_pprior.data(), codes[snp_pos[i]], snps[i], &lp[(size_t)i * 10]);
-        }
         HIPCHK(c->snp_pos.alloc(n_snp));
         HIPCHK(c->snp_lp.alloc(lp.size()));
         HIPCHK(hipMemcpyAsync(c->snp_pos.p, snp_pos, 4ull * n_snp, hipMemcpyHostToDevice, c->stream));
@@ -429,6 +467,8 @@ int gsnp_set_chromosome(gsnp_ctx* c, const char* name, const uint8_t* codes, uin
     HIPCHK(c->d_name.alloc(nl + 1));
     HIPCHK(hipMemcpyAsync(c->d_name.p, name, nl + 1, hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
+    c->chr_len = len;
+    c->n_snp = n_snp;
     return GSNP_OK;
 }
 
@@ -521,9 +561,10 @@ int gsnp_load_reads(gsnp_ctx* c, const gsnp_read_batch* b)
             if (e.code) return fail(e.code, e.msg);
     }
     HIPCHK(hipStreamSynchronize(c->stream));
-    c->n_reads = b->n;
+    c->n_reads = 0;  // set once the uploads below have succeeded
     c->pitch = L;
     c->buf_gen++;
+    c->chrom_id = next_chrom_id();
     c->h_rpos.assign(b->pos, b->pos + b->n);
     HIPCHK(c->rpos.alloc(b->n + 1));
     HIPCHK(c->rmeta.alloc(b->n + 1));
@@ -535,12 +576,14 @@ int gsnp_load_reads(gsnp_ctx* c, const gsnp_read_batch* b)
         HIPCHK(hipMemcpyAsync(c->rseq.p, seq.get(), nseq, hipMemcpyHostToDevice, c->stream));
     }
     HIPCHK(hipStreamSynchronize(c->stream));
+    c->n_reads = b->n;
     return GSNP_OK;
 }
 
 int gsnp_window_init(gsnp_ctx* c, uint32_t start)
 {
     if (!c) return fail(GSNP_E_ARG, "null context");
+    if (!c->ref.p || !c->d_name.p) return fail(GSNP_E_ARG, "no chromosome set");
     const uint32_t L = (uint32_t)c->p.read_length;
     int rc = c->alloc_window(GSNP_WIN + L + 1);
     if (rc) return rc;
@@ -686,8 +729,8 @@ int gsnp_batched_call(gsnp_ctx* c, uint32_t start, uint32_t n, int emit_text, ui
     int rc = c->span_prepare(n);
     if (rc) return rc;
     const FusedTune& t = c->tune;
-    std::vector<uint64_t> key = {c->buf_gen, start, n, (uint64_t)c->p.rank_sum_mode, (uint64_t)c->keep_lk,
-                                 t.min_class};
+    std::vector<uint64_t> key = {c->buf_gen, c->chrom_id, start, n, (uint64_t)c->p.rank_sum_mode,
+                                 (uint64_t)c->keep_lk, t.min_class};
     for (int i = 0; i < 6; i++) key.push_back(((uint64_t)t.wpb[i] << 32) | t.max_blocks[i]);
     rc = c->launch_captured(c->graph_slot(key), key, [&] { c->span_enqueue(start, n); });
     if (rc) return rc;
@@ -825,6 +868,22 @@ int gsnp_stats(gsnp_ctx* c, uint64_t* n_records, int reset)
     return GSNP_OK;
 }
 
+int gsnp_counters(gsnp_ctx* c, uint64_t* out, int n, int reset)
+{
+    if (!c || (n > 0 && !out)) return fail(GSNP_E_ARG, "null argument");
+    unsigned long long rec = 0;
+    HIPCHK(hipMemcpyAsync(&rec, c->stat.p, 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    const uint64_t v[3] = {rec, c->fixups_total, c->fixups_normal};
+    for (int i = 0; i < n && i < 3; i++) out[i] = v[i];
+    if (reset) {
+        HIPCHK(hipMemsetAsync(c->stat.p, 0, 8, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        c->fixups_total = c->fixups_normal = 0;
+    }
+    return GSNP_OK;
+}
+
 int gsnp_format_g(double v, char* out)
 {
     int n = fmt_g(v, out);
@@ -910,7 +969,7 @@ int Ctx::launch_captured(GraphSlot& slot, const std::vector<uint64_t>& key, F en
     return GSNP_OK;
 }
 
-constexpr size_t kMaxGraphs = 64;
+constexpr size_t kMaxGraphs = 256;
 
 // The cached graph for `key`, or an empty slot for it.  Graphs captured
 // before the last buffer (re)allocation are dropped: their pointers are gone.
@@ -958,17 +1017,23 @@ FusedArgs Ctx::fused_args(uint32_t start)
     return f;
 }
 
+// Bytes of the rows of n sites at most: name, 16 columns and tabs.
+static uint64_t text_bound(uint64_t n_sites, size_t name_len) { return n_sites * (name_len + 160) + 64; }
+
+// Formats the rows of sites [offset, offset + n_sites) into `out` (sized by
+// text_bound), or into the context's text buffer when out is null.
 int Ctx::format_text(uint32_t offset, uint32_t ws_abs, uint32_t n_sites, uint32_t pos_base, uint64_t* nbytes,
-                     bool graph)
+                     bool graph, char* out)
 {
     const size_t name_len = chr_name.size();
-    const uint64_t need = (uint64_t)n_sites * (name_len + 160) + 64;
-    if (need > text_cap) {
+    const uint64_t need = text_bound(n_sites, name_len);
+    if (!out && need > text_cap) {
         HIPCHK(hipStreamSynchronize(stream));
         HIPCHK(text.alloc(need));
         text_cap = need;
         buf_gen++;
     }
+    char* dst = out ? out : text.p;
     if (gcount.n < 1) HIPCHK(gcount.alloc(1));
     RowCtx rc;
     rc.name = d_name.p;
@@ -986,11 +1051,12 @@ int Ctx::format_text(uint32_t offset, uint32_t ws_abs, uint32_t n_sites, uint32_
                       blocksum.p);
         launch_gtext(stream, a, res, n_sites, gtext.p, gcount.p, rowlen.p, blocksum.p);
         launch_scan_blocks(stream, blocksum.p, nb, total.p);
-        launch_format(stream, ref_view(), ws_abs, a, res, n_sites, rc, rowlen.p, blocksum.p, text.p);
+        launch_format(stream, ref_view(), ws_abs, a, res, n_sites, rc, rowlen.p, blocksum.p, dst);
         timer.end(kStFormat, stream);
     };
     if (graph) {
-        const std::vector<uint64_t> key = {buf_gen, 1u, offset, ws_abs, n_sites, pos_base};
+        const std::vector<uint64_t> key = {buf_gen, chrom_id, 1u, offset, ws_abs, n_sites, pos_base,
+                                           (uint64_t)(uintptr_t)dst};
         int r = launch_captured(graph_slot(key), key, enqueue);
         if (r) return r;
     } else {
@@ -1000,9 +1066,98 @@ int Ctx::format_text(uint32_t offset, uint32_t ws_abs, uint32_t n_sites, uint32_
     uint64_t t = 0;
     HIPCHK(hipMemcpyAsync(&t, total.p, 8, hipMemcpyDeviceToHost, stream));
     HIPCHK(hipStreamSynchronize(stream));
+    if (out) {
+        if (nbytes) *nbytes = n_sites ? t : 0;
+        return GSNP_OK;
+    }
     text_len = n_sites ? t : 0;
     if (nbytes) *nbytes = text_len;
     return GSNP_OK;
+}
+
+int Ctx::text_to_host(uint32_t start, uint32_t n, uint64_t* nbytes, const TextSink& consume)
+{
+    TextPipe& tp = pipe;
+    if (!tp.copy) {
+        HIPCHK(hipStreamCreateWithFlags(&tp.copy, hipStreamNonBlocking));
+        for (hipEvent_t* e : {&tp.ev_done[0], &tp.ev_done[1], &tp.ev_fmt})
+            HIPCHK(hipEventCreateWithFlags(e, hipEventDisableTiming));
+    }
+    const int i = tp.cur, j = i ^ 1;
+    const uint64_t need = text_bound(n, chr_name.size());
+    if (tp.dev[i].n < need) {
+        // nothing may still read the old buffer: its copy, or a graph that writes it
+        HIPCHK(hipStreamSynchronize(tp.copy));
+        HIPCHK(hipStreamSynchronize(stream));
+        HIPCHK(tp.dev[i].alloc(need + need / 8));
+        buf_gen++;
+    }
+    // the copy of the span formatted into this buffer two spans ago reads it
+    if (tp.recorded[i]) HIPCHK(hipStreamWaitEvent(stream, tp.ev_done[i], 0));
+    uint64_t len = 0;
+    int rc = format_text(0, start, n, start + 1, &len, true, tp.dev[i].p);
+    if (rc) return rc;
+    if (tp.host_cap[i] < len) {
+        if (tp.recorded[i]) HIPCHK(hipEventSynchronize(tp.ev_done[i]));
+        if (tp.host[i]) HIPCHK(hipHostFree(tp.host[i]));
+        tp.host[i] = nullptr;
+        tp.host_cap[i] = 0;
+        const uint64_t cap = len + len / 8 + 4096;
+        HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&tp.host[i]), cap, hipHostMallocDefault));
+        tp.host_cap[i] = cap;
+    }
+    HIPCHK(hipEventRecord(tp.ev_fmt, stream));
+    HIPCHK(hipStreamWaitEvent(tp.copy, tp.ev_fmt, 0));
+    if (len) HIPCHK(hipMemcpyAsync(tp.host[i], tp.dev[i].p, len, hipMemcpyDeviceToHost, tp.copy));
+    HIPCHK(hipEventRecord(tp.ev_done[i], tp.copy));
+    tp.recorded[i] = true;
+    tp.len[i] = len;
+    tp.copying[i] = true;
+    tp.cur = j;
+    if (nbytes) *nbytes = len;
+    // the previous span's rows: landed by now or soon (its copy ran beside this span's kernels)
+    if (tp.copying[j]) {
+        HIPCHK(hipEventSynchronize(tp.ev_done[j]));
+        tp.copying[j] = false;
+        return consume(tp.host[j], tp.len[j]);
+    }
+    return GSNP_OK;
+}
+
+int Ctx::reset_text()
+{
+    TextPipe& tp = pipe;
+    if (tp.copy) HIPCHK(hipStreamSynchronize(tp.copy));
+    tp.copying[0] = tp.copying[1] = false;
+    tp.cur = 0;
+    return GSNP_OK;
+}
+
+int Ctx::window_text_to_host(const TextSink& consume)
+{
+    TextPipe& tp = pipe;
+    if (tp.copy) HIPCHK(hipStreamSynchronize(tp.copy));
+    if (tp.host_cap[0] < text_len) {
+        if (tp.host[0]) HIPCHK(hipHostFree(tp.host[0]));
+        tp.host[0] = nullptr;
+        tp.host_cap[0] = 0;
+        const uint64_t cap = text_len + text_len / 8 + 4096;
+        HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&tp.host[0]), cap, hipHostMallocDefault));
+        tp.host_cap[0] = cap;
+    }
+    if (text_len) HIPCHK(hipMemcpyAsync(tp.host[0], text.p, text_len, hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+    return consume(tp.host[0], text_len);
+}
+
+int Ctx::finish_text(const TextSink& consume)
+{
+    TextPipe& tp = pipe;
+    const int j = tp.cur ^ 1;  // the last span queued
+    if (!tp.copying[j]) return GSNP_OK;
+    HIPCHK(hipEventSynchronize(tp.ev_done[j]));
+    tp.copying[j] = false;
+    return consume(tp.host[j], tp.len[j]);
 }
 
 }  // namespace gsnp

@@ -565,17 +565,65 @@ int matrix_read(const char* path, double* pm, std::string& err)
 
 int matrix_write(const char* path, const double* pm, const gsnp_params& p, std::string& err)
 {
-    FilePtr f(std::fopen(path, "w"));
+    FILE* f = std::fopen(path, "w");
     if (!f) {
         err = std::string("cannot create ") + path;
         return GSNP_E_IO;
     }
-    for (int q = 0; q <= p.q_max - p.q_min; q++)
-        for (int c = 0; c < p.read_length; c++) {
-            std::fprintf(f.get(), "%d\t%d", q, c);
-            for (int t = 0; t < 16; t++) std::fprintf(f.get(), "\t%.16e", pm[((size_t)q << 12) | ((size_t)c << 4) | t]);
-            std::fputc('\n', f.get());
+    bool ok = true;
+    for (int q = 0; q <= p.q_max - p.q_min && ok; q++)
+        for (int c = 0; c < p.read_length && ok; c++) {
+            ok = std::fprintf(f, "%d\t%d", q, c) > 0;
+            for (int t = 0; t < 16 && ok; t++)
+                ok = std::fprintf(f, "\t%.16e", pm[((size_t)q << 12) | ((size_t)c << 4) | t]) > 0;
+            ok = ok && std::fputc('\n', f) != EOF;
         }
+    ok = std::fclose(f) == 0 && ok;
+    if (!ok) {
+        err = std::string("cannot write ") + path;
+        return GSNP_E_IO;
+    }
+    return GSNP_OK;
+}
+
+// Genome::read_region and Chr_info::set_region (cpp/chromosome.cc:66-170):
+// each line "chr start end" (1-based, inclusive) marks the windows that
+// [start - read_length, end - 1], clipped to [0, length], touches.  Lines of
+// unknown chromosomes are skipped, as in the reference.
+int load_regions(const char* path, const Genome& g, int read_length, std::vector<std::vector<uint64_t>>& win_mask,
+                 std::string& err)
+{
+    FilePtr f(std::fopen(path, "r"));
+    if (!f) {
+        err = std::string("cannot open ") + path;
+        return GSNP_E_IO;
+    }
+    win_mask.assign(g.chr.size(), {});
+    LineReader lr(f.get());
+    const char* s;
+    size_t n;
+    while (lr.next(s, n)) {
+        std::string line(s, n);
+        char name[1024];
+        long long a, b;
+        if (std::sscanf(line.c_str(), "%1023s %lld %lld", name, &a, &b) != 3) {
+            err = "Wrong format in target region file";
+            return GSNP_E_FORMAT;
+        }
+        auto it = g.index.find(name);
+        if (it == g.index.end()) continue;  // "Unexpected Chromosome"
+        const uint64_t len = g.chr[it->second].code.size();
+        long long lo = a - read_length, hi = b - 1;
+        lo = lo < 0 ? 0 : (lo >= (long long)len ? (long long)len : lo);
+        hi = hi < 0 ? 0 : (hi >= (long long)len ? (long long)len : hi);
+        if (lo > hi) {
+            err = "Invalid region: " + std::to_string(lo) + "-" + std::to_string(hi);
+            return GSNP_E_FORMAT;
+        }
+        std::vector<uint64_t>& m = win_mask[it->second];
+        if (m.empty()) m.assign((len / GSNP_WIN + 1) / 64 + 1, 0);
+        for (uint64_t w = (uint64_t)lo / GSNP_WIN; w <= (uint64_t)hi / GSNP_WIN; w++) m[w / 64] |= 1ull << (63 - w % 64);
+    }
     return GSNP_OK;
 }
 

@@ -49,4 +49,14 @@ void matrix_from_counts(const uint64_t* cnt, const gsnp_params& p, double* pm);
 int matrix_read(const char* path, double* pm, std::string& err);
 int matrix_write(const char* path, const double* pm, const gsnp_params& p, std::string& err);
 
+// -T target regions (cpp/chromosome.cc:66-170): per chromosome of g, a bit
+// per window (window w at bit 63 - w % 64 of word w / 64) set where a region
+// touches the window; empty for a chromosome without a region line.
+int load_regions(const char* path, const Genome& g, int read_length, std::vector<std::vector<uint64_t>>& win_mask,
+                 std::string& err);
+inline bool window_called(const std::vector<uint64_t>& m, uint64_t w)
+{
+    return w / 64 < m.size() && ((m[w / 64] >> (63 - w % 64)) & 1);
+}
+
 }  // namespace gsnp

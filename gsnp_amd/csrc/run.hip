@@ -15,6 +15,7 @@
 #include "cwriter.h"
 #include "engine.h"
 #include "inputs.h"
+#include "synth.h"
 
 namespace gsnp {
 
@@ -39,41 +40,61 @@ struct CtxDeleter {
 
 }  // namespace
 
+// Text modes of a calling pass: no rows, rows formatted in device memory, or
+// rows copied on to host memory.
+enum TextMode : int { kTextNone = 0, kTextDevice = 1, kTextHost = 2 };
+
 // One calling pass over a chromosome already loaded into the context.
 // batched = 0 runs the per-window path (init, update, likelihood, posterior,
 // recycle per window, cpp/call_genotype.cc:474-644); batched = k calls spans
 // of k windows straight from the device read store.  After each window
 // (per-window path) or span (batched path) the results are handed to
-// on_results(offset, span_start, span_sites) and, with emit_text, the span's
-// rows are formatted in device memory and their size given to on_text.
-template <class OnResults, class OnText>
-int call_chromosome(gsnp_ctx* ctx, const std::vector<int32_t>& pos, uint32_t N, int batched, bool emit_text,
-                    OnResults on_results, OnText on_text)
+// on_results(offset, span_start, span_sites).  With kTextDevice the rows are
+// formatted in device memory; with kTextHost they also go to host memory and
+// to sink(ptr, len) in order: per window with one synchronous copy (the
+// reference's structure), per span through the context's text pipe, whose
+// copy of one span runs beside the next span's kernels (the caller ends the
+// pass with finish_text).  *text_bytes adds up the rows.
+template <class OnResults>
+int call_chromosome(gsnp_ctx* ctx, const std::vector<int32_t>& pos, uint32_t N, int batched, int text,
+                    OnResults on_results, uint64_t* text_bytes, const Ctx::TextSink& sink,
+                    const std::vector<uint64_t>* mask = nullptr)
 {
     int rc;
     const uint32_t nwin = N / GSNP_WIN + 1;
-    if (batched) {
-        const uint32_t per = (uint32_t)batched;
-        for (uint32_t w0 = 0; w0 < nwin; w0 += per) {
+    // with -T regions only the windows they touch are called, each span a run
+    // of consecutive called windows (the per-window path then calls one-window
+    // spans: a window after a skipped one has no carried tail)
+    if (batched || mask) {
+        const uint64_t per = batched > 0 ? (uint64_t)batched : 1u;
+        for (uint32_t w0 = 0; w0 < nwin;) {
+            if (mask && !window_called(*mask, w0)) {
+                w0++;
+                continue;
+            }
+            uint32_t w1 = w0 + 1;
+            while (w1 < nwin && w1 - w0 < per && (!mask || window_called(*mask, w1))) w1++;
             const uint32_t ws = w0 * GSNP_WIN;
-            const uint32_t we = (uint32_t)std::min<uint64_t>((uint64_t)(w0 + per) * GSNP_WIN, N);
+            const uint32_t we = (uint32_t)std::min<uint64_t>((uint64_t)w1 * GSNP_WIN, N);
             const uint32_t n = we > ws ? we - ws : 0;
             uint64_t nb = 0;
-            rc = gsnp_batched_call(ctx, ws, n, emit_text, &nb);
+            rc = gsnp_batched_call(ctx, ws, n, text == kTextDevice, &nb);
             if (rc) return rc;
             rc = on_results(0u, ws, n);
             if (rc) return rc;
             // a chromosome whose length is a multiple of the window ends with an
             // empty window (cpp/call_genotype.cc:636-644); report it on its own
             const uint32_t last = N / GSNP_WIN;
-            if (N % GSNP_WIN == 0 && n > 0 && last >= w0 && last < w0 + per) {
+            if (N % GSNP_WIN == 0 && n > 0 && last >= w0 && last < w1) {
                 rc = on_results(0u, N, 0u);
                 if (rc) return rc;
             }
-            if (emit_text) {
-                rc = on_text(nb);
+            if (text == kTextHost) {
+                rc = ctx->text_to_host(ws, n, &nb, sink);
                 if (rc) return rc;
             }
+            if (text_bytes) *text_bytes += nb;
+            w0 = w1;
         }
         return GSNP_OK;
     }
@@ -92,11 +113,12 @@ int call_chromosome(gsnp_ctx* ctx, const std::vector<int32_t>& pos, uint32_t N, 
         if (rc) return rc;
         rc = on_results(0u, ws, C);
         if (rc) return rc;
-        if (emit_text) {
+        if (text != kTextNone) {
             uint64_t nb = 0;
             rc = gsnp_window_text(ctx, C, &nb);
-            if (!rc) rc = on_text(nb);
+            if (!rc && text == kTextHost) rc = ctx->window_text_to_host(sink);
             if (rc) return rc;
+            if (text_bytes) *text_bytes += nb;
         }
         rc = gsnp_window_recycle(ctx);
         if (rc) return rc;
@@ -118,8 +140,16 @@ struct gsnp_job {
     std::vector<double> pm;
     gsnp_ctx* ctx = nullptr;
     int cur = -1;
+    // every chromosome's device inputs stay resident once uploaded: the ones
+    // not selected are parked here (the selected one's slot is empty)
+    std::vector<gsnp::ChromState> parked;
+    // -T: per chromosome of g, the windows to call (gsnp_job_set_regions)
+    bool use_regions = false;
+    std::vector<std::vector<uint64_t>> win_mask;
+    const std::vector<uint64_t>* mask(int chr) const { return use_regions ? &win_mask[chr] : nullptr; }
     ~gsnp_job()
     {
+        parked.clear();
         if (ctx) gsnp_destroy(ctx);
     }
 };
@@ -127,6 +157,10 @@ struct gsnp_job {
 namespace gsnp {
 
 int job_select(gsnp_job* job, int index);
+
+// The calibration matrix (read, or counted on the device over every
+// chromosome) and the context of a job whose genome and reads are loaded.
+int job_prepare(gsnp_job* job, const char* matrix_in, const char* matrix_out);
 
 int job_open(const char* fasta, const char* soap, const char* dbsnp, const char* matrix_in, const char* matrix_out,
              const gsnp_params& p, std::unique_ptr<gsnp_job>& job)
@@ -138,6 +172,16 @@ int job_open(const char* fasta, const char* soap, const char* dbsnp, const char*
     if (rc) return fail(rc, err);
     rc = load_alignments(soap, p, job->g, job->blocks, err);
     if (rc) return fail(rc, err);
+    return job_prepare(job.get(), matrix_in, matrix_out);
+}
+
+int job_prepare(gsnp_job* job, const char* matrix_in, const char* matrix_out)
+{
+    std::string err;
+    const gsnp_params& p = job->p;
+    int rc;
+    job->parked.clear();
+    job->parked.resize(job->blocks.size());
     job->pm.assign((size_t)256 * 256 * 16, 1.0);
     if (matrix_in && matrix_in[0]) {
         rc = matrix_read(matrix_in, job->pm.data(), err);
@@ -151,7 +195,7 @@ int job_open(const char* fasta, const char* soap, const char* dbsnp, const char*
         if (rc) return rc;
         std::vector<uint64_t> cnt(kCalibBins, 0);
         for (int i = 0; i < (int)job->blocks.size(); i++) {
-            rc = job_select(job.get(), i);
+            rc = job_select(job, i);
             if (!rc) rc = gsnp_calib_count(job->ctx, cnt.data());
             if (rc) return rc;
         }
@@ -170,6 +214,13 @@ int job_select(gsnp_job* job, int index)
 {
     if (index < 0 || index >= (int)job->blocks.size()) return fail(GSNP_E_ARG, "chromosome index out of range");
     if (job->cur == index) return GSNP_OK;  // already on the device
+    if (job->cur >= 0) job->ctx->swap_chrom(job->parked[job->cur]);  // park the selected chromosome
+    job->cur = -1;  // nothing is selected until the chromosome is swapped in or uploaded
+    if (job->parked[index].chrom_id) {
+        job->ctx->swap_chrom(job->parked[index]);
+        job->cur = index;
+        return GSNP_OK;
+    }
     const ReadBlock& b = job->blocks[index];
     const Chrom& chr = job->g.chr[b.chr];
     int rc = gsnp_set_chromosome(job->ctx, chr.name.c_str(), chr.code.data(), (uint32_t)chr.code.size(),
@@ -189,49 +240,59 @@ int job_select(gsnp_job* job, int index)
     return GSNP_OK;
 }
 
-int run_pipeline(const char* fasta, const char* soap, const char* dbsnp, const char* matrix_in, const char* matrix_out,
-                 const gsnp_params& p, const char* out, int compress, int batched)
+// Every chromosome of an open job called and written to `out` (.cns text) or,
+// with compress, to the -C files out.1_2 ... out.15_17.
+int job_write(gsnp_job* job, const char* out, int compress, int batched)
 {
-    std::unique_ptr<gsnp_job> job;
-    int rc = job_open(fasta, soap, dbsnp, matrix_in, matrix_out, p, job);
-    if (rc) return rc;
+    int rc;
     std::string err;
     std::unique_ptr<FILE, FileCloser> fo(std::fopen(out, "w"));
     if (!fo) return fail(GSNP_E_IO, std::string("cannot create ") + out);
     CWriter cw;
     if (compress && cw.open(out, err)) return fail(GSNP_E_IO, err);
-    std::vector<char> host_text;
     gsnp_ctx* ctx = job->ctx;
+    // rows land in pinned host memory and go to the file in order
+    const Ctx::TextSink sink = [&](const char* rows, uint64_t len) -> int {
+        if (len && std::fwrite(rows, 1, len, fo.get()) != len) return fail(GSNP_E_IO, std::string("cannot write ") + out);
+        return GSNP_OK;
+    };
+    rc = ctx->reset_text();
+    if (rc) return rc;
     for (int bi = 0; bi < (int)job->blocks.size(); bi++) {
-        rc = job_select(job.get(), bi);
+        rc = job_select(job, bi);
         if (rc) return rc;
         const ReadBlock& b = job->blocks[bi];
         const Chrom& chr = job->g.chr[b.chr];
         const uint32_t N = (uint32_t)chr.code.size();
         auto on_results = [&](uint32_t offset, uint32_t ws, uint32_t n) -> int {
             if (!compress) return GSNP_OK;
-            if (n == 0) {  // the empty window after a chromosome that fills its last window
-                cw.write_empty_window(chr.name, ws);
-                return GSNP_OK;
-            }
+            if (n == 0)  // the empty window after a chromosome that fills its last window
+                return cw.write_empty_window(chr.name, ws, err) ? fail(GSNP_E_IO, err) : GSNP_OK;
             // one -C chunk per reference window of the span, encoded on the device
             return encode_columns(*ctx, offset, ws, n, [&](const ColumnPass& pass) -> int {
-                cw.write_chunks(chr.name, pass);
-                return GSNP_OK;
+                return cw.write_chunks(chr.name, pass, err) ? fail(GSNP_E_IO, err) : GSNP_OK;
             });
         };
-        auto on_text = [&](uint64_t nb) -> int {
-            host_text.resize(nb);
-            int r2 = gsnp_fetch_text(ctx, host_text.data(), nb, &nb);
-            if (r2) return r2;
-            std::fwrite(host_text.data(), 1, nb, fo.get());
-            return GSNP_OK;
-        };
-        rc = call_chromosome(ctx, b.pos, N, batched, !compress, on_results, on_text);
+        rc = call_chromosome(ctx, b.pos, N, batched, compress ? kTextNone : kTextHost, on_results, nullptr, sink,
+                             job->mask(b.chr));
+        if (rc) return rc;
+    }
+    if (!compress) {
+        rc = ctx->finish_text(sink);
         if (rc) return rc;
     }
     if (compress && cw.close(err)) return fail(GSNP_E_IO, err);
+    if (std::fclose(fo.release()) != 0) return fail(GSNP_E_IO, std::string("cannot write ") + out);
     return GSNP_OK;
+}
+
+int run_pipeline(const char* fasta, const char* soap, const char* dbsnp, const char* matrix_in, const char* matrix_out,
+                 const gsnp_params& p, const char* out, int compress, int batched)
+{
+    std::unique_ptr<gsnp_job> job;
+    int rc = job_open(fasta, soap, dbsnp, matrix_in, matrix_out, p, job);
+    if (rc) return rc;
+    return job_write(job.get(), out, compress, batched);
 }
 
 }  // namespace gsnp
@@ -242,18 +303,21 @@ int gsnp_run(const char* fasta, const char* soap, const char* dbsnp, const char*
              const gsnp_params* p, const char* out, int compress, int batched)
 {
     if (!fasta || !soap || !p || !out) return gsnp::fail(GSNP_E_ARG, "null argument");
-    return gsnp::run_pipeline(fasta, soap, dbsnp, matrix_in, matrix_out, *p, out, compress, batched);
+    return gsnp::guarded(
+        [&] { return gsnp::run_pipeline(fasta, soap, dbsnp, matrix_in, matrix_out, *p, out, compress, batched); });
 }
 
 int gsnp_job_open(const char* fasta, const char* soap, const char* dbsnp, const char* matrix_in, const gsnp_params* p,
                   gsnp_job** out)
 {
     if (!fasta || !soap || !p || !out) return gsnp::fail(GSNP_E_ARG, "null argument");
-    std::unique_ptr<gsnp_job> job;
-    int rc = gsnp::job_open(fasta, soap, dbsnp, matrix_in, nullptr, *p, job);
-    if (rc) return rc;
-    *out = job.release();
-    return GSNP_OK;
+    return gsnp::guarded([&] {
+        std::unique_ptr<gsnp_job> job;
+        int rc = gsnp::job_open(fasta, soap, dbsnp, matrix_in, nullptr, *p, job);
+        if (rc) return rc;
+        *out = job.release();
+        return GSNP_OK;
+    });
 }
 
 int gsnp_job_count(gsnp_job* job) { return job ? (int)job->blocks.size() : 0; }
@@ -282,13 +346,14 @@ int gsnp_job_call(gsnp_job* job, int batched, int emit_text, uint64_t* nbytes)
     if (!job || job->cur < 0) return gsnp::fail(GSNP_E_ARG, "no chromosome selected");
     const gsnp::ReadBlock& b = job->blocks[job->cur];
     const uint32_t N = (uint32_t)job->g.chr[b.chr].code.size();
+    if (emit_text < 0 || emit_text > 2) return gsnp::fail(GSNP_E_ARG, "emit_text must be 0, 1 or 2");
     uint64_t total = 0;
     auto on_results = [](uint32_t, uint32_t, uint32_t) -> int { return GSNP_OK; };
-    auto on_text = [&](uint64_t nb) -> int {
-        total += nb;
-        return GSNP_OK;
-    };
-    int rc = gsnp::call_chromosome(job->ctx, b.pos, N, batched, emit_text != 0, on_results, on_text);
+    const gsnp::Ctx::TextSink landed = [](const char*, uint64_t) -> int { return GSNP_OK; };
+    int rc = job->ctx->reset_text();
+    if (!rc)
+        rc = gsnp::call_chromosome(job->ctx, b.pos, N, batched, emit_text, on_results, &total, landed, job->mask(b.chr));
+    if (!rc && emit_text == gsnp::kTextHost) rc = job->ctx->finish_text(landed);
     if (nbytes) *nbytes = total;
     return rc;
 }
@@ -306,10 +371,134 @@ int gsnp_job_columns(gsnp_job* job, int batched, uint64_t* nbytes)
             return GSNP_OK;
         });
     };
-    auto on_text = [](uint64_t) -> int { return GSNP_OK; };
-    int rc = gsnp::call_chromosome(job->ctx, b.pos, N, batched, false, on_results, on_text);
+    int rc = gsnp::call_chromosome(job->ctx, b.pos, N, batched, gsnp::kTextNone, on_results, nullptr, nullptr,
+                                   job->mask(b.chr));
     if (nbytes) *nbytes = total;
     return rc;
+}
+
+int gsnp_job_synth(int config, unsigned long long seed, double scale, const gsnp_params* p, const char* matrix_in,
+                   gsnp_job** out, int* read_len)
+{
+    if (!p || !out) return gsnp::fail(GSNP_E_ARG, "null argument");
+    return gsnp::guarded([&] {
+        gsnp::SynthSet s;
+        gsnp::synth_generate(config, seed, scale, nullptr, gsnp::synth_threads(), s);
+        std::unique_ptr<gsnp_job> job(new gsnp_job());
+        job->p = *p;
+        job->p.read_length = s.read_len;
+        const uint32_t L = (uint32_t)s.read_len;
+        for (gsnp::SynthContig& ct : s.contigs) {
+            for (const gsnp::SynthRead& r : ct.reads)
+                if (r.indel || r.hit == 0)
+                    return gsnp::fail(GSNP_E_ARG, "in-memory synthetic jobs take configurations without indel edits");
+            const int ci = (int)job->g.chr.size();
+            job->g.index[ct.name] = ci;
+            job->g.chr.emplace_back();
+            gsnp::Chrom& c = job->g.chr.back();
+            c.name = ct.name;
+            c.code.swap(ct.ref);  // A0 C1 T2 G3 and 7 for N: the FASTA binarization (ch >> 1) & 7
+            for (const gsnp::SynthSnp& t : ct.snps) {
+                gsnp_snp g{};
+                g.hapmap = t.hapmap;
+                g.validated = t.validated;
+                g.indel = t.indel;
+                std::memcpy(g.freq, t.freq, sizeof g.freq);
+                c.snp_pos.push_back(t.pos);
+                c.snp.push_back(g);
+                c.code[t.pos] |= 8;
+            }
+            if (ct.reads.empty()) continue;
+            job->blocks.emplace_back();
+            gsnp::ReadBlock& b = job->blocks.back();
+            b.chr = ci;
+            b.pitch = L;
+            const size_t n = ct.reads.size();
+            b.pos.resize(n);
+            b.len_hit.resize(n);
+            b.strand.resize(n);
+            for (size_t i = 0; i < n; i++) {
+                b.pos[i] = ct.reads[i].pos;
+                b.len_hit[i] = (int32_t)((L << 16) | ct.reads[i].hit);
+                b.strand[i] = ct.reads[i].strand ? '-' : '+';
+            }
+            std::vector<gsnp::SynthRead>().swap(ct.reads);
+            b.seq.swap(ct.seq);
+            b.qual.swap(ct.qual);
+        }
+        int rc = gsnp::job_prepare(job.get(), matrix_in, nullptr);
+        if (rc) return rc;
+        if (read_len) *read_len = s.read_len;
+        *out = job.release();
+        return GSNP_OK;
+    });
+}
+
+int gsnp_job_pass(gsnp_job* job, int batched, int emit_text, uint64_t* nbytes, uint64_t* n_sites)
+{
+    if (!job || !job->ctx) return gsnp::fail(GSNP_E_ARG, "null job");
+    if (emit_text < 0 || emit_text > 3) return gsnp::fail(GSNP_E_ARG, "emit_text must be 0, 1, 2 or 3");
+    uint64_t total = 0, sites = 0;
+    const bool columns = emit_text == 3;
+    auto on_results = [&](uint32_t offset, uint32_t ws, uint32_t n) -> int {
+        if (!columns || n == 0) return GSNP_OK;
+        return gsnp::encode_columns(*job->ctx, offset, ws, n, [&](const gsnp::ColumnPass& pass) -> int {
+            for (int f = 0; f < 5; f++) total += pass.bytes[f];
+            return GSNP_OK;
+        });
+    };
+    const gsnp::Ctx::TextSink landed = [](const char*, uint64_t) -> int { return GSNP_OK; };
+    int rc = job->ctx->reset_text();
+    for (int i = 0; !rc && i < (int)job->blocks.size(); i++) {
+        rc = gsnp::job_select(job, i);
+        if (rc) break;
+        const gsnp::ReadBlock& b = job->blocks[i];
+        const uint32_t N = (uint32_t)job->g.chr[b.chr].code.size();
+        uint32_t per = (uint32_t)batched;
+        if (batched < 0) per = N / GSNP_WIN + 1;  // the whole chromosome in one span
+        rc = gsnp::call_chromosome(job->ctx, b.pos, N, (int)per, columns ? (int)gsnp::kTextNone : emit_text,
+                                   on_results, &total, landed, job->mask(b.chr));
+        sites += N;
+    }
+    if (!rc && emit_text == gsnp::kTextHost) rc = job->ctx->finish_text(landed);
+    if (nbytes) *nbytes = total;
+    if (n_sites) *n_sites = sites;
+    return rc;
+}
+
+int gsnp_job_set_regions(gsnp_job* job, const char* path)
+{
+    if (!job) return gsnp::fail(GSNP_E_ARG, "null job");
+    return gsnp::guarded([&] {
+        if (!path || !path[0]) {
+            job->use_regions = false;
+            job->win_mask.clear();
+            return GSNP_OK;
+        }
+        std::string err;
+        std::vector<std::vector<uint64_t>> m;
+        int rc = gsnp::load_regions(path, job->g, job->p.read_length, m, err);
+        if (rc) return gsnp::fail(rc, err);
+        job->win_mask.swap(m);
+        job->use_regions = true;
+        return GSNP_OK;
+    });
+}
+
+int gsnp_job_write_matrix(gsnp_job* job, const char* path)
+{
+    if (!job || !path) return gsnp::fail(GSNP_E_ARG, "null argument");
+    return gsnp::guarded([&] {
+        std::string err;
+        int rc = gsnp::matrix_write(path, job->pm.data(), job->p, err);
+        return rc ? gsnp::fail(rc, err) : GSNP_OK;
+    });
+}
+
+int gsnp_job_write(gsnp_job* job, const char* out, int compress, int batched)
+{
+    if (!job || !job->ctx || !out) return gsnp::fail(GSNP_E_ARG, "null argument");
+    return gsnp::guarded([&] { return gsnp::job_write(job, out, compress, batched); });
 }
 
 gsnp_ctx* gsnp_job_context(gsnp_job* job) { return job ? job->ctx : nullptr; }
@@ -318,20 +507,25 @@ void gsnp_job_close(gsnp_job* job) { delete job; }
 
 int gsnp_decompress(const char* prefix, const char* out)
 {
-    std::string err;
-    int rc = gsnp::decompress_files(prefix, out, err);
-    return rc ? gsnp::fail(rc == 1 ? GSNP_E_IO : GSNP_E_FORMAT, err) : GSNP_OK;
+    if (!prefix || !out) return gsnp::fail(GSNP_E_ARG, "null argument");
+    return gsnp::guarded([&] {
+        std::string err;
+        int rc = gsnp::decompress_files(prefix, out, err);
+        return rc ? gsnp::fail(rc == 1 ? GSNP_E_IO : GSNP_E_FORMAT, err) : GSNP_OK;
+    });
 }
 
 int gsnp_cns_open(const char* prefix, gsnp_cns_reader** out)
 {
     if (!prefix || !out) return gsnp::fail(GSNP_E_ARG, "null argument");
-    std::unique_ptr<gsnp_cns_reader> r(new gsnp_cns_reader());
-    std::string err;
-    int rc = r->rd.open(prefix, err);
-    if (rc) return gsnp::fail(rc == 1 ? GSNP_E_IO : GSNP_E_FORMAT, err);
-    *out = r.release();
-    return GSNP_OK;
+    return gsnp::guarded([&] {
+        std::unique_ptr<gsnp_cns_reader> r(new gsnp_cns_reader());
+        std::string err;
+        int rc = r->rd.open(prefix, err);
+        if (rc) return gsnp::fail(rc == 1 ? GSNP_E_IO : GSNP_E_FORMAT, err);
+        *out = r.release();
+        return GSNP_OK;
+    });
 }
 
 int gsnp_cns_next(gsnp_cns_reader* r, gsnp_cns_window* w, int* has_window)
@@ -339,7 +533,9 @@ int gsnp_cns_next(gsnp_cns_reader* r, gsnp_cns_window* w, int* has_window)
     if (!r || !w || !has_window) return gsnp::fail(GSNP_E_ARG, "null argument");
     std::string err;
     bool got = false;
-    int rc = r->rd.next(r->w, got, err);
+    *has_window = 0;
+    int rc = gsnp::guarded([&] { return r->rd.next(r->w, got, err); });
+    if (rc == GSNP_E_NOMEM) return rc;
     *has_window = got ? 1 : 0;
     if (rc) return gsnp::fail(GSNP_E_FORMAT, err);
     if (!got) return GSNP_OK;
@@ -359,11 +555,13 @@ void gsnp_cns_close(gsnp_cns_reader* r) { delete r; }
 int gsnp_matrix_gen(const char* fasta, const char* soap, const char* dbsnp, const gsnp_params* p, double* pm)
 {
     if (!fasta || !soap || !p || !pm) return gsnp::fail(GSNP_E_ARG, "null argument");
-    std::unique_ptr<gsnp_job> job;
-    int rc = gsnp::job_open(fasta, soap, dbsnp, nullptr, nullptr, *p, job);
-    if (rc) return rc;
-    std::copy(job->pm.begin(), job->pm.end(), pm);
-    return GSNP_OK;
+    return gsnp::guarded([&] {
+        std::unique_ptr<gsnp_job> job;
+        int rc = gsnp::job_open(fasta, soap, dbsnp, nullptr, nullptr, *p, job);
+        if (rc) return rc;
+        std::copy(job->pm.begin(), job->pm.end(), pm);
+        return GSNP_OK;
+    });
 }
 
 }  // extern "C"

@@ -235,14 +235,21 @@ int gsnp_sort_likelihood(gsnp_ctx* ctx, uint32_t* records, uint32_t pitch, const
 /* Per-stage device time from HIP events recorded on the context's stream:
  * returns (and resets) the accumulated milliseconds and launch counts of the
  * stages site build, sort, likelihood, posterior, rank-sum, format, recycle,
- * fused, adjoint (9 entries), then turns recording on or off.  Replaces the cutil
+ * fused, adjoint, -C columns, calibration counts (11 entries), then turns
+ * recording on or off.  Replaces the cutil
  * timers of the reference (kernel/cuda_header.h:37-50). */
-#define GSNP_N_STAGES 10
+#define GSNP_N_STAGES 11
 int gsnp_profile(gsnp_ctx* ctx, int enable, double* stage_ms, uint64_t* stage_count, int n_stages);
 
 /* Observation records the likelihood stage has consumed since the last
  * reset (throughput accounting). */
 int gsnp_stats(gsnp_ctx* ctx, uint64_t* n_records, int reset);
+
+/* Counters since the last reset, the first n of: [0] observation records
+ * called, [1] sites finished on the host with the host libm (rank-sum normal
+ * approximation or a consensus quality within the guard band of an integer),
+ * [2] those of [1] that took the normal approximation (n1 + n2 >= 64). */
+int gsnp_counters(gsnp_ctx* ctx, uint64_t* out, int n, int reset);
 
 /* Session over input files: parses FASTA, SOAP and known SNPs, trains (or
  * reads) the calibration matrix and creates the context, so that calling
@@ -257,16 +264,46 @@ int gsnp_job_info(gsnp_job* job, int index, char* name, uint64_t cap, uint32_t* 
 /* upload a chromosome's reference and reads to the device */
 int gsnp_job_select(gsnp_job* job, int index);
 /* one full calling pass over the selected chromosome (batched = windows per
- * fused span, 0 for the per-window path); rows stay in device memory and
- * *nbytes sums their size.  Replaces the window loop of new_soap2cns,
- * cpp/call_genotype.cc:474-644. */
+ * fused span, 0 for the per-window path).  emit_text 0: no rows; 1: rows
+ * formatted in device memory; 2: rows also copied to pinned host memory (per
+ * window with one synchronous copy; per span with the copy of each span
+ * running beside the next span's kernels).  *nbytes sums the rows.  Replaces
+ * the window loop of new_soap2cns, cpp/call_genotype.cc:474-644. */
 int gsnp_job_call(gsnp_job* job, int batched, int emit_text, uint64_t* nbytes);
+/* the same over every chromosome of the job in call order (a whole-genome
+ * pass; chromosomes stay resident on the device once uploaded, so a pass
+ * moves no inputs); batched < 0 makes each chromosome one span.  emit_text 3
+ * encodes the -C column chunks on the device and copies them to host memory
+ * instead of rows (*nbytes then sums the chunk bytes).  *n_sites sums the
+ * called sites. */
+int gsnp_job_pass(gsnp_job* job, int batched, int emit_text, uint64_t* nbytes, uint64_t* n_sites);
 /* the same pass with the -C column chunks of every window encoded on the
  * device and copied to host memory, but not written to files (the encoder
  * of gsnp_run with compress = 1); *nbytes sums the chunk bytes.  Replaces the
  * extraction and compression of Call_win::call_cns2 with -C,
  * cpp/compress.h:33-1312. */
 int gsnp_job_columns(gsnp_job* job, int batched, uint64_t* nbytes);
+/* Every chromosome of the job called and written to `out` (.cns text) or,
+ * with compress, to the -C files <out>.1_2 ... <out>.15_17; batched as in
+ * gsnp_run.  The calling and output half of main (cpp/main.cc:391-406). */
+int gsnp_job_write(gsnp_job* job, const char* out, int compress, int batched);
+/* -T target regions (Genome::read_region and Chr_info::set_region,
+ * cpp/chromosome.cc:66-170; the window gating of cpp/call_genotype.cc:535,
+ * :593): lines "chr start end" (1-based, inclusive) select the windows that
+ * [start - read_length, end - 1] touches; later calls and writes of the job
+ * call and output only those windows (a chromosome without a line: none),
+ * each built from every read that covers it.  A null or empty path clears
+ * the regions.  Lines of unknown chromosomes are skipped. */
+int gsnp_job_set_regions(gsnp_job* job, const char* path);
+/* The job's calibration matrix as the -M text file (cpp/matrix.cc:230-241). */
+int gsnp_job_write_matrix(gsnp_job* job, const char* path);
+/* A job over a seeded synthetic data set generated in memory (the data
+ * gsnp_synth_write writes as files, for configurations without indel edits:
+ * 1-5), calibrated on the device as gsnp_job_open does (or from matrix_in).
+ * p's read length is replaced by the configuration's (*read_len).  No
+ * reference counterpart: it feeds benchmarks without text files. */
+int gsnp_job_synth(int config, unsigned long long seed, double scale, const gsnp_params* p, const char* matrix_in,
+                   gsnp_job** out, int* read_len);
 gsnp_ctx* gsnp_job_context(gsnp_job* job);
 void gsnp_job_close(gsnp_job* job);
 
@@ -304,6 +341,11 @@ int gsnp_format_g(double v, char* out);
 /* Seeded synthetic inputs: <prefix>.fa, .soap (and .snp for config 2).
  * Returns the number of alignment lines. */
 long long gsnp_synth_write(int config, unsigned long long seed, double scale, const char* prefix, int* read_len);
+/* The same files for the listed contigs only (indices in generation order). */
+long long gsnp_synth_write_contigs(int config, unsigned long long seed, double scale, const char* prefix,
+                                   const int* contigs, int n_contigs, int* read_len);
+/* Contig count of a configuration; the first `cap` lengths go to lens (may be NULL). */
+int gsnp_synth_contigs(int config, unsigned long long seed, double scale, uint64_t* lens, int cap);
 
 #ifdef __cplusplus
 }

@@ -36,9 +36,10 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <pthread.h>
 #include <time.h>
 
-static char g_err[512];
+static __thread char g_err[512]; /* per thread: the threaded run calls windows on several */
 
 static int fail(int code, const char* fmt, ...)
 {
@@ -1493,7 +1494,134 @@ static void cwrite_window(cwriter* w, const char* chr, uint32_t ws, uint32_t C, 
     }
 }
 
+/* ============================================================ regions */
+
+/* Chr_info::set_region, cpp/chromosome.cc:66-107: clip [start, end] to
+ * [0, len], then set sites start..end (inclusive) in the site mask and
+ * windows start/W..end/W in the window mask; words of 64 positions, position
+ * p at bit 63 - p % 64.  (The reference's shift by 64 when a range fills one
+ * whole word sets nothing on x86; the intended bits are set here.) */
+static void mask_set(uint64_t* m, int64_t a, int64_t b)
+{
+    for (int64_t p = a; p <= b; p++) m[p / 64] |= 1ull << (63 - p % 64);
+}
+
+int orc_region_set(uint64_t* site_mask, uint64_t* win_mask, uint32_t len, int start, int end)
+{
+    if (start < 0)
+        start = 0;
+    else if ((uint32_t)start >= len)
+        start = (int)len;
+    if (end < 0)
+        end = 0;
+    else if ((uint32_t)end >= len)
+        end = (int)len;
+    if (start > end) return fail(ORC_E_FORMAT, "Invalid region: %d-%d", start, end);
+    if (site_mask) mask_set(site_mask, start, end);
+    if (win_mask) mask_set(win_mask, start / ORC_WIN, end / ORC_WIN);
+    return ORC_OK;
+}
+
+/* Chr_info::is_in_region_win, cpp/soap_snp.h:796-800 */
+int orc_region_in_win(const uint64_t* win_mask, uint32_t pos)
+{
+    const uint32_t w = pos / ORC_WIN;
+    return (int)((win_mask[w / 64] >> (63 - w % 64)) & 1);
+}
+
+/* Words of a mask over n positions (one spare position: set_region clips the
+ * end to len, not len - 1). */
+uint64_t orc_region_words(uint64_t n) { return n / 64 + 1; }
+
+/* Genome::read_region, cpp/chromosome.cc:127-170: lines "chr start end"
+ * (1-based, inclusive); each sets set_region(start - read_length, end - 1) on
+ * its chromosome, whose masks start empty; lines of unknown chromosomes are
+ * skipped.  win_masks[c] stays NULL for a chromosome without a line. */
+static int read_regions(const char* path, const orc_genome* g, int read_length, uint64_t** win_masks)
+{
+    FILE* f = fopen(path, "r");
+    if (!f) return fail(ORC_E_IO, "cannot open %s", path);
+    char line[4096], name[1024];
+    long long a, b;
+    int rc = ORC_OK;
+    while (!rc && fgets(line, sizeof line, f)) {
+        if (sscanf(line, "%1023s %lld %lld", name, &a, &b) != 3) {
+            rc = fail(ORC_E_FORMAT, "Wrong format in target region file");
+            break;
+        }
+        int ci = orc_genome_find(g, name);
+        if (ci < 0) {
+            fprintf(stderr, "Unexpected Chromosome:%s\n", name);
+            continue;
+        }
+        const orc_chr* c = &g->chr[ci];
+        if (!win_masks[ci])
+            win_masks[ci] = (uint64_t*)calloc(orc_region_words(c->len / ORC_WIN + 1), sizeof(uint64_t));
+        if (a - read_length < INT_MIN || b - 1 > INT_MAX) {
+            rc = fail(ORC_E_FORMAT, "region bounds out of range");
+            break;
+        }
+        rc = orc_region_set(NULL, win_masks[ci], c->len, (int)(a - read_length), (int)(b - 1));
+    }
+    fclose(f);
+    return rc;
+}
+
 /* ============================================================ pipeline */
+
+static FILE* open_sfx(const char* prefix, const char* sfx, const char* mode);
+
+/* .1_2 (cpp/compress.h:151-165): window count, then "chr\tposStart\twinSize" per window */
+static int cwrite_index(const cwriter* w, const char* prefix)
+{
+    FILE* f12 = open_sfx(prefix, ".1_2", "w");
+    if (!f12) return fail(ORC_E_IO, "cannot create .1_2");
+    fprintf(f12, "%lu\n", (unsigned long)w->nwin);
+    for (uint64_t i = 0; i < w->nwin; i++)
+        fprintf(f12, "%s\t%lu\t%lu\n", w->names[i], (unsigned long)w->starts[i], (unsigned long)w->sizes[i]);
+    fclose(f12);
+    return ORC_OK;
+}
+
+static int cwriter_open(cwriter* w, const char* prefix)
+{
+    memset(w, 0, sizeof *w);
+    w->f346 = open_sfx(prefix, ".3_4_6", "wb");
+    w->f5 = open_sfx(prefix, ".5_7_8_9_14_16", "wb");
+    w->f10 = open_sfx(prefix, ".10", "wb");
+    w->f11 = open_sfx(prefix, ".11_12_13", "wb");
+    w->f15 = open_sfx(prefix, ".15_17", "wb");
+    if (!w->f346 || !w->f5 || !w->f10 || !w->f11 || !w->f15) return fail(ORC_E_IO, "cannot create -C files");
+    return ORC_OK;
+}
+
+static void cwriter_close(cwriter* w)
+{
+    if (w->f346) fclose(w->f346);
+    if (w->f5) fclose(w->f5);
+    if (w->f10) fclose(w->f10);
+    if (w->f11) fclose(w->f11);
+    if (w->f15) fclose(w->f15);
+    for (uint64_t i = 0; i < w->nwin; i++) free(w->names[i]);
+    free(w->names), free(w->starts), free(w->sizes);
+    memset(w, 0, sizeof *w);
+}
+
+/* The -C files of one window of given sites (the chunk writer on its own,
+ * for byte-level known-answer tests of the layout). */
+int orc_cwrite_sites(const char* prefix, const char* chr, uint32_t ws, uint32_t C, const orc_site* sites)
+{
+    cwriter cw;
+    int rc = cwriter_open(&cw, prefix);
+    if (!rc) {
+        cwrite_window(&cw, chr, ws, C, sites);
+        rc = cwrite_index(&cw, prefix);
+    }
+    cwriter_close(&cw);
+    return rc;
+}
+
+
 
 static FILE* open_sfx(const char* prefix, const char* sfx, const char* mode)
 {
@@ -1514,8 +1642,103 @@ static double now_sec(void)
  * window list of a chromosome of length N is N/W+1 windows, the last one
  * of N%W sites (cpp/call_genotype.cc:592-644).  max_sites > 0 stops after
  * that many called sites (a bounded sample for timing). */
+/* Text of one window's rows, built by a worker of the threaded run. */
+typedef struct {
+    char* text;
+    size_t len;
+    int rc, ready;
+} win_text;
+
+typedef struct {
+    const orc_reads* r;
+    const orc_genome* g;
+    const orc_params* p;
+    const double *S, *pp, *pr;
+    const int64_t* idx;
+    const int *wblk;     /* per window: chromosome block */
+    const int64_t *b0, *b1;
+    const int* order;
+    const uint32_t *wstart, *wsize;
+    int nw;
+    int next;            /* next window to take */
+    int written;         /* windows handed to the writer so far */
+    int ahead;           /* at most this many windows past `written` in flight */
+    win_text* out;
+    pthread_mutex_t mu;
+    pthread_cond_t cv;
+} run_pool;
+
+static void* run_worker(void* arg)
+{
+    run_pool* P = (run_pool*)arg;
+    orc_site* sites = (orc_site*)malloc(sizeof(orc_site) * (ORC_WIN + 1));
+    uint32_t* rec = (uint32_t*)malloc(sizeof(uint32_t) * (size_t)(ORC_WIN + 1) * ORC_MAX_REC);
+    char row[4096];
+    for (;;) {
+        pthread_mutex_lock(&P->mu);
+        while (P->next < P->nw && P->next >= P->written + P->ahead) pthread_cond_wait(&P->cv, &P->mu);
+        int w = P->next < P->nw ? P->next++ : -1;
+        pthread_mutex_unlock(&P->mu);
+        if (w < 0) break;
+        const int b = P->wblk[w];
+        const orc_chr* c = &P->g->chr[P->order[b]];
+        const uint32_t ws = P->wstart[w], C = P->wsize[w];
+        win_text t = {NULL, 0, ORC_OK, 1};
+        if (!sites || !rec) {
+            t.rc = ORC_E_NOMEM;
+        } else {
+            t.rc = orc_call_window(P->r, P->idx + P->b0[b], P->b1[b] - P->b0[b], c, ws, C, P->p, P->S, P->pp, P->pr,
+                                   sites, rec, NULL);
+        }
+        if (!t.rc) {
+            size_t cap = (size_t)C * 96 + 64, len = 0;
+            t.text = (char*)malloc(cap);
+            for (uint32_t j = 0; t.text && j < C; j++) {
+                int n = orc_format_row(c->name, ws + j + 1, &sites[j], &sites[j + 1], row);
+                if (len + (size_t)n > cap) {
+                    cap = cap * 2 + (size_t)n;
+                    t.text = (char*)realloc(t.text, cap);
+                    if (!t.text) break;
+                }
+                memcpy(t.text + len, row, (size_t)n);
+                len += (size_t)n;
+            }
+            if (!t.text) t.rc = ORC_E_NOMEM;
+            t.len = len;
+        }
+        pthread_mutex_lock(&P->mu);
+        P->out[w] = t;
+        pthread_cond_broadcast(&P->cv);
+        pthread_mutex_unlock(&P->mu);
+    }
+    free(sites);
+    free(rec);
+    return NULL;
+}
+
 int orc_run(const char* fasta, const char* soap, const char* dbsnp, const char* matrix_in, const char* matrix_out,
             const orc_params* p, const char* out, int compress, int64_t max_sites, orc_run_stats* stats)
+{
+    return orc_run_opts(fasta, soap, dbsnp, matrix_in, matrix_out, p, out, compress, max_sites, 1, NULL, stats);
+}
+
+int orc_run_threads(const char* fasta, const char* soap, const char* dbsnp, const char* matrix_in,
+                    const char* matrix_out, const orc_params* p, const char* out, int compress, int64_t max_sites,
+                    int nthreads, orc_run_stats* stats)
+{
+    return orc_run_opts(fasta, soap, dbsnp, matrix_in, matrix_out, p, out, compress, max_sites, nthreads, NULL,
+                        stats);
+}
+
+/* orc_run with the windows of the text output called on nthreads threads
+ * (each window is independent given the reads: orc_call_window builds its
+ * site matrix from every read that covers it), rows written in window order. */
+/* With regions (-T), only the windows that a region touches are called and
+ * written (the window gating of cpp/call_genotype.cc:535 and :593); each is
+ * built from every read that covers it. */
+int orc_run_opts(const char* fasta, const char* soap, const char* dbsnp, const char* matrix_in,
+                 const char* matrix_out, const orc_params* p, const char* out, int compress, int64_t max_sites,
+                 int nthreads, const char* regions, orc_run_stats* stats)
 {
     orc_genome* g = NULL;
     orc_reads* r = NULL;
@@ -1535,6 +1758,13 @@ int orc_run(const char* fasta, const char* soap, const char* dbsnp, const char* 
     else
         rc = orc_matrix_gen(r, g, p, pm);
     if (!rc && matrix_out && matrix_out[0]) rc = orc_matrix_write(matrix_out, pm, p);
+    /* -T: window masks per chromosome (NULL: no region line, nothing called) */
+    uint64_t** wmask = NULL;
+    if (!rc && regions && regions[0]) {
+        wmask = (uint64_t**)calloc((size_t)g->n + 1, sizeof(uint64_t*));
+        rc = read_regions(regions, g, p->read_length, wmask);
+    }
+#define ORC_CALLED(ci, ws) (!wmask || (wmask[ci] && orc_region_in_win(wmask[ci], (ws))))
     orc_prior_gen(p, pp);
     orc_rank_table_gen(pr);
     orc_score_table(pm, S);
@@ -1582,18 +1812,75 @@ int orc_run(const char* fasta, const char* soap, const char* dbsnp, const char* 
         fo = fopen(out, "w");
         if (!fo) rc = fail(ORC_E_IO, "cannot create %s", out);
     }
-    if (!rc && compress) {
-        cw.f346 = open_sfx(out, ".3_4_6", "wb");
-        cw.f5 = open_sfx(out, ".5_7_8_9_14_16", "wb");
-        cw.f10 = open_sfx(out, ".10", "wb");
-        cw.f11 = open_sfx(out, ".11_12_13", "wb");
-        cw.f15 = open_sfx(out, ".15_17", "wb");
-        if (!cw.f346 || !cw.f5 || !cw.f10 || !cw.f11 || !cw.f15) rc = fail(ORC_E_IO, "cannot create -C files");
-    }
+    if (!rc && compress) rc = cwriter_open(&cw, out);
     orc_site* sites = (orc_site*)malloc(sizeof(orc_site) * (ORC_WIN + 1));
     char* row = (char*)malloc(4096);
     uint64_t nsites = 0, nwin = 0;
     double tcall = 0;
+    if (!rc && nthreads > 1 && !compress) {
+        /* the window list, bounded by max_sites as in the serial loop */
+        int cap = 16, nw = 0;
+        int* wblk = (int*)malloc(sizeof(int) * cap);
+        uint32_t* wstart = (uint32_t*)malloc(sizeof(uint32_t) * cap);
+        uint32_t* wsize = (uint32_t*)malloc(sizeof(uint32_t) * cap);
+        uint64_t planned = 0;
+        for (int b = 0; b < nblk; b++) {
+            const orc_chr* c = &g->chr[order[b]];
+            uint32_t n = c->len / ORC_WIN + 1;
+            for (uint32_t w = 0; w < n; w++) {
+                if (max_sites > 0 && (int64_t)planned >= max_sites) break;
+                if (!ORC_CALLED(order[b], w * ORC_WIN)) continue;
+                if (nw == cap) {
+                    cap *= 2;
+                    wblk = (int*)realloc(wblk, sizeof(int) * cap);
+                    wstart = (uint32_t*)realloc(wstart, sizeof(uint32_t) * cap);
+                    wsize = (uint32_t*)realloc(wsize, sizeof(uint32_t) * cap);
+                }
+                wblk[nw] = b;
+                wstart[nw] = w * ORC_WIN;
+                wsize[nw] = (w < c->len / ORC_WIN) ? ORC_WIN : c->len % ORC_WIN;
+                planned += wsize[nw];
+                nw++;
+            }
+        }
+        run_pool P;
+        memset(&P, 0, sizeof P);
+        P.r = r, P.g = g, P.p = p, P.S = S, P.pp = pp, P.pr = pr, P.idx = idx, P.wblk = wblk, P.b0 = b0, P.b1 = b1;
+        P.order = order, P.wstart = wstart, P.wsize = wsize, P.nw = nw, P.ahead = 2 * nthreads;
+        P.out = (win_text*)calloc((size_t)nw + 1, sizeof(win_text));
+        pthread_mutex_init(&P.mu, NULL);
+        pthread_cond_init(&P.cv, NULL);
+        pthread_t* th = (pthread_t*)malloc(sizeof(pthread_t) * nthreads);
+        double t0 = now_sec();
+        int started = 0;
+        for (int t = 0; t < nthreads; t++)
+            if (pthread_create(&th[started], NULL, run_worker, &P) == 0) started++;
+        if (!started) run_worker(&P);
+        for (int w = 0; w < nw; w++) {
+            pthread_mutex_lock(&P.mu);
+            while (!P.out[w].ready) pthread_cond_wait(&P.cv, &P.mu);
+            win_text t = P.out[w];
+            pthread_mutex_unlock(&P.mu);
+            if (!rc && t.rc) rc = fail(t.rc, "window %d failed", w);
+            if (!rc && t.len && fwrite(t.text, 1, t.len, fo) != t.len) rc = fail(ORC_E_IO, "cannot write %s", out);
+            free(t.text);
+            P.out[w].text = NULL;
+            if (!rc) {
+                nsites += wsize[w];
+                nwin++;
+            }
+            pthread_mutex_lock(&P.mu);
+            P.written = w + 1;
+            pthread_cond_broadcast(&P.cv);
+            pthread_mutex_unlock(&P.mu);
+        }
+        for (int t = 0; t < started; t++) pthread_join(th[t], NULL);
+        tcall = now_sec() - t0;
+        pthread_mutex_destroy(&P.mu);
+        pthread_cond_destroy(&P.cv);
+        free(th), free(P.out), free(wblk), free(wstart), free(wsize);
+        nblk = 0; /* done: skip the serial loop */
+    }
     for (int b = 0; !rc && b < nblk; b++) {
         const orc_chr* c = &g->chr[order[b]];
         uint32_t nw = c->len / ORC_WIN + 1;
@@ -1601,6 +1888,7 @@ int orc_run(const char* fasta, const char* soap, const char* dbsnp, const char* 
             if (max_sites > 0 && (int64_t)nsites >= max_sites) break;
             uint32_t ws = w * ORC_WIN;
             uint32_t C = (w < c->len / ORC_WIN) ? ORC_WIN : c->len % ORC_WIN;
+            if (!ORC_CALLED(order[b], ws)) continue;
             double t0 = now_sec();
             rc = orc_call_window(r, idx + b0[b], b1[b] - b0[b], c, ws, C, p, S, pp, pr, sites, NULL, NULL);
             if (rc) break;
@@ -1617,29 +1905,18 @@ int orc_run(const char* fasta, const char* soap, const char* dbsnp, const char* 
             nwin++;
         }
     }
-    if (!rc && compress) {
-        FILE* f12 = open_sfx(out, ".1_2", "w");
-        if (!f12)
-            rc = fail(ORC_E_IO, "cannot create .1_2");
-        else {
-            fprintf(f12, "%lu\n", (unsigned long)cw.nwin);
-            for (uint64_t i = 0; i < cw.nwin; i++)
-                fprintf(f12, "%s\t%lu\t%lu\n", cw.names[i], (unsigned long)cw.starts[i], (unsigned long)cw.sizes[i]);
-            fclose(f12);
-        }
-    }
+    if (!rc && compress) rc = cwrite_index(&cw, out);
     if (fo) fclose(fo);
-    if (cw.f346) fclose(cw.f346);
-    if (cw.f5) fclose(cw.f5);
-    if (cw.f10) fclose(cw.f10);
-    if (cw.f11) fclose(cw.f11);
-    if (cw.f15) fclose(cw.f15);
-    for (uint64_t i = 0; i < cw.nwin; i++) free(cw.names[i]);
-    free(cw.names), free(cw.starts), free(cw.sizes);
+    cwriter_close(&cw);
     if (stats) {
         stats->sites = nsites;
         stats->windows = nwin;
         stats->seconds_call = tcall;
+    }
+#undef ORC_CALLED
+    if (wmask) {
+        for (int ci = 0; ci < g->n; ci++) free(wmask[ci]);
+        free(wmask);
     }
     free(sites), free(row), free(order), free(b0), free(b1), free(seen), free(idx);
     free(pm), free(S), free(pp), free(pr);

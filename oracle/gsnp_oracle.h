@@ -175,7 +175,23 @@ typedef struct orc_run_stats {
 int orc_run(const char* fasta, const char* soap, const char* dbsnp, const char* matrix_in,
             const char* matrix_out, const orc_params* p, const char* out, int compress,
             int64_t max_sites, orc_run_stats* stats);
+/* orc_run with the text windows called on nthreads threads (-C output stays
+ * serial); seconds_call is then the wall time of the parallel section */
+int orc_run_threads(const char* fasta, const char* soap, const char* dbsnp, const char* matrix_in,
+                    const char* matrix_out, const orc_params* p, const char* out, int compress, int64_t max_sites,
+                    int nthreads, orc_run_stats* stats);
+/* orc_run_threads with -T regions: only windows a region touches are called */
+int orc_run_opts(const char* fasta, const char* soap, const char* dbsnp, const char* matrix_in,
+                 const char* matrix_out, const orc_params* p, const char* out, int compress, int64_t max_sites,
+                 int nthreads, const char* regions, orc_run_stats* stats);
+/* region masks (cpp/chromosome.cc:66-107): words of 64 positions, position p
+ * at bit 63 - p % 64; either mask may be NULL */
+int orc_region_set(uint64_t* site_mask, uint64_t* win_mask, uint32_t len, int start, int end);
+int orc_region_in_win(const uint64_t* win_mask, uint32_t pos);
+uint64_t orc_region_words(uint64_t n);
 int orc_decompress(const char* prefix, const char* out);
+/* the -C files (<prefix>.1_2 ... .15_17) of one window of given sites */
+int orc_cwrite_sites(const char* prefix, const char* chr, uint32_t ws, uint32_t C, const orc_site* sites);
 const char* orc_last_error(void);
 
 #ifdef __cplusplus

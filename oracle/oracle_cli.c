@@ -2,7 +2,9 @@
  * oracle_cli.c -- command line for the CPU oracle (test infrastructure only).
  * Flags follow cpp/main.cc:104 (SOAPsnp's getopt string) for the options the
  * calling path uses; extra flags: -D <prefix> decompress, -X <n> stop after n
- * sites (bounded timing sample), -R print the call-stage seconds.
+ * sites (bounded timing sample), -R print the call-stage seconds, -J <n> call
+ * the windows of text output on n threads.  -T <regions> calls only the
+ * windows the regions touch.
  */
 #include "gsnp_oracle.h"
 
@@ -16,11 +18,12 @@ int main(int argc, char** argv)
 {
     orc_params p;
     orc_params_default(&p);
-    const char *in = NULL, *fa = NULL, *out = NULL, *snp = NULL, *min = NULL, *mout = NULL, *dec = NULL;
-    int compress = 0, report = 0;
+    const char *in = NULL, *fa = NULL, *out = NULL, *snp = NULL, *min = NULL, *mout = NULL, *dec = NULL,
+               *regions = NULL;
+    int compress = 0, report = 0, threads = 1;
     long long max_sites = 0;
     int c;
-    while ((c = getopt(argc, argv, "i:d:o:z:g:p:r:e:ts:2a:b:j:k:umM:I:L:Q:CD:X:R")) != -1) {
+    while ((c = getopt(argc, argv, "i:d:o:z:g:p:r:e:ts:2a:b:j:k:umM:I:L:Q:CD:X:RJ:T:")) != -1) {
         switch (c) {
         case 'i': in = optarg; break;
         case 'd': fa = optarg; break;
@@ -47,6 +50,8 @@ int main(int argc, char** argv)
         case 'D': dec = optarg; break;
         case 'X': max_sites = atoll(optarg); break;
         case 'R': report = 1; break;
+        case 'J': threads = atoi(optarg) < 1 ? 1 : atoi(optarg); break;
+        case 'T': regions = optarg; break;
         default: fprintf(stderr, "unknown option\n"); return 1;
         }
     }
@@ -61,7 +66,7 @@ int main(int argc, char** argv)
         return 1;
     }
     orc_run_stats st;
-    int rc = orc_run(fa, in, snp, min, mout, &p, out, compress, max_sites, &st);
+    int rc = orc_run_opts(fa, in, snp, min, mout, &p, out, compress, max_sites, threads, regions, &st);
     if (rc) {
         fprintf(stderr, "error %d: %s\n", rc, orc_last_error());
         return rc;

@@ -43,6 +43,18 @@ class OrcSite(ctypes.Structure):
     ]
 
 
+class OrcReads(ctypes.Structure):
+    """orc_reads: the parsed alignments (seq/qual hold L chars per read)."""
+
+    _fields_ = [
+        ("n", ctypes.c_int64), ("L", ctypes.c_int),
+        ("pos", ctypes.POINTER(ctypes.c_int32)), ("len", ctypes.POINTER(ctypes.c_int32)),
+        ("hit", ctypes.POINTER(ctypes.c_int32)), ("strand", ctypes.POINTER(ctypes.c_uint8)),
+        ("chr", ctypes.POINTER(ctypes.c_int32)),
+        ("seq", ctypes.POINTER(ctypes.c_char)), ("qual", ctypes.POINTER(ctypes.c_char)),
+    ]
+
+
 _orc = None
 _ref = None
 
@@ -92,6 +104,14 @@ def orc() -> ctypes.CDLL:
             "orc_reads_load": (i, [ctypes.c_char_p, ctypes.POINTER(OrcParams), P, ctypes.POINTER(P)]),
             "orc_reads_free": (None, [P]),
             "orc_matrix_gen": (i, [P, P, ctypes.POINTER(OrcParams), P]),
+            "orc_format_row": (i, [ctypes.c_char_p, u32, ctypes.POINTER(OrcSite), ctypes.POINTER(OrcSite),
+                                   ctypes.c_char_p]),
+            "orc_cwrite_sites": (i, [ctypes.c_char_p, ctypes.c_char_p, u32, u32, ctypes.POINTER(OrcSite)]),
+            "orc_region_set": (i, [P, P, u32, i, i]),
+            "orc_region_in_win": (i, [P, u32]),
+            "orc_region_words": (u64, [u64]),
+            "orc_run_opts": (i, [ctypes.c_char_p] * 5 + [ctypes.POINTER(OrcParams), ctypes.c_char_p, i,
+                                 ctypes.c_int64, i, ctypes.c_char_p, P]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -133,11 +153,11 @@ def params(**kw) -> OrcParams:
     return p
 
 
-def run(fasta, soap, out, dbsnp=None, compress=False, max_sites=0, **kw) -> None:
+def run(fasta, soap, out, dbsnp=None, compress=False, max_sites=0, regions=None, **kw) -> None:
     p = params(**kw)
     b = lambda s: None if s is None else os.fsencode(s)
-    rc = orc().orc_run(b(fasta), b(soap), b(dbsnp), None, None, ctypes.byref(p), b(out), int(compress),
-                       max_sites, None)
+    rc = orc().orc_run_opts(b(fasta), b(soap), b(dbsnp), None, None, ctypes.byref(p), b(out), int(compress),
+                            max_sites, 1, b(regions), None)
     if rc:
         raise RuntimeError(f"oracle error {rc}: {orc().orc_last_error().decode()}")
 

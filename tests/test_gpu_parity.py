@@ -47,7 +47,7 @@ def test_text_matches_oracle(datasets, tmp_path, name):
     assert out.read_bytes() == ref.read_bytes(), _first_diff(out, ref)
 
 
-@pytest.mark.parametrize("name,per", [("c2w3", 1), ("c2w3", 8), ("stress", 2), ("c3s", 4)])
+@pytest.mark.parametrize("name,per", [("c2w3", 1), ("c2w3", 8), ("stress", 2), ("c3s", 4), ("c5s", 3)])
 def test_batched_matches_oracle(datasets, tmp_path, name, per):
     ds = datasets(name)
     print(f"[parity] batched {name} x{per}", flush=True)
@@ -151,7 +151,8 @@ def test_parameter_variants(datasets, tmp_path, name, flags, batched):
 
 @pytest.mark.parametrize("name,per,lds", [("stress", 0, None), ("c2w3", 0, None), ("stress", 4, None),
                                           ("c2w3", 4, None), ("c2x2", 0, None), ("c2x2", 4, None),
-                                          ("c2w3", 4, "0"), ("stress", 0, "0"), ("c2w3", 4, "8")])
+                                          ("c2w3", 4, "0"), ("stress", 0, "0"), ("c2w3", 4, "8"),
+                                          ("c5s", 0, None), ("c5s", 3, None), ("c3s", 4, None)])
 def test_compressed_output(datasets, tmp_path, monkeypatch, name, per, lds):
     """-C files (encoded on the device, one chunk per reference window; with
     per > 0 the windows of a batched span are encoded together) are the
@@ -404,3 +405,92 @@ def test_full_size_batched_equals_per_window(built, tmp_path):
     gsnp_amd.run(fa, so, b, dbsnp=snp, batched=0, read_length=rl, refine_mode=1)
     assert a.stat().st_size > 500_000_000
     assert filecmp.cmp(a, b, shallow=False), _first_diff(a, b)
+
+
+@pytest.mark.parametrize("name,per", [("c2w3", 2), ("c3s", 0), ("c5s", 64)])
+def test_synthetic_job_matches_oracle(datasets, tmp_path, name, per):
+    """A job over the data set generated in memory (gsnp_job_synth, the
+    benchmark's input path) writes the oracle's text for the files the same
+    (config, seed, scale) generates."""
+    import gsnp_amd
+    from conftest import DATASETS
+
+    ds = datasets(name)
+    cfg, seed, scale, _ = DATASETS[name]
+    flags = {k: v for k, v in ds.flags.items() if k != "read_length"}
+    print(f"[parity] in-memory {name} batched={per}", flush=True)
+    job = gsnp_amd.Job(synth=(cfg, seed, scale), **flags)
+    try:
+        assert job.read_len == ds.read_len
+        out = tmp_path / f"{name}.mem.cns"
+        job.write(out, batched=per)
+        ref = ds.oracle_text()
+        assert out.read_bytes() == ref.read_bytes(), _first_diff(out, ref)
+    finally:
+        job.close()
+
+
+@pytest.mark.parametrize("name", ["c3s", "c5s"])
+def test_whole_pass_rows_reach_host_memory(datasets, name):
+    """gsnp_job_pass over every chromosome: the rows copied to host memory
+    (batched spans through the overlapped copy, and the per-window path with
+    one copy per window) add up to the oracle's text, and the rank-sum sites
+    finished on the host are counted (printed for the record)."""
+    import gsnp_amd
+
+    ds = datasets(name)
+    size = ds.oracle_text().stat().st_size
+    job = gsnp_amd.Job(ds.fasta, ds.soap, dbsnp=ds.dbsnp, **ds.flags)
+    try:
+        sites = sum(job.info(i)[1] for i in range(len(job)))
+        job.counters(reset=True)
+        for per, mode in ((-1, 2), (4, 2), (0, 2), (-1, 1)):
+            nb, ns = job.run_pass(batched=per, emit_text=mode)
+            assert (nb, ns) == (size, sites), (per, mode, nb, ns)
+            c = job.counters(reset=True)
+            print(f"[parity] {name} pass batched={per} text={mode}: {c}", flush=True)
+            assert c["records"] > 0 and c["host_fixups"] >= c["host_fixups_normal"]
+    finally:
+        job.close()
+
+
+REGIONS = {
+    # c2w3 (one 600 kb chromosome, windows 0-2): a region inside window 0, one
+    # whose read-length extension crosses into window 1 (window 2 is not
+    # touched), and a line of an unknown chromosome (skipped)
+    "c2w3": "chr1\t100\t200\nchr1\t262100\t262200\nchrZ\t1\t10\n",
+    # stress (chr1 560 kb, chr2 300 kb, chr3 90 kb): window 1 of chr1 only,
+    # window 0 of chr3 (its start clipped at 0), nothing on chr2
+    "stress": "chr1\t300000\t300050\nchr3\t10\t20\n",
+}
+
+
+@pytest.mark.parametrize("name,per,compress", [("c2w3", 0, False), ("c2w3", 4, False), ("stress", 0, False),
+                                               ("stress", 2, False), ("c2w3", 0, True), ("stress", 2, True)])
+def test_regions_match_oracle(datasets, tmp_path, name, per, compress):
+    """-T target regions: only the windows a region touches are called and
+    written (text or -C chunks), each from every read that covers it."""
+    import gsnp_amd
+
+    ds = datasets(name)
+    reg = tmp_path / "regions.txt"
+    reg.write_text(REGIONS[name])
+    print(f"[parity] -T {name} batched={per} compress={compress}", flush=True)
+    gpu, cpu = tmp_path / "gpu", tmp_path / "cpu"
+    job = gsnp_amd.Job(ds.fasta, ds.soap, dbsnp=ds.dbsnp, **ds.flags)
+    try:
+        job.set_regions(reg)
+        job.write(gpu, compress=compress, batched=per)
+    finally:
+        job.close()
+    oracle_lib.run(ds.fasta, ds.soap, cpu, dbsnp=ds.dbsnp, compress=compress, regions=reg, **ds.flags)
+    if compress:
+        for sfx in SUFFIXES:
+            assert filecmp.cmp(str(gpu) + sfx, str(cpu) + sfx, shallow=False), f"{sfx} differs"
+        assert Path(str(gpu) + ".1_2").read_text().splitlines()[0] == "2"  # two windows touched
+    else:
+        a, b = gpu.read_bytes(), cpu.read_bytes()
+        assert a == b, _first_diff(gpu, cpu)
+        # the called windows are whole: every row belongs to a touched window
+        full = ds.oracle_text().read_bytes().splitlines(keepends=True)
+        assert 0 < len(a.splitlines()) < len(full)

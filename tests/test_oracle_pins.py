@@ -397,3 +397,25 @@ def test_two_bit_codec():
     out = ctypes.create_string_buffer(17)
     o.orc_decode_2bit(out, 17, code, 2)
     assert out.raw == seq
+
+
+# ------------------------------------------------- window-parallel oracle run
+def test_threaded_run_equals_serial_run(tmp_path):
+    """The CPU baseline's window-parallel run (-J) writes the serial run's text:
+    each window is called from every read that covers it, independently."""
+    import subprocess
+
+    import gsnp_amd
+
+    pre = tmp_path / "c2"
+    gsnp_amd.synth(2, 12, 0.06, pre)  # 600 kb: three windows
+    cli = str(oracle_lib.ORACLE_CLI)
+    base = ["-i", f"{pre}.soap", "-d", f"{pre}.fa", "-s", f"{pre}.snp", "-2", "-u", "-L", "100"]
+    subprocess.run([cli] + base + ["-o", str(tmp_path / "a.cns")], check=True, capture_output=True)
+    subprocess.run([cli] + base + ["-J", "3", "-o", str(tmp_path / "b.cns")], check=True, capture_output=True)
+    subprocess.run([cli] + base + ["-J", "2", "-X", "300000", "-o", str(tmp_path / "c.cns")], check=True,
+                   capture_output=True)
+    a = (tmp_path / "a.cns").read_bytes()
+    assert (tmp_path / "b.cns").read_bytes() == a
+    # a bounded sample stops at the first window boundary at or past the bound
+    assert (tmp_path / "c.cns").read_bytes() == b"".join(a.splitlines(keepends=True)[:2 * 262144])
