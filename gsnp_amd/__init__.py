@@ -116,6 +116,7 @@ SIGNATURES = {
     "gsnp_dict_compress_i32": (ctypes.c_int, [_P, _P, ctypes.c_uint64, _P, ctypes.POINTER(ctypes.c_uint64), _P,
                                               ctypes.POINTER(ctypes.c_uint64)]),
     "gsnp_sort_likelihood": (ctypes.c_int, [_P, _P, ctypes.c_uint32, _P, ctypes.c_uint32, _P]),
+    "gsnp_sort_likelihood_noalloc": (ctypes.c_int, [_P, _P, _P, ctypes.c_uint32, _P]),
     "gsnp_format_g": (ctypes.c_int, [ctypes.c_double, ctypes.c_char_p]),
     "gsnp_profile": (ctypes.c_int, [_P, ctypes.c_int, _P, _P, ctypes.c_int]),
     "gsnp_stats": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_uint64), ctypes.c_int]),

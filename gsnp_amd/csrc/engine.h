@@ -232,6 +232,7 @@ struct TextPipe {
 
 struct Ctx {
     gsnp_params p{};
+    int device = 0;  // HIP device current when the context was created
     hipStream_t stream = nullptr;
     StageTimer timer;
     // tables
@@ -347,6 +348,8 @@ struct Ctx {
 };
 
 int create_ctx(const gsnp_params* p, const double* p_matrix, gsnp_ctx** out);
+// device address of the fixed working array of gsnp_sort_likelihood_noalloc
+int noalloc_scratch(int device, uint32_t** out);
 
 }  // namespace gsnp
 

@@ -336,6 +336,7 @@ int create_ctx(const gsnp_params* p, const double* p_matrix, gsnp_ctx** out)
         return fail(GSNP_E_ARG, "quality characters out of range");
     std::unique_ptr<gsnp_ctx> c(new gsnp_ctx());
     c->p = *p;
+    HIPCHK(hipGetDevice(&c->device));
     HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     // adjusted qualities, priors, rank table (the score table: set_matrix)
     QadjTable qt = build_qadj_table(p->pcr_dependency, p->global_dependency);
@@ -395,7 +396,8 @@ const char* gsnp_last_error(void) { return g_err.c_str(); }
 int gsnp_device_init(int device)
 {
     HIPCHK(hipSetDevice(device));
-    return GSNP_OK;
+    uint32_t* scratch = nullptr;  // per-device address of the no-allocation entry point's array
+    return noalloc_scratch(device, &scratch);
 }
 
 int gsnp_create(const gsnp_params* p, const double* p_matrix, gsnp_ctx** out)

@@ -232,6 +232,22 @@ int gsnp_dict_compress_i32(gsnp_ctx* ctx, const int32_t* in, uint64_t n, uint32_
 int gsnp_sort_likelihood(gsnp_ctx* ctx, uint32_t* records, uint32_t pitch, const uint32_t* counts,
                          uint32_t n_sites, double* lk10);
 
+/* No counterpart in the reference.  gsnp_sort_likelihood on caller-owned
+ * device memory, for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc): it calls no
+ * hipMalloc, hipFree or other allocating routine.  Its working memory (depth
+ * class lists, class counters, an error word) is a fixed device array of
+ * 6 MiB + 256 B per GPU, held until the process ends; its device address is
+ * looked up by gsnp_device_init (or by the first call on a device).
+ * records: device, n_sites rows of 256 u32; the first counts[i] records of
+ * row i are sorted in place.  counts: device, n_sites int32 in [0, 256].
+ * lk10: device, 10 doubles per site.  n_sites <= GSNP_WIN (262144), else
+ * GSNP_E_ARG.  Runs on the context's stream and returns once it has drained.
+ * Every call on a GPU shares that GPU's array: calls that overlap on one GPU
+ * (two host threads, two contexts) race and must be serialised by the caller. */
+int gsnp_sort_likelihood_noalloc(gsnp_ctx* ctx, uint32_t* records, const int32_t* counts, uint32_t n_sites,
+                                 double* lk10);
+
 /* Per-stage device time from HIP events recorded on the context's stream:
  * returns (and resets) the accumulated milliseconds and launch counts of the
  * stages site build, sort, likelihood, posterior, rank-sum, format, recycle,
