@@ -112,6 +112,7 @@ SIGNATURES = {
     "gsnp_run": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p,
                                 ctypes.POINTER(Params), ctypes.c_char_p, ctypes.c_int, ctypes.c_int]),
     "gsnp_decompress": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p]),
+    "gsnp_decompress_device": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p]),
     "gsnp_matrix_gen": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(Params), _P]),
     "gsnp_dict_compress_i32": (ctypes.c_int, [_P, _P, ctypes.c_uint64, _P, ctypes.POINTER(ctypes.c_uint64), _P,
                                               ctypes.POINTER(ctypes.c_uint64)]),
@@ -205,8 +206,14 @@ def run(fasta, soap, out, dbsnp=None, matrix_in=None, matrix_out=None, compress=
                      int(bool(compress)), int(batched)))
 
 
-def decompress(prefix, out) -> None:
-    check(lib().gsnp_decompress(_b(prefix), _b(out)))
+def decompress(prefix, out, device: bool = False) -> None:
+    """-C files back to .cns text, on the host or (device=True) on the GPU."""
+    L = lib()
+    if device:
+        check(L.gsnp_device_init(0))
+        check(L.gsnp_decompress_device(_b(prefix), _b(out)))
+    else:
+        check(L.gsnp_decompress(_b(prefix), _b(out)))
 
 
 class CnsWindow(ctypes.Structure):

@@ -22,7 +22,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("GSNP_OFFLOAD_ARCH", "gfx950")
 
 LIB_SOURCES = ["tables.cpp", "inputs.cpp", "cwriter.cpp", "synth.cpp", "kernels.hip", "dict.hip", "cols.hip", "calib.hip", "engine.hip", "run.hip",
-               "adjoint.hip", "noalloc.hip"]
+               "adjoint.hip", "noalloc.hip", "cdec.hip"]
 TOOLS = {"gsnp": "cli_main.cpp", "gsnp-decompress": "decompress_main.cpp"}
 
 # -ffp-contract=off everywhere: FP64 sums and comparisons must round exactly as

@@ -186,6 +186,11 @@ int gsnp_run(const char* fasta, const char* soap, const char* dbsnp, const char*
 
 /* -C files back to .cns text.  Replaces decompressAll, cpp/decompression.cpp:130-255. */
 int gsnp_decompress(const char* prefix, const char* out);
+/* The same on the current HIP device: the host reads and checks each
+ * window's chunks; the device rebuilds the 17 columns (2-bit words,
+ * dictionary indices, run lengths, sparse lists) and writes the rows, up to
+ * 16 windows per upload. */
+int gsnp_decompress_device(const char* prefix, const char* out);
 
 /* -C files read in memory one window at a time, as the 17 .cns columns.
  * Replaces CNSDataWindow and the read_decompress loop of decompressAll,

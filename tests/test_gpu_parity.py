@@ -494,3 +494,20 @@ def test_regions_match_oracle(datasets, tmp_path, name, per, compress):
         # the called windows are whole: every row belongs to a touched window
         full = ds.oracle_text().read_bytes().splitlines(keepends=True)
         assert 0 < len(a.splitlines()) < len(full)
+
+
+@pytest.mark.parametrize("name", ["stress", "c2x2", "c5s", "c3s"])
+def test_device_decompression_matches_oracle(datasets, tmp_path, name):
+    """-C files decompressed on the device (gsnp_decompress_device: columns
+    rebuilt and rows written by kernels) give the oracle's text; c2x2 ends
+    with an empty window, stress has all-N windows, known SNPs and rank sums."""
+    import gsnp_amd
+
+    ds = datasets(name)
+    pre = tmp_path / name
+    oracle_lib.run(ds.fasta, ds.soap, pre, dbsnp=ds.dbsnp, compress=True, **ds.flags)
+    print(f"[parity] device decompression {name}", flush=True)
+    out = tmp_path / f"{name}.dev.cns"
+    gsnp_amd.decompress(pre, out, device=True)
+    ref = ds.oracle_text()
+    assert out.read_bytes() == ref.read_bytes(), _first_diff(out, ref)
