@@ -880,7 +880,8 @@ __global__ __launch_bounds__(512) void k_fused(FusedArgs f, const uint32_t* list
     const uint32_t ngroups = (count + G - 1) / G;
     const uint32_t wave = blockIdx.x * (blockDim.x >> 6) + wib;
     const uint32_t nwave = gridDim.x * (blockDim.x >> 6);
-    const uint32_t td = (uint32_t)f.t.td, gd = (uint32_t)f.t.gd;
+    // table extents are at most 256 (the masks let the index use 24-bit multiplies)
+    const uint32_t td = (uint32_t)f.t.td & 511u, gd = (uint32_t)f.t.gd & 511u;
     // phase-2 roles: lane = 5 * site slot + 16-byte piece of the row (lanes 60..63 idle)
     const uint32_t sl = lane / 5u, cl = lane - 5u * sl;
     const char* score = reinterpret_cast<const char*>(f.t.score);
@@ -900,11 +901,13 @@ __global__ __launch_bounds__(512) void k_fused(FusedArgs f, const uint32_t* list
             const uint64_t gmask = (S == 64 ? ~0ull : ((1ull << S) - 1ull)) << gbase;  // lanes of my group
             const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
             const uint64_t upto = below | (1ull << lane);
+            uint32_t qf_min = 63u, qa_min = 255u;  // out-of-table qualities: either reaches 0
             // in chunks of CH passes (bounds the registers)
-            constexpr uint32_t CH = NP < 16 ? NP : 16;
+            constexpr uint32_t CH = NP < 16 ? NP : S == 64 ? 8 : 16;  // 64: fewer registers
 #pragma unroll
             for (uint32_t k0 = 0; k0 < NP; k0 += CH) {
                 uint32_t v[CH];
+                uint32_t pad = 0;  // bit k: my slot of pass k is padding
 #pragma unroll
                 for (uint32_t k = 0; k < CH; k++) {
                     const uint32_t gi = (k0 + k) * P + sub;
@@ -916,41 +919,67 @@ __global__ __launch_bounds__(512) void k_fused(FusedArgs f, const uint32_t* list
                 for (uint32_t k = 0; k < CH; k += 2) bitonic_all2<2, S>(v[k], v[k + 1], e);  // CH is even
 #pragma unroll
                 for (uint32_t k = 0; k < CH; k++) {
-                    // the lanes of my site that agree with me on all eleven key
-                    // bits (one ballot per bit, 32-bit halves, the ballot xor my
-                    // bit replicated: four VALU per bit), below me; and those
+                    // the lanes of my site that agree with me on every key bit
+                    // (one ballot per bit, 32-bit halves), below me; and those
                     // with my base, up to me; padding lanes (all ones: a record's
-                    // occurrence byte is 1) take no part
-                    const bool my = v[k] != 0xFFFFFFFFu;
+                    // occurrence byte is 1) take no part.  Per bit: a signed
+                    // one-bit extract (all ones where my bit is set), the compare
+                    // that makes the ballot, and per half one three-input
+                    // operation that adds the lanes whose bit differs from mine
+                    // (d | (ballot ^ m)) to the difference mask.  The two base
+                    // bits go first, so that the base mask is the key mask's
+                    // start.
+                    const uint32_t w = v[k];
+                    const bool my = w != 0xFFFFFFFFu;
                     const uint64_t live = __ballot(my) & gmask;
-                    uint32_t plo = (uint32_t)(live & below), phi = (uint32_t)((live & below) >> 32);
-                    uint32_t olo = (uint32_t)(live & upto), ohi = (uint32_t)((live & upto) >> 32);
+                    uint32_t dlo = 0, dhi = 0, olo = 0, ohi = 0;
 #pragma unroll
                     for (int b = 0; b < 11; b++) {
-                        const uint32_t bit = b == 0 ? 8u : b < 9 ? (uint32_t)(8 + b) : (uint32_t)(14 + b);  // 8, 9..16, 23, 24
-                        const uint32_t m = 0u - ((v[k] >> bit) & 1u);  // all ones where my bit is set
-                        const uint64_t bal = __ballot(m != 0u);
-                        const uint32_t eql = ~((uint32_t)bal ^ m), eqh = ~((uint32_t)(bal >> 32) ^ m);
-                        plo &= eql;  // lanes whose bit equals mine
-                        phi &= eqh;
-                        if (b >= 9) {
-                            olo &= eql;
-                            ohi &= eqh;
+                        const uint32_t bit = b < 2 ? 23u + (uint32_t)b : b == 2 ? 8u : (uint32_t)(6 + b);  // 23, 24, 8, 9..16
+                        const uint32_t m = (uint32_t)__builtin_amdgcn_sbfe((int)w, bit, 1u);
+                        const uint64_t bal = __ballot((int)m < 0);
+                        dlo = __builtin_amdgcn_bitop3_b32(dlo, (uint32_t)bal, m, 0xF6);
+                        dhi = __builtin_amdgcn_bitop3_b32(dhi, (uint32_t)(bal >> 32), m, 0xF6);
+                        if (b == 1) {
+                            olo = (uint32_t)(live & upto) & ~dlo;  // my base, up to me
+                            ohi = (uint32_t)((live & upto) >> 32) & ~dhi;
                         }
                     }
+                    const uint32_t plo = (uint32_t)(live & below) & ~dlo, phi = (uint32_t)((live & below) >> 32) & ~dhi;
                     const uint32_t t = (uint32_t)(__popc(plo) + __popc(phi));
-                    const uint64_t first = __ballot(my && t == 0);
+                    // first occurrences; olo/ohi hold live lanes only
+                    const uint64_t first = __ballot(t == 0);
                     const uint32_t g =
                         (uint32_t)(__popc(olo & (uint32_t)first) + __popc(ohi & (uint32_t)(first >> 32))) - 1u;
-                    // (all ones stays free: t - 1 = 255 and g = 255 exclude each other)
-                    uint32_t b2 = 0;
-                    v[k] = my ? qadj_slot(v[k], t, g, td, gd, b2) : 0xFFFFFFFFu;
-                    bad |= b2;
+                    // (q, t - 1, g) table index << 10 | cycle << 2 | base;
+                    // padding lanes take a q 63 record's
+                    // (an index inside the table; their row is replaced below).
+                    // A quality field of 0 (q = 64, beyond the table) is
+                    // remembered in qf_min and clamped to q 63.
+                    const uint32_t x = my ? w : 0x00020001u;
+                    const uint32_t qf = (x >> 17) & 63u;  // 64 - q
+                    qf_min = qf < qf_min ? qf : qf_min;
+                    const uint32_t q = 64u - (qf > 1u ? qf : 1u);
+                    const uint32_t tm1 = t < td ? t : td - 1u;
+                    const uint32_t gq = g < gd ? g : gd - 1u;
+                    const uint32_t idx = (q * td + tm1) * gd + gq;  // < 2^22
+                    v[k] = (idx << 10) | ((x >> 7) & 0x3FCu) | ((x >> 23) & 3u);
+                    pad |= my ? 0u : 1u << k;
                 }
+                // score rows into LDS; padding slots [n, S) get row 0, whose
+                // ten values are +0.0, so that phase 2 adds whole groups of
+                // eight slots with no per-slot test (a sum that starts at +0.0
+                // is unchanged by adding +0.0)
 #pragma unroll
-                for (uint32_t k = 0; k < CH; k++)
-                    if (v[k] != 0xFFFFFFFFu) srt[((k0 + k) * P + sub) * RS + e] = score_row(qtab, v[k], bad);
+                for (uint32_t k = 0; k < CH; k++) {
+                    const bool my = !((pad >> k) & 1u);
+                    const uint32_t qa = qtab[v[k] >> 10];  // 0 marks a quality beyond the table
+                    qa_min = my && qa < qa_min ? qa : qa_min;
+                    const uint32_t row = ((qa > 1u ? qa : 1u) << 10) | (v[k] & 1023u);
+                    srt[((k0 + k) * P + sub) * RS + e] = my ? row : 0u;
+                }
             }
+            if (qf_min == 0u || qa_min == 0u) bad |= kErrQadjRange;
         } else {
             constexpr int R = S / 64;
             uint32_t v[G][R];
@@ -975,8 +1004,9 @@ __global__ __launch_bounds__(512) void k_fused(FusedArgs f, const uint32_t* list
                 uint32_t slot[R];
                 wide_slots<R>(v[gi], below, upto, td, gd, bad, slot);
 #pragma unroll
-                for (int r = 0; r < R; r++)
-                    if (slot[r] != 0xFFFFFFFFu) srt[gi * RS + (uint32_t)r * 64u + lane] = score_row(f.t.qadj, slot[r], bad);
+                for (int r = 0; r < R; r++)  // padding slots: row 0, all +0.0
+                    srt[gi * RS + (uint32_t)r * 64u + lane] =
+                        slot[r] != 0xFFFFFFFFu ? score_row(f.t.qadj, slot[r], bad) : 0u;
             }
         }
         wave_lds_sync();
@@ -994,20 +1024,17 @@ __global__ __launch_bounds__(512) void k_fused(FusedArgs f, const uint32_t* list
             const uint32_t* rw = srt + (on ? gi : 0u) * RS;
             double a0 = 0.0, a1 = 0.0;
             // i + u < i + U <= S rounded up to U: inside the row (RS > S, RS >= 21);
-            // slots past the site's records load row 0 and are not added
+            // slots past the site's records hold row 0, all +0.0
             for (uint32_t i = 0; i < m; i += U) {
                 double2 x[U];
 #pragma unroll
-                for (uint32_t u = 0; u < U; u++) {
-                    const uint32_t ri = rw[i + u];
-                    x[u] = *reinterpret_cast<const double2*>(score + (((i + u < m ? ri : 0u) << 7) | (cl << 4)));
-                }
-#pragma unroll
                 for (uint32_t u = 0; u < U; u++)
-                    if (i + u < m) {
-                        a0 += x[u].x;
-                        a1 += x[u].y;
-                    }
+                    x[u] = *reinterpret_cast<const double2*>(score + ((rw[i + u] << 7) | (cl << 4)));
+#pragma unroll
+                for (uint32_t u = 0; u < U; u++) {
+                    a0 += x[u].x;
+                    a1 += x[u].y;
+                }
             }
             wave_lds_sync();
             if (on) {

@@ -27,6 +27,10 @@ std::vector<double> build_score_table(const double* pm)
             }
         }
     }
+    // Row 0 (adjusted quality 0, cycle 0, base A) is never a record's row:
+    // adjusted qualities are at least 1.  It holds +0.0, the value the fused
+    // kernel adds for the padding slots of a site.
+    for (uint32_t g = 0; g < kScoreStride; g++) s[g] = 0.0;
     return s;
 }
 
