@@ -107,6 +107,9 @@ SIGNATURES = {
     "gsnp_fetch_text": (ctypes.c_int, [_P, ctypes.c_char_p, ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint64)]),
     "gsnp_window_results": (ctypes.c_int, [_P, ctypes.c_uint32, ctypes.POINTER(WindowOut)]),
     "gsnp_window_recycle": (ctypes.c_int, [_P]),
+    "gsnp_window_columns": (ctypes.c_int, [_P, ctypes.c_uint32, _P, _P, _P]),
+    "gsnp_set_stream": (ctypes.c_int, [_P, _P]),
+    "gsnp_job_columns_to": (ctypes.c_int, [_P, ctypes.c_int, _P, _P, ctypes.POINTER(ctypes.c_uint64)]),
     "gsnp_batched_call": (ctypes.c_int, [_P, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int,
                                          ctypes.POINTER(ctypes.c_uint64)]),
     "gsnp_run": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p,
@@ -115,6 +118,12 @@ SIGNATURES = {
     "gsnp_decompress_device": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p]),
     "gsnp_matrix_gen": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(Params), _P]),
     "gsnp_dict_compress_i32": (ctypes.c_int, [_P, _P, ctypes.c_uint64, _P, ctypes.POINTER(ctypes.c_uint64), _P,
+                                              ctypes.POINTER(ctypes.c_uint64)]),
+    "gsnp_dict_compress_char": (ctypes.c_int, [_P, _P, ctypes.c_uint64, _P, ctypes.POINTER(ctypes.c_uint64), _P,
+                                               ctypes.POINTER(ctypes.c_uint64)]),
+    "gsnp_dict_compress_u32": (ctypes.c_int, [_P, _P, ctypes.c_uint64, _P, ctypes.POINTER(ctypes.c_uint64), _P,
+                                              ctypes.POINTER(ctypes.c_uint64)]),
+    "gsnp_dict_compress_f64": (ctypes.c_int, [_P, _P, ctypes.c_uint64, _P, ctypes.POINTER(ctypes.c_uint64), _P,
                                               ctypes.POINTER(ctypes.c_uint64)]),
     "gsnp_sort_likelihood": (ctypes.c_int, [_P, _P, ctypes.c_uint32, _P, ctypes.c_uint32, _P]),
     "gsnp_sort_likelihood_noalloc": (ctypes.c_int, [_P, _P, _P, ctypes.c_uint32, _P]),
@@ -308,6 +317,24 @@ STAGES = ["site_build", "sort", "likelihood", "posterior", "rank", "format", "re
           "calib"]
 
 
+# gsnp_chunk_fn: (user, chromosome, 1-based start, sites, chunk pointers, chunk bytes) -> 0 to go on
+CHUNK_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_char_p, ctypes.c_uint64, ctypes.c_uint32,
+                            ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.c_uint64))
+
+
+def window_columns(ctx, call_length: int) -> list[bytes]:
+    """The five -C chunks of the window just called on `ctx`
+    (gsnp_window_columns): sizes first, then the bytes."""
+    L = lib()
+    sizes = (ctypes.c_uint64 * 5)()
+    check(L.gsnp_window_columns(ctx, int(call_length), None, None, sizes))
+    bufs = [ctypes.create_string_buffer(max(1, sizes[f])) for f in range(5)]
+    dst = (ctypes.c_void_p * 5)(*[ctypes.addressof(b) for b in bufs])
+    cap = (ctypes.c_uint64 * 5)(*[len(b) for b in bufs])
+    check(L.gsnp_window_columns(ctx, int(call_length), dst, cap, sizes))
+    return [bufs[f].raw[:sizes[f]] for f in range(5)]
+
+
 class Job:
     """A calling session over input files (gsnp_job_*): parse and calibrate
     once, then repeat calling passes on device-resident inputs."""
@@ -374,6 +401,24 @@ class Job:
         nb = ctypes.c_uint64()
         check(lib().gsnp_job_columns(self.h, int(batched), ctypes.byref(nb)))
         return nb.value
+
+    def column_chunks(self, batched: int = 0):
+        """A calling pass over the selected chromosome handing back every
+        window's -C chunks (gsnp_job_columns_to): a list of
+        (chromosome, 1-based start, sites, [five chunk byte strings])."""
+        out = []
+
+        def take(user, chr_, start, n, chunk, nbytes):
+            try:
+                out.append((chr_.decode(), start, n, [ctypes.string_at(chunk[f], nbytes[f]) for f in range(5)]))
+                return 0
+            except Exception:  # an exception must not cross the C frame
+                return 1
+
+        cb = CHUNK_FN(take)
+        nb = ctypes.c_uint64()
+        check(lib().gsnp_job_columns_to(self.h, int(batched), cb, None, ctypes.byref(nb)))
+        return out
 
     @property
     def ctx(self):

@@ -18,6 +18,7 @@ struct ColumnPass {
     uint64_t bytes[5];
     const uint64_t* starts;
     const uint32_t* sizes;
+    const uint64_t* offsets;  // [n_windows][5]: where window w's chunk starts in file f's bytes
     uint32_t n_windows;
 };
 using ColumnSink = std::function<int(const ColumnPass&)>;

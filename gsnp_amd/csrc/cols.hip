@@ -753,6 +753,7 @@ int encode_columns(Ctx& c, uint32_t offset, uint32_t ws_abs, uint32_t n, const C
         int buf = 0;
         std::vector<uint64_t> starts;
         std::vector<uint32_t> sizes;
+        std::vector<uint64_t> offsets;
     };
     Done prev;
     bool have_prev = false;
@@ -760,6 +761,7 @@ int encode_columns(Ctx& c, uint32_t offset, uint32_t ws_abs, uint32_t n, const C
         HIPCHK(hipEventSynchronize(b.ev_copied[d.buf]));
         d.pass.starts = d.starts.data();
         d.pass.sizes = d.sizes.data();
+        d.pass.offsets = d.offsets.data();
         d.pass.n_windows = (uint32_t)d.sizes.size();
         return sink(d.pass);
     };
@@ -809,7 +811,9 @@ int encode_columns(Ctx& c, uint32_t offset, uint32_t ws_abs, uint32_t n, const C
         c.timer.end(kStColumns, st);
         HIPCHK(hipGetLastError());
         uint64_t tot[5];
+        std::vector<uint64_t> offsets((size_t)nwin * 5);
         HIPCHK(hipMemcpyAsync(tot, b.ftot.p, sizeof tot, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipMemcpyAsync(offsets.data(), b.foff.p, 8ull * nwin * 5, hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
         // buffer `cur` was last copied from two passes ago, and that pass has
         // been handed over, so it is free
@@ -835,6 +839,7 @@ int encode_columns(Ctx& c, uint32_t offset, uint32_t ws_abs, uint32_t n, const C
         }
         Done d;
         d.buf = cur;
+        d.offsets.swap(offsets);
         uint64_t at = 0;
         for (int f = 0; f < 5; f++) {
             if (tot[f])

@@ -233,7 +233,8 @@ struct TextPipe {
 struct Ctx {
     gsnp_params p{};
     int device = 0;  // HIP device current when the context was created
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr;      // where the context's work is queued
+    hipStream_t own_stream = nullptr;  // the stream the context created (stream unless a caller's is set)
     StageTimer timer;
     // tables
     DevBuf<double> score, lprior, prank;

@@ -338,6 +338,7 @@ int create_ctx(const gsnp_params* p, const double* p_matrix, gsnp_ctx** out)
     c->p = *p;
     HIPCHK(hipGetDevice(&c->device));
     HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    c->own_stream = c->stream;
     // adjusted qualities, priors, rank table (the score table: set_matrix)
     QadjTable qt = build_qadj_table(p->pcr_dependency, p->global_dependency);
     c->td = qt.td;
@@ -416,9 +417,20 @@ void gsnp_destroy(gsnp_ctx* c)
 {
     if (!c) return;
     if (c->stream) hipStreamSynchronize(c->stream);
-    hipStream_t s = c->stream;
+    hipStream_t s = c->own_stream;
     delete c;
     if (s) hipStreamDestroy(s);
+}
+
+int gsnp_set_stream(gsnp_ctx* c, void* stream)
+{
+    if (!c) return fail(GSNP_E_ARG, "null context");
+    hipStream_t next = stream ? static_cast<hipStream_t>(stream) : c->own_stream;
+    if (next == c->stream) return GSNP_OK;
+    // the work queued so far ends before the next call's work starts
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->stream = next;
+    return GSNP_OK;
 }
 
 int gsnp_set_chromosome(gsnp_ctx* c, const char* name, const uint8_t* codes, uint32_t len, const uint32_t* snp_pos,

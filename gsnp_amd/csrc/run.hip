@@ -126,6 +126,32 @@ int call_chromosome(gsnp_ctx* ctx, const std::vector<int32_t>& pos, uint32_t N, 
     return GSNP_OK;
 }
 
+// The five chunks of a window with no sites (the window after a chromosome
+// that fills its last window, cpp/call_genotype.cc:636-644): an all-N
+// .3_4_6 header and the N indicator of the four other files.
+struct EmptyChunks {
+    uint64_t h346[7] = {1, 0, 0, 0, 0, 0, 0};
+    int32_t nind = 1;
+    const uint8_t* chunk[5];
+    uint64_t bytes[5] = {sizeof(h346), 4, 4, 4, 4};
+    EmptyChunks()
+    {
+        chunk[0] = reinterpret_cast<const uint8_t*>(h346);
+        for (int f = 1; f < 5; f++) chunk[f] = reinterpret_cast<const uint8_t*>(&nind);
+    }
+};
+
+// Window w's chunk in each file of an encoder pass.
+inline void pass_window(const ColumnPass& pass, uint32_t w, const uint8_t* chunk[5], uint64_t bytes[5])
+{
+    for (int f = 0; f < 5; f++) {
+        const uint64_t a = pass.offsets[(size_t)w * 5 + f];
+        const uint64_t b = w + 1 < pass.n_windows ? pass.offsets[(size_t)(w + 1) * 5 + f] : pass.bytes[f];
+        chunk[f] = pass.file[f] + a;
+        bytes[f] = b - a;
+    }
+}
+
 }  // namespace gsnp
 
 struct gsnp_cns_reader {
@@ -375,6 +401,74 @@ int gsnp_job_columns(gsnp_job* job, int batched, uint64_t* nbytes)
                                    job->mask(b.chr));
     if (nbytes) *nbytes = total;
     return rc;
+}
+
+int gsnp_job_columns_to(gsnp_job* job, int batched, gsnp_chunk_fn fn, void* user, uint64_t* nbytes)
+{
+    if (!job || job->cur < 0) return gsnp::fail(GSNP_E_ARG, "no chromosome selected");
+    if (!fn) return gsnp::fail(GSNP_E_ARG, "null callback");
+    return gsnp::guarded([&] {
+        const gsnp::ReadBlock& b = job->blocks[job->cur];
+        const std::string& name = job->g.chr[b.chr].name;
+        const uint32_t N = (uint32_t)job->g.chr[b.chr].code.size();
+        uint64_t total = 0;
+        auto hand = [&](uint64_t start, uint32_t n, const uint8_t* const* chunk, const uint64_t* bytes) -> int {
+            for (int f = 0; f < 5; f++) total += bytes[f];
+            const int r = fn(user, name.c_str(), start, n, chunk, bytes);
+            return r ? gsnp::fail(GSNP_E_IO, "the chunk callback returned " + std::to_string(r)) : GSNP_OK;
+        };
+        auto on_results = [&](uint32_t offset, uint32_t ws, uint32_t n) -> int {
+            if (n == 0) {
+                const gsnp::EmptyChunks e;
+                return hand((uint64_t)ws + 1, 0, e.chunk, e.bytes);
+            }
+            return gsnp::encode_columns(*job->ctx, offset, ws, n, [&](const gsnp::ColumnPass& pass) -> int {
+                for (uint32_t w = 0; w < pass.n_windows; w++) {
+                    const uint8_t* chunk[5];
+                    uint64_t bytes[5];
+                    gsnp::pass_window(pass, w, chunk, bytes);
+                    const int r = hand(pass.starts[w], pass.sizes[w], chunk, bytes);
+                    if (r) return r;
+                }
+                return GSNP_OK;
+            });
+        };
+        int rc = gsnp::call_chromosome(job->ctx, b.pos, N, batched, gsnp::kTextNone, on_results, nullptr, nullptr,
+                                       job->mask(b.chr));
+        if (nbytes) *nbytes = total;
+        return rc;
+    });
+}
+
+int gsnp_window_columns(gsnp_ctx* c, uint32_t call_length, uint8_t* const* dst, const uint64_t* cap, uint64_t* sizes)
+{
+    if (!c || !sizes) return gsnp::fail(GSNP_E_ARG, "null argument");
+    if (!c->win_cap) return gsnp::fail(GSNP_E_ARG, "window not initialised");
+    if (call_length > GSNP_WIN) return gsnp::fail(GSNP_E_ARG, "call length above the window size");
+    if (dst && !cap) return gsnp::fail(GSNP_E_ARG, "destinations without capacities");
+    return gsnp::guarded([&] {
+        auto deliver = [&](const uint8_t* const* chunk, const uint64_t* bytes) -> int {
+            for (int f = 0; f < 5; f++) sizes[f] = bytes[f];
+            if (!dst) return GSNP_OK;
+            for (int f = 0; f < 5; f++)
+                if (bytes[f] && (!dst[f] || cap[f] < bytes[f]))
+                    return gsnp::fail(GSNP_E_ARG, "chunk of file " + std::to_string(f) + " needs " +
+                                                      std::to_string(bytes[f]) + " bytes");
+            for (int f = 0; f < 5; f++)
+                if (bytes[f]) std::memcpy(dst[f], chunk[f], bytes[f]);
+            return GSNP_OK;
+        };
+        if (call_length == 0) {
+            const gsnp::EmptyChunks e;
+            return deliver(e.chunk, e.bytes);
+        }
+        return gsnp::encode_columns(*c, 0, c->ws, call_length, [&](const gsnp::ColumnPass& pass) -> int {
+            const uint8_t* chunk[5];
+            uint64_t bytes[5];
+            gsnp::pass_window(pass, 0, chunk, bytes);
+            return deliver(chunk, bytes);
+        });
+    });
 }
 
 int gsnp_job_synth(int config, unsigned long long seed, double scale, const gsnp_params* p, const char* matrix_in,

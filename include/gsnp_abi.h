@@ -117,6 +117,15 @@ int gsnp_calib_count(gsnp_ctx* ctx, uint64_t* counts);
 /* Replaces GSNPKernelFinalize, kernel/gsnp_kernel.cu:36-42. */
 void gsnp_destroy(gsnp_ctx* ctx);
 
+/* Queue the context's device work on the caller's HIP stream (a
+ * hipStream_t, not the null stream), so that its calls are ordered against
+ * the caller's own work and two contexts can run side by side; NULL goes back
+ * to the stream the context created.  Work already queued is finished first.
+ * Entry points that hand results to host memory still synchronize that stream
+ * before they return.  The reference runs on CUDA's default stream and has no
+ * counterpart (cpp/gsnp.h:375-516 takes no stream). */
+int gsnp_set_stream(gsnp_ctx* ctx, void* stream);
+
 /* Reference sequence of the chromosome to call, as 4-bit codes per base
  * (cpp/soap_snp.h:763-789: bit3 known SNP, bit2 N, bits1:0 A0 C1 T2 G3), and
  * its known-SNP entries (sorted positions).  Replaces the per-window host
@@ -152,6 +161,16 @@ int gsnp_window_text(gsnp_ctx* ctx, uint32_t call_length, uint64_t* nbytes);
 
 /* Copy the last formatted rows (gsnp_window_text / gsnp_batched_call) to host. */
 int gsnp_fetch_text(gsnp_ctx* ctx, char* dst, uint64_t cap, uint64_t* nbytes);
+
+/* The -C chunks of the window just called (gsnp_window_posterior), encoded
+ * on the device: sizes[f] receives the bytes of file f's chunk (.3_4_6,
+ * .5_7_8_9_14_16, .10, .11_12_13, .15_17) and, with dst non-NULL, dst[f]
+ * (cap[f] bytes) the chunk itself; a buffer too small fails with GSNP_E_ARG
+ * after sizes is filled.  call_length 0 gives the chunks of an empty window.
+ * The window's .1_2 row is (chromosome, start + 1, call_length).  Replaces
+ * the -C branch of Call_win::call_cns2, cpp/call_genotype.cc:312-352. */
+int gsnp_window_columns(gsnp_ctx* ctx, uint32_t call_length, uint8_t* const* dst, const uint64_t* cap,
+                        uint64_t* sizes);
 
 /* Copy per-site results of the last called window to host. */
 int gsnp_window_results(gsnp_ctx* ctx, uint32_t call_length, const gsnp_window_out* out);
@@ -229,6 +248,17 @@ int gsnp_matrix_gen(const char* fasta, const char* soap, const char* dbsnp, cons
  * (int overload), cpp/dict_compression.h:26-31.  words needs n entries, dict n. */
 int gsnp_dict_compress_i32(gsnp_ctx* ctx, const int32_t* in, uint64_t n, uint32_t* words, uint64_t* n_words,
                            int32_t* dict, uint64_t* n_dict);
+/* The char, uint32 and double overloads of cuda_dictCompression,
+ * cpp/dict_compression.h:26-37 (kernel/compress_kernel.cu:270-334): the
+ * same layout, the dictionary in the value type's order (char signed).  For
+ * double, +0.0 and -0.0 are one value (the dictionary holds +0.0 unless the
+ * input has only -0.0) and NaN is rejected (GSNP_E_ARG).  n <= 2^30. */
+int gsnp_dict_compress_char(gsnp_ctx* ctx, const char* in, uint64_t n, uint32_t* words, uint64_t* n_words,
+                            char* dict, uint64_t* n_dict);
+int gsnp_dict_compress_u32(gsnp_ctx* ctx, const uint32_t* in, uint64_t n, uint32_t* words, uint64_t* n_words,
+                           uint32_t* dict, uint64_t* n_dict);
+int gsnp_dict_compress_f64(gsnp_ctx* ctx, const double* in, uint64_t n, uint32_t* words, uint64_t* n_words,
+                           double* dict, uint64_t* n_dict);
 
 /* Segmented sort + likelihood on caller-supplied records: n_sites rows of
  * `pitch` u32 records, counts[i] valid in row i; writes 10 log-likelihoods
@@ -304,6 +334,16 @@ int gsnp_job_pass(gsnp_job* job, int batched, int emit_text, uint64_t* nbytes, u
  * extraction and compression of Call_win::call_cns2 with -C,
  * cpp/compress.h:33-1312. */
 int gsnp_job_columns(gsnp_job* job, int batched, uint64_t* nbytes);
+/* The same pass handing every window's five chunks to fn, in window order:
+ * chunk[f] / bytes[f] for the files .3_4_6, .5_7_8_9_14_16, .10, .11_12_13,
+ * .15_17, and the window's .1_2 row (chr, 1-based start, n_sites).  The
+ * chunks are valid during the call only; a nonzero return from fn ends the
+ * pass with GSNP_E_IO.  *nbytes sums the chunk bytes.  Replaces the -C
+ * branch of Call_win::call_cns2 (cpp/call_genotype.cc:312-352) for a whole
+ * chromosome. */
+typedef int (*gsnp_chunk_fn)(void* user, const char* chr, uint64_t start, uint32_t n_sites,
+                             const uint8_t* const* chunk, const uint64_t* bytes);
+int gsnp_job_columns_to(gsnp_job* job, int batched, gsnp_chunk_fn fn, void* user, uint64_t* nbytes);
 /* Every chromosome of the job called and written to `out` (.cns text) or,
  * with compress, to the -C files <out>.1_2 ... <out>.15_17; batched as in
  * gsnp_run.  The calling and output half of main (cpp/main.cc:391-406). */

@@ -28,6 +28,24 @@ unsigned long ref_dict_encode_f64(const double* in, unsigned long n, unsigned in
     return nw;
 }
 
+unsigned long ref_dict_encode_char(const char* in, unsigned long n, unsigned int* words, char* dict,
+                                   unsigned long* n_dict)
+{
+    uint64 nw = n, nd = n;
+    dictCompression(words, &nw, dict, &nd, in, n);
+    *n_dict = nd;
+    return nw;
+}
+
+unsigned long ref_dict_encode_u32(const unsigned int* in, unsigned long n, unsigned int* words, unsigned int* dict,
+                                  unsigned long* n_dict)
+{
+    uint64 nw = n, nd = n;
+    dictCompression(words, &nw, dict, &nd, in, n);
+    *n_dict = nd;
+    return nw;
+}
+
 void ref_dict_decode_i32(int* out, unsigned long n, const unsigned int* words, unsigned long nw, const int* dict,
                          unsigned long nd)
 {

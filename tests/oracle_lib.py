@@ -136,6 +136,10 @@ def dictref() -> ctypes.CDLL | None:
         L.ref_dict_encode_i32.argtypes = [P, u64, P, P, ctypes.POINTER(u64)]
         L.ref_dict_encode_f64.restype = u64
         L.ref_dict_encode_f64.argtypes = [P, u64, P, P, ctypes.POINTER(u64)]
+        L.ref_dict_encode_char.restype = u64
+        L.ref_dict_encode_char.argtypes = [P, u64, P, P, ctypes.POINTER(u64)]
+        L.ref_dict_encode_u32.restype = u64
+        L.ref_dict_encode_u32.argtypes = [P, u64, P, P, ctypes.POINTER(u64)]
         L.ref_dict_decode_i32.restype = None
         L.ref_dict_decode_i32.argtypes = [P, u64, P, u64, P, u64]
         _ref = L

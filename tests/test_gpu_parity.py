@@ -303,6 +303,81 @@ def test_sort_likelihood_unit(built):
         L.gsnp_destroy(ctx)
 
 
+def _unpack(words, n, nd):
+    """Ranks from MSB-first packed words (cpp/dict_compression.h:116-166):
+    floor(32 / bits) per word, the last word right-aligned."""
+    bits = max(1, int(np.ceil(np.log(float(nd)) / np.log(2.0)))) if nd > 1 else 1
+    per = 32 // bits
+    out = []
+    nw = (n + per - 1) // per
+    for i in range(nw):
+        m = n - per * i if i == nw - 1 else per
+        for k in range(m):
+            out.append((int(words[i]) >> (bits * (m - k - 1))) & ((1 << bits) - 1))
+    return np.array(out, np.int64)
+
+
+@pytest.mark.parametrize("kind", ["char", "u32", "f64"])
+def test_device_dictionary_overloads_match_reference_codec(built, kind):
+    """The char, uint32 and double overloads of cuda_dictCompression
+    (cpp/dict_compression.h:26-37) against the reference's own template
+    compiled from its header, and against what the format fixes: the
+    dictionary is the sorted distinct values and the ranks index it."""
+    import gsnp_amd
+
+    L = gsnp_amd.lib()
+    ref = oracle_lib.dictref()
+    p = gsnp_amd.params(read_length=100)
+    pm = np.full(256 * 256 * 16, 0.5)
+    ctx = ctypes.c_void_p()
+    gsnp_amd.check(L.gsnp_device_init(0))
+    gsnp_amd.check(L.gsnp_create(ctypes.byref(p), pm.ctypes.data, ctypes.byref(ctx)))
+    rng = np.random.default_rng(11)
+    dt = {"char": np.int8, "u32": np.uint32, "f64": np.float64}[kind]
+    if kind == "char":
+        cases = [rng.integers(-128, 128, n).astype(np.int8) for n in (1, 5, 300, 70000)]
+        cases.append(np.frombuffer(b"ACGTNACGTTTN" * 50, np.int8).copy())
+    elif kind == "u32":
+        cases = [rng.integers(0, 2**32, n, dtype=np.uint64).astype(np.uint32) for n in (1, 7, 3000)]
+        cases += [rng.integers(0, 300, 70000).astype(np.uint32), np.array([2**32 - 1, 0, 2**31, 2**31 - 1] * 9, np.uint32)]
+    else:
+        cases = [np.round(rng.normal(0, 50, n), 1) for n in (1, 9, 5000)]
+        cases += [np.array([1.0, 255.0, 0.5, -3.25, 1e-300, -1e300, np.inf, -np.inf, 0.0] * 7),
+                  rng.choice(np.array([0.125, 1.0, 2.5, 1e9]), 40000), np.array([-0.0, 1.0, -0.0])]
+    fn = getattr(L, f"gsnp_dict_compress_{kind}")
+    try:
+        for v in cases:
+            v = np.ascontiguousarray(v.astype(dt))
+            n = len(v)
+            w = np.zeros(n + 1, np.uint32)
+            d = np.zeros(n + 1, dt)
+            nw, nd = ctypes.c_uint64(), ctypes.c_uint64()
+            gsnp_amd.check(fn(ctx, v.ctypes.data, n, w.ctypes.data, ctypes.byref(nw), d.ctypes.data, ctypes.byref(nd)))
+            dict_ = d[:nd.value]
+            assert np.array_equal(dict_, np.unique(v)), "dictionary is not the sorted distinct values"
+            assert np.array_equal(dict_[_unpack(w, n, nd.value)], v), "ranks do not index the values"
+            zs = np.signbit(v[v == 0]) if kind == "f64" else np.zeros(0, bool)
+            if zs.size and (zs.all() or not zs.any()):  # every zero of one sign: the dictionary keeps it
+                assert np.signbit(dict_[dict_ == 0]).tolist() == [bool(zs[0])]
+            if ref is not None:
+                w3 = np.zeros(n + 1, np.uint32)
+                d3 = np.zeros(n + 1, dt)
+                nd3 = ctypes.c_ulong()
+                nw3 = getattr(ref, f"ref_dict_encode_{kind}")(v.ctypes.data, n, w3.ctypes.data, d3.ctypes.data,
+                                                              ctypes.byref(nd3))
+                assert (nw3, nd3.value) == (nw.value, nd.value)
+                assert np.array_equal(w[:nw3], w3[:nw3])
+                assert d[:nd3.value].tobytes() == d3[:nd3.value].tobytes()
+        nan = np.array([1.0, np.nan])
+        if kind == "f64":
+            w = np.zeros(3, np.uint32)
+            d = np.zeros(3)
+            nw, nd = ctypes.c_uint64(), ctypes.c_uint64()
+            assert fn(ctx, nan.ctypes.data, 2, w.ctypes.data, ctypes.byref(nw), d.ctypes.data, ctypes.byref(nd)) != 0
+    finally:
+        L.gsnp_destroy(ctx)
+
+
 def test_device_dictionary_matches_reference_codec(built):
     """gsnp_dict_compress_i32 vs the reference's own dictCompression (and the oracle)."""
     import gsnp_amd
@@ -511,3 +586,85 @@ def test_device_decompression_matches_oracle(datasets, tmp_path, name):
     gsnp_amd.decompress(pre, out, device=True)
     ref = ds.oracle_text()
     assert out.read_bytes() == ref.read_bytes(), _first_diff(out, ref)
+
+
+def _assemble_cfiles(prefix: Path, windows) -> None:
+    """-C files from (chromosome, start, sites, five chunks) per window, in
+    the layout of cpp/compress.h: the chunks appended file by file, and the
+    .1_2 index (window count, then one chromosome/start/size row each)."""
+    with open(str(prefix) + ".1_2", "w") as f:
+        f.write(f"{len(windows)}\n")
+        for name, start, n, _ in windows:
+            f.write(f"{name}\t{start}\t{n}\n")
+    for k, sfx in enumerate(SUFFIXES[1:]):
+        with open(str(prefix) + sfx, "wb") as f:
+            for *_, chunks in windows:
+                f.write(chunks[k])
+
+
+@pytest.mark.parametrize("name", ["stress", "c2x2", "c5s"])
+def test_window_and_callback_columns_match_oracle(datasets, tmp_path, name):
+    """The -C chunks through the two chunk-level entry points: the window
+    driver (init, update, likelihood, posterior, gsnp_window_columns,
+    recycle per window, the structure of cpp/call_genotype.cc:474-644 with
+    the -C branch of call_cns2) and the whole-chromosome callback
+    (gsnp_job_columns_to, batched spans of 3 windows); both assembled into
+    files equal the oracle's -C files byte for byte.  c2x2 ends with an empty
+    window."""
+    import gsnp_amd
+
+    ds = datasets(name)
+    cpu = tmp_path / f"{name}.cpu"
+    oracle_lib.run(ds.fasta, ds.soap, cpu, dbsnp=ds.dbsnp, compress=True, **ds.flags)
+    job = gsnp_amd.Job(ds.fasta, ds.soap, ds.dbsnp, **ds.flags)
+    L = gsnp_amd.lib()
+    W = 262144
+    per_window, callback = [], []
+    for i in range(len(job)):
+        cname, N, n_reads = job.info(i)
+        print(f"[parity] window columns {name} {cname}", flush=True)
+        job.select(i)
+        ctx = job.ctx
+        gsnp_amd.check(L.gsnp_window_init(ctx, 0))
+        for w in range(N // W + 1):
+            C = W if w < N // W else N % W
+            # every read of the batch: the update keeps those starting in the window
+            gsnp_amd.check(L.gsnp_window_update(ctx, 0, n_reads, None))
+            gsnp_amd.check(L.gsnp_window_likelihood(ctx, C))
+            gsnp_amd.check(L.gsnp_window_posterior(ctx, C))
+            per_window.append((cname, w * W + 1, C, gsnp_amd.window_columns(ctx, C)))
+            gsnp_amd.check(L.gsnp_window_recycle(ctx))
+        callback += job.column_chunks(batched=3)
+    for label, windows in (("window", per_window), ("callback", callback)):
+        pre = tmp_path / f"{name}.{label}"
+        _assemble_cfiles(pre, windows)
+        for sfx in SUFFIXES:
+            a, b = Path(str(pre) + sfx), Path(str(cpu) + sfx)
+            assert filecmp.cmp(a, b, shallow=False), f"{label}: {sfx} differs"
+
+
+def test_caller_stream_gives_the_same_rows(datasets, tmp_path):
+    """gsnp_set_stream queues the context's work on a caller's stream (a
+    torch stream here); the rows written through it, and after switching
+    back to the context's own stream, are the oracle's."""
+    import torch
+
+    import gsnp_amd
+
+    ds = datasets("c2w3")
+    job = gsnp_amd.Job(ds.fasta, ds.soap, ds.dbsnp, **ds.flags)
+    L = gsnp_amd.lib()
+    ref = ds.oracle_text().read_bytes()
+    try:
+        s = torch.cuda.Stream()
+        print("[parity] caller stream c2w3", flush=True)
+        gsnp_amd.check(L.gsnp_set_stream(job.ctx, ctypes.c_void_p(s.cuda_stream)))
+        out = tmp_path / "caller.cns"
+        job.write(out, batched=4)
+        assert out.read_bytes() == ref, _first_diff(out, ds.oracle_text())
+        gsnp_amd.check(L.gsnp_set_stream(job.ctx, None))
+        out2 = tmp_path / "own.cns"
+        job.write(out2, batched=0)
+        assert out2.read_bytes() == ref, _first_diff(out2, ds.oracle_text())
+    finally:
+        job.close()
