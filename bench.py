@@ -163,7 +163,9 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-mb", type=float, default=12.0, help="Mb of the workload in the CPU sample")
     ap.add_argument("--cpu-sites", type=int, default=16 * 262144, help="sites of the single-thread CPU run")
-    ap.add_argument("--no-tune", action="store_true", help="keep the default fused launch shape")
+    ap.add_argument("--tune", action="store_true", help="autotune the fused launch shape on the largest contig "
+                                                         "first (default: the library's default shape)")
+    ap.add_argument("--no-tune", action="store_true", help=argparse.SUPPRESS)  # the default; kept for old commands
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint leg (2 Mb miscalibrated workload)")
     ap.add_argument("--no-columns", action="store_true", help="skip the -C leg (column chunks instead of rows)")
     ap.add_argument("--no-stages", action="store_true", help="skip the per-stage timing passes")
@@ -202,7 +204,7 @@ def main() -> int:
 
     per = args.per
     tuned = None
-    if not args.no_tune:
+    if args.tune:
         t0 = time.time()
         big = max(range(len(contigs)), key=lambda i: contigs[i][1])
         job.select(big)

@@ -104,10 +104,13 @@ struct FusedArgs {
 
 // Launch shape of the fused stage: bucket classes below min_class run on
 // min_class's network (the smallest bucket edge), and each class gets its
-// waves per block (0: the most that fit 64 KB of LDS) and a grid cap.
+// waves per block (0: the most that fit 64 KB of LDS) and a grid cap.  The
+// defaults are what gsnp_autotune picks on a 100 bp, 30x-mean contig of
+// 8 Mb: sites of up to 8 records on the 16-wide network, four waves per
+// block for that class and the wide ones, eight for the 32 and 64 classes.
 struct FusedTune {
-    uint32_t min_class = 0;
-    uint32_t wpb[6] = {0, 0, 0, 0, 0, 0};
+    uint32_t min_class = 1;
+    uint32_t wpb[6] = {0, 4, 8, 8, 4, 4};
     uint32_t max_blocks[6] = {8192, 8192, 8192, 8192, 8192, 8192};
 };
 
