@@ -254,7 +254,7 @@ def main() -> int:
         achieved = fused_bytes / t_f / 1e9
         traffic, src = measured_traffic(args.config, args.scale)
         t_call_gbs = fused_bytes / secs / 1e9
-        roofline = {"bound": "hbm", "kernel": "fused stage (k_bucket + k_fused<S> per depth class)",
+        roofline = {"bound": "hbm", "kernel": "k_fused<S>: sort + likelihood + posterior, one launch per depth class",
                     "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "traffic_source": src,
                     "bytes_per_pass": int(fused_bytes), "ms_per_pass": round(t_f * 1e3, 4),

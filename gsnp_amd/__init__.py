@@ -314,7 +314,7 @@ def format_g(v: float) -> str:
 
 
 STAGES = ["site_build", "sort", "likelihood", "posterior", "rank", "format", "recycle", "fused", "adjoint", "columns",
-          "calib"]
+          "calib", "bucket"]
 
 
 # gsnp_chunk_fn: (user, chromosome, 1-based start, sites, chunk pointers, chunk bytes) -> 0 to go on

@@ -42,7 +42,10 @@ constexpr int kNC = 16;                      // counters per block
 constexpr int kNDict = 15;                   // dictionary-coded arrays per window
 constexpr uint32_t kTabStride = 2 * kWin;    // hash slots per (window, array)
 constexpr uint32_t kDictThreads = 1024;
-constexpr uint32_t kLdsSlots = 4096;         // LDS hash set of a dictionary: up to kLdsSlots/2 distinct values
+constexpr int kLdsLg = 13;
+constexpr uint32_t kLdsSlots = 1u << kLdsLg;  // LDS hash set of a dictionary: up to kLdsSlots/2 distinct values
+constexpr uint32_t kBmpWords = kLdsSlots;    // presence bitmap of small-range integer arrays, in the hash set's LDS
+constexpr uint32_t kBmpBits = 32 * kBmpWords;  // 262,144 values: any window's
 constexpr uint32_t kSubWindows = 16;         // windows per encoder pass
 constexpr uint64_t kEmpty = ~0ull;           // never a key: int keys use 32 bits, f64 keys are not NaN
 
@@ -343,8 +346,26 @@ struct DictArgs {
     uint32_t* nd;
     uint32_t* nw;
     uint32_t lds_max;      // distinct values kept in the LDS set before falling back to HBM (0: HBM only)
+    uint32_t bitmap;       // integer arrays of small range through the LDS presence bitmap
     uint8_t pow2bits[32];  // index bits of a dictionary of 2^e entries, as the host's libm gives them
 };
+
+// Distinct values above this count are ranked by sorting them (below it, by
+// counting the smaller ones: O(distinct^2) compares, cheap for a few hundred).
+constexpr uint32_t kRankSortMin = 512;
+
+// unsigned keys in the order of the values: int32 bits (zero-extended), or
+// double bits with negatives below positives and their order reversed
+__device__ __forceinline__ uint64_t order_key(uint64_t raw, bool dbl)
+{
+    if (!dbl) return (uint32_t)raw ^ 0x80000000u;
+    return (raw >> 63) ? ~raw : (raw | 0x8000000000000000ull);
+}
+__device__ __forceinline__ uint64_t raw_key(uint64_t k, bool dbl)
+{
+    if (!dbl) return (uint32_t)k ^ 0x80000000u;
+    return (k >> 63) ? (k & ~0x8000000000000000ull) : ~k;
+}
 
 __device__ __forceinline__ uint32_t slot_of(uint64_t key, int lg)
 {
@@ -383,6 +404,129 @@ __global__ __launch_bounds__(kDictThreads) void k_col_dict(DictArgs a)
     const uint32_t* in = a.in32 + k * a.stride + (size_t)w * kWin;
     const uint64_t* in64 = a.in64 + (size_t)w * kWin;
     auto key_at = [&](uint32_t e) -> uint64_t { return dbl ? in64[e] : (uint64_t)in[e]; };
+    if (!dbl && a.bitmap) {
+        // integer arrays whose values span fewer than kBmpBits: a presence
+        // bitmap over [min, min + kBmpBits) in LDS (ltab's space) gives the
+        // distinct values in order; a value's rank is the set bits below it
+        // (the word prefix in lrank's space plus a popcount in its word)
+        // (every pass over the elements keeps eight loads in flight per thread:
+        // one at a time, the passes wait on memory latency)
+        constexpr uint32_t U = 8;
+        __shared__ int32_t s_lo, s_hi;
+        int32_t lo = INT32_MAX, hi = INT32_MIN;
+        for (uint32_t e0 = t; e0 < m; e0 += U * kDictThreads) {
+            int32_t v[U];
+#pragma unroll
+            for (uint32_t u = 0; u < U; u++) {
+                const uint32_t e = e0 + u * kDictThreads;
+                v[u] = e < m ? (int32_t)in[e] : (int32_t)in[e0];
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < U; u++) {
+                lo = min(lo, v[u]);
+                hi = max(hi, v[u]);
+            }
+        }
+        for (int off = 32; off > 0; off >>= 1) {
+            lo = min(lo, __shfl_xor(lo, off));
+            hi = max(hi, __shfl_xor(hi, off));
+        }
+        if (t == 0) s_lo = INT32_MAX, s_hi = INT32_MIN;
+        __syncthreads();
+        if ((t & 63) == 0) {
+            atomicMin(&s_lo, lo);
+            atomicMax(&s_hi, hi);
+        }
+        __syncthreads();
+        const int32_t vmin = s_lo;
+        const int64_t range = (int64_t)s_hi - (int64_t)vmin + 1;
+        if (range <= (int64_t)kBmpBits) {
+            uint32_t* bits = reinterpret_cast<uint32_t*>(ltab);
+            uint32_t* wpre = lrank;
+            const uint32_t nbw = (uint32_t)((range + 31) / 32);
+            for (uint32_t i = t; i < nbw; i += kDictThreads) bits[i] = 0u;
+            __syncthreads();
+            for (uint32_t e0 = t; e0 < m; e0 += U * kDictThreads) {
+                int32_t v[U];
+#pragma unroll
+                for (uint32_t u = 0; u < U; u++) {
+                    const uint32_t e = e0 + u * kDictThreads;
+                    v[u] = e < m ? (int32_t)in[e] : (int32_t)in[e0];
+                }
+#pragma unroll
+                for (uint32_t u = 0; u < U; u++) {
+                    const uint32_t d = (uint32_t)(v[u] - vmin);
+                    const uint32_t b = 1u << (d & 31u);
+                    if (!(bits[d >> 5] & b)) atomicOr(&bits[d >> 5], b);  // most values are already present
+                }
+            }
+            __syncthreads();
+            // exclusive prefix of the words' popcounts: kBmpWords / kDictThreads
+            // words per thread, then a block scan of the thread sums
+            constexpr uint32_t PW = kBmpWords / kDictThreads;
+            uint32_t c[PW], sum = 0;
+#pragma unroll
+            for (uint32_t j = 0; j < PW; j++) {
+                const uint32_t i = t * PW + j;
+                c[j] = i < nbw ? (uint32_t)__popc(bits[i]) : 0u;
+                sum += c[j];
+            }
+            uint32_t incl = sum;
+            for (int off = 1; off < 64; off <<= 1) {
+                const uint32_t y = __shfl_up(incl, off);
+                if ((int)(t & 63) >= off) incl += y;
+            }
+            uint32_t* wsum = reinterpret_cast<uint32_t*>(tile);
+            if ((t & 63) == 63) wsum[t >> 6] = incl;
+            __syncthreads();
+            uint32_t run = incl - sum;
+            for (uint32_t q = 0; q < (t >> 6); q++) run += wsum[q];
+            if (t == kDictThreads - 1) s_nd = run + sum;
+#pragma unroll
+            for (uint32_t j = 0; j < PW; j++) {
+                const uint32_t i = t * PW + j;
+                if (i < nbw) {
+                    wpre[i] = run;
+                    uint32_t x = bits[i], r = run;
+                    while (x) {  // the dictionary, in order
+                        a.dict32[wk * kWin + r++] = (uint32_t)(vmin + (int32_t)(i * 32u + (uint32_t)(__ffs(x) - 1)));
+                        x &= x - 1u;
+                    }
+                }
+                run += c[j];
+            }
+            __syncthreads();
+            const uint32_t nd = s_nd;
+            const int bits_per = (nd & (nd - 1)) == 0 ? (int)a.pow2bits[__ffs(nd) - 1] : 32 - __clz(nd - 1);
+            const uint32_t per = 32u / (uint32_t)bits_per;
+            const uint32_t nwords = (m + per - 1) / per;
+            uint32_t* wo = a.words + wk * kWin;
+            for (uint32_t q = t; q < nwords; q += kDictThreads) {
+                const uint32_t e0 = q * per, e1 = min(e0 + per, m);
+                uint32_t acc = 0;
+                for (uint32_t eb = e0; eb < e1; eb += U) {
+                    int32_t v[U];
+#pragma unroll
+                    for (uint32_t u = 0; u < U; u++) v[u] = eb + u < e1 ? (int32_t)in[eb + u] : vmin;
+#pragma unroll
+                    for (uint32_t u = 0; u < U; u++) {
+                        if (eb + u >= e1) break;
+                        const uint32_t d = (uint32_t)(v[u] - vmin);
+                        const uint32_t r = wpre[d >> 5] + (uint32_t)__popc(bits[d >> 5] & ((1u << (d & 31u)) - 1u));
+                        acc = bits_per >= 32 ? 0u : (acc << bits_per);
+                        acc |= r;
+                    }
+                }
+                wo[q] = acc;
+            }
+            if (t == 0) {
+                a.nd[wk] = nd;
+                a.nw[wk] = nwords;
+            }
+            return;
+        }
+        __syncthreads();  // s_lo / s_hi read by every thread before the hash path reuses LDS
+    }
     uint32_t cap = 64;
     while (cap < 2 * m) cap <<= 1;
     const int lg = __ffs(cap) - 1;
@@ -411,7 +555,7 @@ __global__ __launch_bounds__(kDictThreads) void k_col_dict(DictArgs a)
                 const uint32_t e = e0 + u * kDictThreads;
                 if (e >= m) break;
                 const uint64_t key = keys[u];
-                uint32_t h = slot_of(key, 12);
+                uint32_t h = slot_of(key, kLdsLg);
                 for (uint32_t probe = 0; probe < kLdsSlots; probe++) {
                     unsigned long long cur = ltab[h];
                     if (cur == kEmpty) {
@@ -442,29 +586,85 @@ __global__ __launch_bounds__(kDictThreads) void k_col_dict(DictArgs a)
         for (uint32_t h = t; h < cap; h += kDictThreads) tab[h] = kEmpty;
         __syncthreads();
     }
-    for (uint32_t e = t; e < m && !lds; e += kDictThreads) {
-        const uint64_t key = key_at(e);
-        uint32_t h = slot_of(key, lg);
-        for (uint32_t probe = 0; probe < cap; probe++) {
-            uint64_t cur = tab_load(&tab[h]);
-            if (cur == kEmpty) {
-                cur = atomicCAS(reinterpret_cast<unsigned long long*>(&tab[h]), (unsigned long long)kEmpty,
-                                (unsigned long long)key);
-                if (cur == kEmpty) {
-                    const uint32_t p = atomicAdd(&s_nd, 1u);
-                    dk[p] = key;
-                    ds[p] = h;
-                    break;
-                }
-            }
-            if (cur == key) break;
-            h = (h + 1) & (cap - 1);
+    // (four keys loaded before any is inserted, as in the LDS set)
+    for (uint32_t e0 = t; e0 < m && !lds; e0 += 4 * kDictThreads) {
+        uint64_t keys[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const uint32_t e = e0 + u * kDictThreads;
+            keys[u] = e < m ? key_at(e) : 0ull;
         }
-        es[e] = h;
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const uint32_t e = e0 + u * kDictThreads;
+            if (e >= m) break;
+            const uint64_t key = keys[u];
+            uint32_t h = slot_of(key, lg);
+            for (uint32_t probe = 0; probe < cap; probe++) {
+                uint64_t cur = tab_load(&tab[h]);
+                if (cur == kEmpty) {
+                    cur = atomicCAS(reinterpret_cast<unsigned long long*>(&tab[h]), (unsigned long long)kEmpty,
+                                    (unsigned long long)key);
+                    if (cur == kEmpty) {
+                        const uint32_t p = atomicAdd(&s_nd, 1u);
+                        dk[p] = key;
+                        ds[p] = h;
+                        break;
+                    }
+                }
+                if (cur == key) break;
+                h = (h + 1) & (cap - 1);
+            }
+            es[e] = h;
+        }
     }
     __syncthreads();
     const uint32_t nd = s_nd;
-    for (uint32_t base = 0; base < nd; base += kDictThreads) {
+    if (nd > kRankSortMin) {
+        // many distinct values: sort (order key, slot) pairs of the distinct
+        // values in place (a bitonic network over the padded power of two,
+        // one block barrier per step); a value's rank is its sorted position
+        uint32_t npad = 1;
+        while (npad < nd) npad <<= 1;
+        // in LDS when the set was (the hash keys are no longer needed: the
+        // first half of their space takes the order keys, the second the
+        // slots), else in the distinct-key arrays in HBM
+        uint64_t* sk = lds ? reinterpret_cast<uint64_t*>(ltab) : dk;
+        uint32_t* ss = lds ? reinterpret_cast<uint32_t*>(ltab + kLdsSlots / 2) : ds;
+        for (uint32_t d = t; d < npad; d += kDictThreads) {
+            sk[d] = d < nd ? order_key(dk[d], dbl) : ~0ull;
+            ss[d] = d < nd ? ds[d] : 0u;
+        }
+        __syncthreads();
+        for (uint32_t kk = 2; kk <= npad; kk <<= 1)
+            for (uint32_t j = kk >> 1; j > 0; j >>= 1) {
+                for (uint32_t i = t; i < npad; i += kDictThreads) {
+                    const uint32_t l = i ^ j;
+                    if (l <= i) continue;
+                    const uint64_t x = sk[i], y = sk[l];
+                    if ((i & kk) == 0 ? x > y : x < y) {
+                        sk[i] = y;
+                        sk[l] = x;
+                        const uint32_t si = ss[i];
+                        ss[i] = ss[l];
+                        ss[l] = si;
+                    }
+                }
+                __syncthreads();
+            }
+        for (uint32_t r = t; r < nd; r += kDictThreads) {
+            const uint64_t x = raw_key(sk[r], dbl);
+            if (lds)
+                lrank[ss[r]] = r;
+            else
+                trank[ss[r]] = r;
+            if (dbl)
+                a.dict64[(size_t)w * kWin + r] = x;
+            else
+                a.dict32[wk * kWin + r] = (uint32_t)x;
+        }
+    }
+    for (uint32_t base = 0; base < nd && nd <= kRankSortMin; base += kDictThreads) {
         const uint32_t d = base + t;
         const uint64_t x = d < nd ? dk[d] : 0ull;
         uint32_t r = 0;
@@ -502,10 +702,16 @@ __global__ __launch_bounds__(kDictThreads) void k_col_dict(DictArgs a)
     for (uint32_t q = t; q < nwords; q += kDictThreads) {
         const uint32_t e0 = q * per, e1 = min(e0 + per, m);
         uint32_t acc = 0;
-#pragma unroll 8
-        for (uint32_t e = e0; e < e1; e++) {
-            acc = bits >= 32 ? 0u : (acc << bits);
-            acc |= lds ? lrank[es[e]] : trank[es[e]];
+        for (uint32_t eb = e0; eb < e1; eb += 8) {
+            uint32_t sl[8];  // the element slots first: eight loads in flight
+#pragma unroll
+            for (uint32_t u = 0; u < 8; u++) sl[u] = eb + u < e1 ? es[eb + u] : 0u;
+#pragma unroll
+            for (uint32_t u = 0; u < 8; u++) {
+                if (eb + u >= e1) break;
+                acc = bits >= 32 ? 0u : (acc << bits);
+                acc |= lds ? lrank[sl[u]] : trank[sl[u]];
+            }
         }
         wo[q] = acc;
     }
@@ -771,6 +977,9 @@ int encode_columns(Ctx& c, uint32_t offset, uint32_t ws_abs, uint32_t n, const C
     // 2048, the default; 0 builds every dictionary in HBM)
     da.lds_max = kLdsSlots / 2;
     if (const char* v = std::getenv("GSNP_DICT_LDS_MAX")) da.lds_max = (uint32_t)std::strtoul(v, nullptr, 10);
+    // GSNP_DICT_BITMAP=0 sends every integer array through the hash set too
+    da.bitmap = 1;
+    if (const char* v = std::getenv("GSNP_DICT_BITMAP")) da.bitmap = (uint32_t)std::strtoul(v, nullptr, 10);
     const size_t S = (size_t)b.win * kWin;
     int cur = 0;
     for (uint32_t s0 = 0; s0 < n; s0 += kSubWindows * kWin) {

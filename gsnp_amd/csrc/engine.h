@@ -108,6 +108,7 @@ enum Stage : int {
     kStAdjoint,
     kStColumns,
     kStCalib,
+    kStBucket,
     kNumStages
 };
 

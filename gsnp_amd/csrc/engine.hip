@@ -938,8 +938,10 @@ void Ctx::span_enqueue(uint32_t start, uint32_t n)
     launch_fill_u32(stream, job_count.p, 1, 0u);
     launch_fill_u32(stream, fix_count.p, 1, 0u);
     SiteArrays a = site_view(0);
-    timer.begin(kStFused, stream);
+    timer.begin(kStBucket, stream);
     launch_bucket(stream, a.nrec, n, lists.p, win_cap, counts.p, 1, stat.p, (int)tune.min_class);
+    timer.end(kStBucket, stream);
+    timer.begin(kStFused, stream);
     launch_fused(stream, fused_args(start), lists.p, win_cap, counts.p, n, tune);
     timer.end(kStFused, stream);
     if (p.rank_sum_mode) {

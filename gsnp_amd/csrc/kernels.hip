@@ -43,7 +43,8 @@ __device__ uint64_t lower_pos(const int32_t* pos, uint64_t lo, uint64_t hi, int6
 // repeat for every covering read; for an observed base (ACGT, quality >=
 // q_min) while dep_uni < 255: unique hits append a record and bump
 // count_uni/q_sum/dep_uni; every hit bumps count_all.  Records are buffered
-// four at a time in registers and stored as 16-byte words.
+// four at a time in registers (a shift register, newest last) and stored as
+// 16-byte words.
 // Sites below `carry` continue from the values already in memory (the
 // recycled read-length tail of the previous window); the others start at 0.
 // Only reads starting in [lo_min, hi_max] take part; `range` holds each
@@ -82,7 +83,7 @@ __global__ __launch_bounds__(kBuildSites) void k_site_build(ReadView r, const ui
     __shared__ __align__(16) uint8_t sseq[kStageBytes + 32];
     __shared__ int32_t spos[kStageReads];
     __shared__ uint32_t smeta[kStageReads];
-    __shared__ int sca[kBuildSites * 4];  // count_all per lane and base (one LDS add per base)
+    __shared__ int sca[4 * kBuildSites];  // count_all, one LDS add per observed base
     const uint32_t s = blockIdx.x * kBuildSites + threadIdx.x;
     const uint32_t pitch = r.pitch;
     const uint32_t kr = kStageBytes / pitch < kStageReads ? kStageBytes / pitch : kStageReads;
@@ -98,7 +99,8 @@ __global__ __launch_bounds__(kBuildSites) void k_site_build(ReadView r, const ui
     uint64_t qs = 0;   // 4 x 16-bit q_sum
     int ca0 = 0, ca1 = 0, ca2 = 0, ca3 = 0;
     uint32_t* row = a.rec + (size_t)s * kSlots;
-    uint32_t pb0 = 0, pb1 = 0, pb2 = 0, pb3 = 0;  // records of slots [nrec & ~3, nrec)
+    // records of slots [nrec & ~3, nrec), right-aligned: the newest in pb3
+    uint32_t pb0 = 0, pb1 = 0, pb2 = 0, pb3 = 0;
     if (live && s < carry) {
         depth = a.depth[s];
         repeat = a.repeat[s];
@@ -108,19 +110,25 @@ __global__ __launch_bounds__(kBuildSites) void k_site_build(ReadView r, const ui
         qs = (uint64_t)(uint32_t)q.x | ((uint64_t)(uint32_t)q.y << 16) | ((uint64_t)(uint32_t)q.z << 32) |
              ((uint64_t)(uint32_t)q.w << 48);
         ca0 = al.x, ca1 = al.y, ca2 = al.z, ca3 = al.w;
-        if (nrec & 3) {
+        const int k = nrec & 3;
+        if (k) {
             const uint4 v = *reinterpret_cast<const uint4*>(row + (nrec & ~3));
-            pb0 = v.x, pb1 = v.y, pb2 = v.z, pb3 = v.w;
+            pb1 = v.x;
+            pb2 = k == 2 ? v.x : v.y;
+            pb3 = k == 1 ? v.x : k == 2 ? v.y : v.z;
         }
     }
-    sca[threadIdx.x * 4 + 0] = ca0;
-    sca[threadIdx.x * 4 + 1] = ca1;
-    sca[threadIdx.x * 4 + 2] = ca2;
-    sca[threadIdx.x * 4 + 3] = ca3;
+    // count_all per base and lane, base-major (word o * 64 + lane): a lane
+    // stays in its own bank whatever base it adds to
+    sca[0 * kBuildSites + threadIdx.x] = ca0;
+    sca[1 * kBuildSites + threadIdx.x] = ca1;
+    sca[2 * kBuildSites + threadIdx.x] = ca2;
+    sca[3 * kBuildSites + threadIdx.x] = ca3;
     const int64_t p = (int64_t)ws + s;
     const int64_t p0 = (int64_t)ws + (int64_t)blockIdx.x * kBuildSites;  // the block's first site
     const int32_t lane = (int32_t)threadIdx.x;
     const bool head = head_qs && live && (p % kWin) == 0;
+    const bool any_head = __ballot(head) != 0;  // block-uniform: the snapshot test leaves the read loop of other blocks
     bool snapped = false;
     for (uint64_t c0 = b; c0 < e; c0 += kr) {
         const uint32_t cn = (uint32_t)(e - c0 < kr ? e - c0 : kr);
@@ -144,7 +152,7 @@ __global__ __launch_bounds__(kBuildSites) void k_site_build(ReadView r, const ui
         for (uint32_t j = 0; j < cn; j++) {
             const int32_t start = __builtin_amdgcn_readfirstlane(spos[j]);  // block-uniform
             const uint32_t m = __builtin_amdgcn_readfirstlane(smeta[j]);
-            if (head && !snapped && start >= lane) {
+            if (any_head && head && !snapped && start >= lane) {
                 head_qs[p / kWin - ws / kWin] = make_int4((int)(qs & 0xffff), (int)((qs >> 16) & 0xffff),
                                                           (int)((qs >> 32) & 0xffff), (int)(qs >> 48));
                 snapped = true;
@@ -163,21 +171,23 @@ __global__ __launch_bounds__(kBuildSites) void k_site_build(ReadView r, const ui
                 const uint32_t strand = rm_strand(m);
                 const uint32_t cyc = strand ? (uint32_t)(len - 1 - coord) : (uint32_t)coord;
                 const uint32_t w = rec_encode(cyc, strand, o, q, 1);
-                const int k = nrec & 3;
-                pb0 = k == 0 ? w : pb0;
-                pb1 = k == 1 ? w : pb1;
-                pb2 = k == 2 ? w : pb2;
-                pb3 = k == 3 ? w : pb3;
+                pb0 = pb1;
+                pb1 = pb2;
+                pb2 = pb3;
+                pb3 = w;
                 nrec += 1;
                 if ((nrec & 3) == 0) *reinterpret_cast<uint4*>(row + (nrec - 4)) = make_uint4(pb0, pb1, pb2, pb3);
                 cu += 1u << (8 * o);
                 qs += (uint64_t)q << (16 * o);
             }
-            atomicAdd(&sca[threadIdx.x * 4 + o], 1);
+            atomicAdd(&sca[o * kBuildSites + threadIdx.x], 1);
         }
     }
     if (!live) return;
-    if (nrec & 3) *reinterpret_cast<uint4*>(row + (nrec & ~3)) = make_uint4(pb0, pb1, pb2, pb3);
+    if (const int k = nrec & 3) {  // the last k records, left-aligned
+        const uint32_t x = k == 1 ? pb3 : k == 2 ? pb2 : pb1;
+        *reinterpret_cast<uint4*>(row + (nrec & ~3)) = make_uint4(x, k == 2 ? pb3 : pb2, pb3, 0u);
+    }
     if (head && !snapped)
         head_qs[p / kWin - ws / kWin] =
             make_int4((int)(qs & 0xffff), (int)((qs >> 16) & 0xffff), (int)((qs >> 32) & 0xffff), (int)(qs >> 48));
@@ -186,8 +196,8 @@ __global__ __launch_bounds__(kBuildSites) void k_site_build(ReadView r, const ui
     a.nrec[s] = nrec;
     a.cu[s] = make_int4((int)(cu & 0xff), (int)((cu >> 8) & 0xff), (int)((cu >> 16) & 0xff), (int)(cu >> 24));
     a.qs[s] = make_int4((int)(qs & 0xffff), (int)((qs >> 16) & 0xffff), (int)((qs >> 32) & 0xffff), (int)(qs >> 48));
-    a.ca[s] = make_int4(sca[threadIdx.x * 4 + 0], sca[threadIdx.x * 4 + 1], sca[threadIdx.x * 4 + 2],
-                        sca[threadIdx.x * 4 + 3]);
+    a.ca[s] = make_int4(sca[0 * kBuildSites + threadIdx.x], sca[1 * kBuildSites + threadIdx.x],
+                        sca[2 * kBuildSites + threadIdx.x], sca[3 * kBuildSites + threadIdx.x]);
 }
 
 void launch_site_build(hipStream_t st, const ReadView& r, uint64_t r0, uint64_t r1, uint32_t ws, uint32_t n_sites,
@@ -803,27 +813,44 @@ __device__ __forceinline__ void wide_slots(const uint32_t* v, uint64_t below, ui
     uint64_t live[R], eq[R][R], ob[R][R];  // [mine][theirs], theirs <= mine
 #pragma unroll
     for (int r = 0; r < R; r++) live[r] = __ballot(v[r] != 0xFFFFFFFFu);
+    // lanes whose key bit differs from mine, per 32-bit half: one signed
+    // one-bit extract and one ballot per register and bit, one three-input
+    // operation per (mine, theirs) pair and half; base bits first, so that
+    // the base masks are the key masks' start
+    uint32_t dlo[R][R], dhi[R][R], olo[R][R], ohi[R][R];
 #pragma unroll
     for (int r = 0; r < R; r++)
 #pragma unroll
-        for (int q = 0; q <= r; q++) eq[r][q] = ob[r][q] = live[q];
+        for (int q = 0; q <= r; q++) dlo[r][q] = dhi[r][q] = olo[r][q] = ohi[r][q] = 0u;
 #pragma unroll
     for (int b = 0; b < 11; b++) {
-        const uint32_t bit = b == 0 ? 8u : b < 9 ? (uint32_t)(8 + b) : (uint32_t)(14 + b);  // 8, 9..16, 23, 24
-        uint64_t m[R], bal[R];
+        const uint32_t bit = b < 2 ? 23u + (uint32_t)b : b == 2 ? 8u : (uint32_t)(6 + b);  // 23, 24, 8, 9..16
+        uint32_t m[R];
+        uint64_t bal[R];
 #pragma unroll
         for (int r = 0; r < R; r++) {
-            m[r] = 0ull - (uint64_t)((v[r] >> bit) & 1u);
-            bal[r] = __ballot(m[r] != 0ull);
+            m[r] = (uint32_t)__builtin_amdgcn_sbfe((int)v[r], bit, 1u);
+            bal[r] = __ballot((int)m[r] < 0);
         }
 #pragma unroll
         for (int r = 0; r < R; r++)
 #pragma unroll
             for (int q = 0; q <= r; q++) {
-                eq[r][q] &= ~(bal[q] ^ m[r]);
-                if (b >= 9) ob[r][q] &= ~(bal[q] ^ m[r]);
+                dlo[r][q] = __builtin_amdgcn_bitop3_b32(dlo[r][q], (uint32_t)bal[q], m[r], 0xF6);
+                dhi[r][q] = __builtin_amdgcn_bitop3_b32(dhi[r][q], (uint32_t)(bal[q] >> 32), m[r], 0xF6);
+                if (b == 1) {
+                    olo[r][q] = dlo[r][q];
+                    ohi[r][q] = dhi[r][q];
+                }
             }
     }
+#pragma unroll
+    for (int r = 0; r < R; r++)
+#pragma unroll
+        for (int q = 0; q <= r; q++) {
+            eq[r][q] = live[q] & ~(((uint64_t)dhi[r][q] << 32) | dlo[r][q]);
+            ob[r][q] = live[q] & ~(((uint64_t)ohi[r][q] << 32) | olo[r][q]);
+        }
     uint32_t t[R];
     uint64_t first[R];
 #pragma unroll
@@ -854,8 +881,11 @@ __host__ __device__ constexpr uint32_t qadj_lds_words(uint32_t td, uint32_t gd)
     return ((64u * td * gd + 15u) / 16u) * 4u;  // whole 16-byte units
 }
 
-template <int S, bool QL = false>
-__global__ __launch_bounds__(512) void k_fused(FusedArgs f, const uint32_t* list, const uint32_t* count_ptr)
+// Up to 128 records the kernel is held to 128 VGPRs, four waves per SIMD
+// (the 64-record class would take 148 and three waves; with four, the
+// stage is about 3 % faster); the 256 class keeps its registers.
+template <int S, bool QL = false, bool POST = true>
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(S <= 128 ? 4 : 1, 8))) void k_fused(FusedArgs f, const uint32_t* list, const uint32_t* count_ptr)
 {
     constexpr uint32_t G = fused_group<S>();
     constexpr uint32_t RS = fused_row_stride<S>();
@@ -904,6 +934,7 @@ __global__ __launch_bounds__(512) void k_fused(FusedArgs f, const uint32_t* list
             uint32_t qf_min = 63u, qa_min = 255u;  // out-of-table qualities: either reaches 0
             // in chunks of CH passes (bounds the registers)
             constexpr uint32_t CH = NP < 16 ? NP : S == 64 ? 8 : 16;  // 64: fewer registers
+            static_assert(G % P == 0 && NP % CH == 0 && CH % 2 == 0, "whole chunks of paired passes");
 #pragma unroll
             for (uint32_t k0 = 0; k0 < NP; k0 += CH) {
                 uint32_t v[CH];
@@ -1059,7 +1090,7 @@ __global__ __launch_bounds__(512) void k_fused(FusedArgs f, const uint32_t* list
 #pragma unroll
             for (int h = 0; h < 5; h++) out[h] = make_double2(acc[2 * h], acc[2 * h + 1]);
         }
-        if (f.posterior)
+        if constexpr (POST)
             posterior_site(f.t, f.ref, f.ws, f.a, site, [&](int g) { return acc[g]; }, f.head_qs, f.out, f.jobs,
                            f.job_count);
     }
@@ -1084,14 +1115,24 @@ static void launch_fused_class(hipStream_t st, const FusedArgs& f, const uint32_
     uint32_t blocks = (groups + wpb - 1) / wpb;
     if (blocks > max_blocks) blocks = max_blocks;
     if (!blocks) blocks = 1;
+    // the batched form's launches (posterior fused) and the per-window path's
+    // (likelihoods only) are separate instantiations
     if constexpr (S <= 32) {
         if (ql) {
-            hipLaunchKernelGGL((k_fused<S, true>), dim3(blocks), dim3(64 * wpb), table + wpb * per_wave, st, f, list,
-                               count);
+            if (f.posterior)
+                hipLaunchKernelGGL((k_fused<S, true, true>), dim3(blocks), dim3(64 * wpb), table + wpb * per_wave, st,
+                                   f, list, count);
+            else
+                hipLaunchKernelGGL((k_fused<S, true, false>), dim3(blocks), dim3(64 * wpb), table + wpb * per_wave, st,
+                                   f, list, count);
             return;
         }
     }
-    hipLaunchKernelGGL((k_fused<S, false>), dim3(blocks), dim3(64 * wpb), wpb * per_wave, st, f, list, count);
+    if (f.posterior)
+        hipLaunchKernelGGL((k_fused<S, false, true>), dim3(blocks), dim3(64 * wpb), wpb * per_wave, st, f, list, count);
+    else
+        hipLaunchKernelGGL((k_fused<S, false, false>), dim3(blocks), dim3(64 * wpb), wpb * per_wave, st, f, list,
+                           count);
 }
 
 void launch_fused_one(hipStream_t st, const FusedArgs& f, const uint32_t* lists, uint32_t pitch,

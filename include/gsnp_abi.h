@@ -286,10 +286,12 @@ int gsnp_sort_likelihood_noalloc(gsnp_ctx* ctx, uint32_t* records, const int32_t
 /* Per-stage device time from HIP events recorded on the context's stream:
  * returns (and resets) the accumulated milliseconds and launch counts of the
  * stages site build, sort, likelihood, posterior, rank-sum, format, recycle,
- * fused, adjoint, -C columns, calibration counts (11 entries), then turns
+ * fused (the batched form's sort + likelihood + posterior launches), adjoint,
+ * -C columns, calibration counts, depth-class bucketing of a span (12
+ * entries), then turns
  * recording on or off.  Replaces the cutil
  * timers of the reference (kernel/cuda_header.h:37-50). */
-#define GSNP_N_STAGES 11
+#define GSNP_N_STAGES 12
 int gsnp_profile(gsnp_ctx* ctx, int enable, double* stage_ms, uint64_t* stage_count, int n_stages);
 
 /* Observation records the likelihood stage has consumed since the last

@@ -156,13 +156,15 @@ def test_parameter_variants(datasets, tmp_path, name, flags, batched):
 def test_compressed_output(datasets, tmp_path, monkeypatch, name, per, lds):
     """-C files (encoded on the device, one chunk per reference window; with
     per > 0 the windows of a batched span are encoded together) are the
-    oracle's bytes, and decompress to the oracle's text.  lds "0" builds every
-    dictionary's hash set in HBM, "8" makes most of them overflow the LDS set
-    and start again in HBM."""
+    oracle's bytes, and decompress to the oracle's text.  By default integer
+    arrays of small range take the LDS presence bitmap; with lds set every
+    array takes the hash set, "0" building each one in HBM, "8" making most
+    of them overflow the LDS set and start again in HBM."""
     import gsnp_amd
 
     if lds is not None:
         monkeypatch.setenv("GSNP_DICT_LDS_MAX", lds)
+        monkeypatch.setenv("GSNP_DICT_BITMAP", "0")
     ds = datasets(name)
     print(f"[parity] -C {name} batched={per} lds={lds}", flush=True)
     gpu = tmp_path / f"{name}.gpu"
@@ -644,21 +646,22 @@ def test_window_and_callback_columns_match_oracle(datasets, tmp_path, name):
 
 
 def test_caller_stream_gives_the_same_rows(datasets, tmp_path):
-    """gsnp_set_stream queues the context's work on a caller's stream (a
-    torch stream here); the rows written through it, and after switching
-    back to the context's own stream, are the oracle's."""
-    import torch
-
+    """gsnp_set_stream queues the context's work on a caller's stream (one
+    created through the HIP runtime the library uses, as a C caller would);
+    the rows written through it, and after switching back to the context's
+    own stream, are the oracle's."""
     import gsnp_amd
 
     ds = datasets("c2w3")
     job = gsnp_amd.Job(ds.fasta, ds.soap, ds.dbsnp, **ds.flags)
     L = gsnp_amd.lib()
+    hip = ctypes.CDLL("libamdhip64.so.7")  # already loaded by the library: the same runtime
+    stream = ctypes.c_void_p()
+    assert hip.hipStreamCreate(ctypes.byref(stream)) == 0
     ref = ds.oracle_text().read_bytes()
     try:
-        s = torch.cuda.Stream()
         print("[parity] caller stream c2w3", flush=True)
-        gsnp_amd.check(L.gsnp_set_stream(job.ctx, ctypes.c_void_p(s.cuda_stream)))
+        gsnp_amd.check(L.gsnp_set_stream(job.ctx, stream))
         out = tmp_path / "caller.cns"
         job.write(out, batched=4)
         assert out.read_bytes() == ref, _first_diff(out, ds.oracle_text())
@@ -668,3 +671,4 @@ def test_caller_stream_gives_the_same_rows(datasets, tmp_path):
         assert out2.read_bytes() == ref, _first_diff(out2, ds.oracle_text())
     finally:
         job.close()
+        assert hip.hipStreamDestroy(stream) == 0
