@@ -258,7 +258,8 @@ def main() -> int:
                     "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "traffic_source": src,
                     "bytes_per_pass": int(fused_bytes), "ms_per_pass": round(t_f * 1e3, 4),
-                    "launches_per_pass": launches.get("fused"),
+                    "spans_per_pass": launches.get("fused"),
+                    "note": "one k_fused launch per depth class and span; achieved is over their HIP-event time",
                     "t_call": {"achieved": round(t_call_gbs, 2), "frac": round(t_call_gbs / HBM_PEAK_GBS, 4),
                                "note": "the same 4n + 80 bytes per site over the whole timed pass, rows to host"}}
 

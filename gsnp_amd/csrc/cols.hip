@@ -501,23 +501,33 @@ __global__ __launch_bounds__(kDictThreads) void k_col_dict(DictArgs a)
             const uint32_t per = 32u / (uint32_t)bits_per;
             const uint32_t nwords = (m + per - 1) / per;
             uint32_t* wo = a.words + wk * kWin;
-            for (uint32_t q = t; q < nwords; q += kDictThreads) {
-                const uint32_t e0 = q * per, e1 = min(e0 + per, m);
-                uint32_t acc = 0;
-                for (uint32_t eb = e0; eb < e1; eb += U) {
-                    int32_t v[U];
+            // each thread packs a contiguous run of whole words, sixteen
+            // elements in flight per step (a word at a time left only as many
+            // loads in flight as a word holds elements)
+            constexpr uint32_t PK = 16;
+            const uint32_t wpt = (nwords + kDictThreads - 1) / kDictThreads;
+            const uint32_t q0 = t * wpt, q1 = min(q0 + wpt, nwords);
+            if (q0 < q1) {
+                const uint32_t eb0 = q0 * per, eb1 = min(q1 * per, m);
+                uint32_t acc = 0, k = 0, q = q0;  // k: ranks in acc
+                for (uint32_t eb = eb0; eb < eb1; eb += PK) {
+                    int32_t v[PK];
 #pragma unroll
-                    for (uint32_t u = 0; u < U; u++) v[u] = eb + u < e1 ? (int32_t)in[eb + u] : vmin;
+                    for (uint32_t u = 0; u < PK; u++) v[u] = eb + u < eb1 ? (int32_t)in[eb + u] : vmin;
 #pragma unroll
-                    for (uint32_t u = 0; u < U; u++) {
-                        if (eb + u >= e1) break;
+                    for (uint32_t u = 0; u < PK; u++) {
+                        if (eb + u >= eb1) break;
                         const uint32_t d = (uint32_t)(v[u] - vmin);
                         const uint32_t r = wpre[d >> 5] + (uint32_t)__popc(bits[d >> 5] & ((1u << (d & 31u)) - 1u));
                         acc = bits_per >= 32 ? 0u : (acc << bits_per);
                         acc |= r;
+                        if (++k == per || eb + u + 1 == m) {  // a full word, or the window's last (right-aligned)
+                            wo[q++] = acc;
+                            acc = 0;
+                            k = 0;
+                        }
                     }
                 }
-                wo[q] = acc;
             }
             if (t == 0) {
                 a.nd[wk] = nd;
@@ -699,21 +709,29 @@ __global__ __launch_bounds__(kDictThreads) void k_col_dict(DictArgs a)
     const uint32_t per = 32u / (uint32_t)bits;
     const uint32_t nwords = (m + per - 1) / per;
     uint32_t* wo = a.words + wk * kWin;
-    for (uint32_t q = t; q < nwords; q += kDictThreads) {
-        const uint32_t e0 = q * per, e1 = min(e0 + per, m);
-        uint32_t acc = 0;
-        for (uint32_t eb = e0; eb < e1; eb += 8) {
-            uint32_t sl[8];  // the element slots first: eight loads in flight
+    // a contiguous run of whole words per thread, sixteen slots in flight
+    constexpr uint32_t PK = 16;
+    const uint32_t wpt = (nwords + kDictThreads - 1) / kDictThreads;
+    const uint32_t q0 = t * wpt, q1 = min(q0 + wpt, nwords);
+    if (q0 < q1) {
+        const uint32_t eb0 = q0 * per, eb1 = min(q1 * per, m);
+        uint32_t acc = 0, k = 0, q = q0;  // k: ranks in acc
+        for (uint32_t eb = eb0; eb < eb1; eb += PK) {
+            uint32_t sl[PK];
 #pragma unroll
-            for (uint32_t u = 0; u < 8; u++) sl[u] = eb + u < e1 ? es[eb + u] : 0u;
+            for (uint32_t u = 0; u < PK; u++) sl[u] = eb + u < eb1 ? es[eb + u] : 0u;
 #pragma unroll
-            for (uint32_t u = 0; u < 8; u++) {
-                if (eb + u >= e1) break;
+            for (uint32_t u = 0; u < PK; u++) {
+                if (eb + u >= eb1) break;
                 acc = bits >= 32 ? 0u : (acc << bits);
                 acc |= lds ? lrank[sl[u]] : trank[sl[u]];
+                if (++k == per || eb + u + 1 == m) {  // a full word, or the window's last (right-aligned)
+                    wo[q++] = acc;
+                    acc = 0;
+                    k = 0;
+                }
             }
         }
-        wo[q] = acc;
     }
     if (t == 0) {
         a.nd[wk] = nd;
