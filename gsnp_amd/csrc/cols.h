@@ -9,7 +9,7 @@ namespace gsnp {
 
 struct Ctx;
 
-// One encoder pass (up to 16 windows) of a span: the bytes each of the five
+// One encoder pass (up to 32 windows) of a span: the bytes each of the five
 // column files receives (.3_4_6, .5_7_8_9_14_16, .10, .11_12_13, .15_17), in
 // window order, in pinned host memory that the next pass reuses, and the
 // start (1-based) and size of every window for the index file .1_2.

@@ -13,10 +13,14 @@
 //   k_col_emit    per site: the sparse lists in site order, run values and
 //                 starts, the 2-bit words of columns 3 and 10
 //   k_col_runlen  run lengths from the run starts
-//   k_col_dict    per (window, array), one workgroup: sorted-unique
-//                 dictionary and packed ranks of the 15 dictionary-coded
-//                 arrays (a hash set gives the distinct values, each one's
-//                 rank is the number of distinct values below it)
+//   k_dict_setbits, k_dict_rank, k_dict_pack
+//                 the sorted-unique dictionary and packed ranks of the 14
+//                 integer arrays, over several workgroups per array: a
+//                 presence bitmap of the values, its prefix popcounts, then
+//                 each element's rank
+//   k_col_dict    per (window, array), one workgroup: the same for the
+//                 copy numbers (doubles) and any integer array the bitmap
+//                 does not cover (a hash set gives the distinct values)
 //   k_col_sizes   chunk sizes and their offsets in each file
 //   k_col_write   the chunks
 #include <hip/hip_runtime.h>
@@ -46,7 +50,7 @@ constexpr int kLdsLg = 13;
 constexpr uint32_t kLdsSlots = 1u << kLdsLg;  // LDS hash set of a dictionary: up to kLdsSlots/2 distinct values
 constexpr uint32_t kBmpWords = kLdsSlots;    // presence bitmap of small-range integer arrays, in the hash set's LDS
 constexpr uint32_t kBmpBits = 32 * kBmpWords;  // 262,144 values: any window's
-constexpr uint32_t kSubWindows = 16;         // windows per encoder pass
+constexpr uint32_t kSubWindows = 32;         // windows per encoder pass
 constexpr uint64_t kEmpty = ~0ull;           // never a key: int keys use 32 bits, f64 keys are not NaN
 
 // counters: sparse lists, N lists, non-N sites, then the run heads of
@@ -345,8 +349,9 @@ struct DictArgs {
     uint64_t* dict64;
     uint32_t* nd;
     uint32_t* nw;
-    uint32_t lds_max;      // distinct values kept in the LDS set before falling back to HBM (0: HBM only)
-    uint32_t bitmap;       // integer arrays of small range through the LDS presence bitmap
+    const uint32_t* state;  // [wk][3] of the split integer path: done flags at [2] (null: not run)
+    uint32_t lds_max;       // distinct values kept in the LDS set before falling back to HBM (0: HBM only)
+    uint32_t bitmap;        // integer arrays of small range through the LDS presence bitmap
     uint8_t pow2bits[32];  // index bits of a dictionary of 2^e entries, as the host's libm gives them
 };
 
@@ -396,6 +401,7 @@ __global__ __launch_bounds__(kDictThreads) void k_col_dict(DictArgs a)
     const bool all_n = tw[kCNotN] == 0;
     const uint32_t m = all_n ? 0u : k < 12 ? tw[kCRun0 + k % 6] : tw[kCCom];
     const size_t wk = (size_t)w * kNDict + k;
+    if (a.state && a.state[3 * wk + 2]) return;  // encoded by the split integer path
     if (m == 0) {
         if (t == 0) a.nd[wk] = a.nw[wk] = 0;
         return;
@@ -739,6 +745,419 @@ __global__ __launch_bounds__(kDictThreads) void k_col_dict(DictArgs a)
     }
 }
 
+// The dictionary-coded arrays are encoded by several workgroups each:
+//
+// Integer arrays whose values all lie in [0, kBmpBits) (every one on real
+// data), in three launches:
+//  k_dict_setbits  kDictParts workgroups per array set the values' bits in an
+//                  LDS bitmap of their slice, then OR its non-zero words into
+//                  the array's bitmap in HBM, with the largest value and an
+//                  out-of-range flag;
+//  k_dict_rank     one workgroup per array scans the words' popcounts, writes
+//                  the dictionary (the set bits in order), the word prefix and
+//                  a copy of the bits, and clears the bitmap and the flags for
+//                  the next pass;
+//  k_dict_pack     workgroups per array pack tiles of whole words: a tile's
+//                  elements are loaded coalesced, 16 per thread in flight, and
+//                  ranked (the word prefix plus the set bits below the value
+//                  in its word) into LDS, then each thread assembles words.
+// The copy numbers (doubles), with at most kHashRankMax distinct values per
+// window:
+//  k_hash_insert   workgroups per array insert the values into an open-
+//                  addressed set in HBM (compare-and-swap), each element
+//                  keeping its slot and each new value appended to a list;
+//  k_hash_rank     one workgroup per array ranks the listed values (the count
+//                  of smaller ones), writes the dictionary and each slot's
+//                  rank, and empties the used slots for the next pass;
+//  k_dict_pack     as above, with an element's rank from its slot.
+// An array these do not cover is left (done flag 0) to k_col_dict.
+constexpr uint32_t kDictParts = 16;
+constexpr uint32_t kDictPartThreads = 512;
+constexpr uint32_t kIntDicts = kNDict - 1;  // all but the copy numbers (doubles, array 5)
+constexpr uint32_t kPackK = 16;             // elements per thread of a pack tile
+constexpr uint32_t kPackTile = kPackK * kDictPartThreads;
+constexpr uint32_t kPackLds = 2048;         // bitmap words a pack workgroup keeps in LDS
+constexpr uint32_t kHashRankMax = 1024;     // distinct copy numbers ranked by k_hash_rank
+constexpr uint32_t kHashLg = 19;            // slots of a window's set: 2 kWin (load at most 1/2)
+constexpr uint32_t kHashSlots = 1u << kHashLg;
+static_assert(kHashSlots >= 2 * kWin, "the set holds a window's values at load 1/2");
+
+struct SplitArgs {
+    const uint32_t* tot;
+    const uint32_t* in32;
+    const uint64_t* in64;
+    size_t stride;
+    uint32_t* gbits;  // [wk][kBmpWords], zero between passes
+    uint32_t* rbits;  // [wk][kBmpWords]
+    uint32_t* wpre;   // [wk][kBmpWords]
+    uint32_t* state;  // [wk][3]: largest value (doubles: distinct count), out of range, done;
+                      // the first two zero between passes
+    uint64_t* htab;   // [w][kHashSlots], all empty between passes
+    uint32_t* hrank;  // [w][kHashSlots]
+    uint64_t* hkeys;  // [w][kWin] distinct values in insertion order
+    uint32_t* hslot;  // [w][kWin] their slots
+    uint32_t* eslot;  // [wk][kWin] each element's slot
+    uint32_t* dict32;
+    uint64_t* dict64;
+    uint32_t* words;
+    uint32_t* nd;
+    uint32_t* nw;
+    uint8_t pow2bits[32];
+};
+
+// array index of blockIdx.y in the integer launches (array 5 is skipped)
+__device__ __forceinline__ uint32_t int_dict(uint32_t y) { return y < 5 ? y : y + 1; }
+
+__device__ __forceinline__ uint32_t dict_elems(const uint32_t* tot, uint32_t w, uint32_t k)
+{
+    const uint32_t* tw = tot + (size_t)w * kNC;
+    return tw[kCNotN] == 0 ? 0u : k < 12 ? tw[kCRun0 + k % 6] : tw[kCCom];
+}
+
+__device__ __forceinline__ int dict_index_bits(uint32_t nd, const uint8_t* pow2bits)
+{
+    return (nd & (nd - 1)) == 0 ? (int)pow2bits[__ffs(nd) - 1] : 32 - __clz(nd - 1);
+}
+
+// grid (window, integer array, part)
+__global__ __launch_bounds__(kDictPartThreads) void k_dict_setbits(SplitArgs a)
+{
+    __shared__ uint32_t bits[kBmpWords];
+    __shared__ uint32_t s_max, s_over;
+    const uint32_t w = blockIdx.x, k = int_dict(blockIdx.y), p = blockIdx.z, t = threadIdx.x;
+    const uint32_t m = dict_elems(a.tot, w, k);
+    const uint32_t e0 = (uint32_t)((uint64_t)m * p / kDictParts), e1 = (uint32_t)((uint64_t)m * (p + 1) / kDictParts);
+    if (e0 == e1) return;
+    const size_t wk = (size_t)w * kNDict + k;
+    for (uint32_t i = t; i < kBmpWords; i += kDictPartThreads) bits[i] = 0u;
+    if (t == 0) s_max = 0u, s_over = 0u;
+    __syncthreads();
+    const uint32_t* in = a.in32 + k * a.stride + (size_t)w * kWin;
+    uint32_t hi = 0, over = 0;
+    constexpr uint32_t U = 16;  // loads in flight per thread
+    for (uint32_t e = e0 + t; e < e1; e += U * kDictPartThreads) {
+        uint32_t v[U];
+#pragma unroll
+        for (uint32_t u = 0; u < U; u++) {
+            const uint32_t ee = e + u * kDictPartThreads;
+            v[u] = in[ee < e1 ? ee : e];  // a repeat of element e sets its bit again
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < U; u++) {
+            if (v[u] >= kBmpBits) {  // negative ones too, as unsigned
+                over = 1u;
+                continue;
+            }
+            hi = max(hi, v[u]);
+            const uint32_t b = 1u << (v[u] & 31u);
+            if (!(bits[v[u] >> 5] & b)) atomicOr(&bits[v[u] >> 5], b);  // most values are already present
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        hi = max(hi, (uint32_t)__shfl_xor((int)hi, off));
+        over |= (uint32_t)__shfl_xor((int)over, off);
+    }
+    if ((t & 63) == 0) {
+        atomicMax(&s_max, hi);
+        if (over) atomicOr(&s_over, 1u);
+    }
+    __syncthreads();
+    const uint32_t nbw = s_max / 32u + 1u;
+    uint32_t* gb = a.gbits + wk * kBmpWords;
+    for (uint32_t i = t; i < nbw; i += kDictPartThreads) {
+        const uint32_t x = bits[i];
+        if (x) atomicOr(&gb[i], x);
+    }
+    if (t == 0) {
+        atomicMax(&a.state[3 * wk], s_max);
+        if (s_over) atomicOr(&a.state[3 * wk + 1], 1u);
+    }
+}
+
+// block-wide exclusive scan of one value per thread (kDictThreads threads);
+// `total` receives the sum
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* wsum, uint32_t* total)
+{
+    const uint32_t t = threadIdx.x;
+    uint32_t incl = v;
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t y = (uint32_t)__shfl_up((int)incl, off);
+        if ((int)(t & 63) >= off) incl += y;
+    }
+    if ((t & 63) == 63) wsum[t >> 6] = incl;
+    __syncthreads();
+    uint32_t run = incl - v;
+    for (uint32_t q = 0; q < (t >> 6); q++) run += wsum[q];
+    if (t == kDictThreads - 1) *total = run + v;
+    return run;
+}
+
+// grid (window, integer array)
+__global__ __launch_bounds__(kDictThreads) void k_dict_rank(SplitArgs a)
+{
+    __shared__ uint32_t s_nbw, s_over, s_nd;
+    __shared__ uint32_t wsum[kDictThreads / 64];
+    const uint32_t w = blockIdx.x, k = int_dict(blockIdx.y), t = threadIdx.x;
+    const uint32_t m = dict_elems(a.tot, w, k);
+    const size_t wk = (size_t)w * kNDict + k;
+    if (m == 0) {  // no part touched the bitmap or the flags
+        if (t == 0) a.nd[wk] = a.nw[wk] = 0, a.state[3 * wk + 2] = 1u;
+        return;
+    }
+    if (t == 0) {
+        s_nbw = a.state[3 * wk] / 32u + 1u;  // every set bit lies below
+        s_over = a.state[3 * wk + 1];
+    }
+    __syncthreads();
+    const uint32_t nbw = s_nbw;
+    uint32_t* gb = a.gbits + wk * kBmpWords;
+    if (s_over) {  // left to k_col_dict; the bitmap is cleared for the next pass
+        for (uint32_t i = t; i < nbw; i += kDictThreads) gb[i] = 0u;
+        if (t == 0) a.state[3 * wk] = a.state[3 * wk + 1] = a.state[3 * wk + 2] = 0u;
+        return;
+    }
+    uint32_t* rb = a.rbits + wk * kBmpWords;
+    uint32_t* wp = a.wpre + wk * kBmpWords;
+    // kBmpWords / kDictThreads words per thread, then a block scan of the
+    // thread sums
+    constexpr uint32_t PW = kBmpWords / kDictThreads;
+    uint32_t x[PW], sum = 0;
+#pragma unroll
+    for (uint32_t j = 0; j < PW; j++) {
+        const uint32_t i = t * PW + j;
+        x[j] = i < nbw ? gb[i] : 0u;
+        sum += (uint32_t)__popc(x[j]);
+    }
+    uint32_t run = block_excl_scan(sum, wsum, &s_nd);
+    uint32_t* dict = a.dict32 + wk * kWin;
+#pragma unroll
+    for (uint32_t j = 0; j < PW; j++) {
+        const uint32_t i = t * PW + j;
+        if (i < nbw) {
+            rb[i] = x[j];
+            wp[i] = run;
+            gb[i] = 0u;
+            uint32_t y = x[j], r = run;
+            while (y) {  // the dictionary, in order
+                dict[r++] = i * 32u + (uint32_t)(__ffs(y) - 1);
+                y &= y - 1u;
+            }
+        }
+        run += (uint32_t)__popc(x[j]);
+    }
+    __syncthreads();
+    if (t == 0) {
+        const uint32_t nd = s_nd;
+        const uint32_t per = 32u / (uint32_t)dict_index_bits(nd, a.pow2bits);
+        a.nd[wk] = nd;
+        a.nw[wk] = (m + per - 1) / per;
+        a.state[3 * wk] = a.state[3 * wk + 1] = 0u;
+        a.state[3 * wk + 2] = 1u;
+    }
+}
+
+__device__ __forceinline__ uint32_t hash_slot(uint64_t key)
+{
+    return (uint32_t)((key * 0x9E3779B97F4A7C15ull) >> (64 - kHashLg));
+}
+
+// the slot of `key` in the window's set, inserting it (and listing it as
+// new) when absent; load at most 1/2, so every probe sequence meets an empty
+// slot
+__device__ __forceinline__ uint32_t hash_insert(uint64_t* tab, uint64_t key, uint32_t* count, uint64_t* keys,
+                                                uint32_t* slots)
+{
+    uint32_t h = hash_slot(key);
+    for (uint32_t probe = 0; probe < kHashSlots; probe++) {
+        uint64_t cur = tab_load(&tab[h]);
+        if (cur == kEmpty) {
+            cur = atomicCAS(reinterpret_cast<unsigned long long*>(&tab[h]), (unsigned long long)kEmpty,
+                            (unsigned long long)key);
+            if (cur == kEmpty) {
+                const uint32_t d = atomicAdd(count, 1u);
+                keys[d] = key;
+                slots[d] = h;
+                return h;
+            }
+        }
+        if (cur == key) return h;
+        h = (h + 1u) & (kHashSlots - 1u);
+    }
+    return h;  // not reached
+}
+
+constexpr uint32_t kHashParts = kWin / (8 * kDictPartThreads);  // eight elements per thread: 64
+static_assert(kHashParts * 8 * kDictPartThreads == kWin, "the parts cover a window's array");
+constexpr uint32_t kHashLdsLg = 11;                             // a workgroup's own set: 2048 slots
+constexpr uint32_t kHashLds = 1u << kHashLdsLg;
+constexpr uint32_t kHashLdsProbes = 64;  // past this an element goes to the window's set directly
+
+// grid (window, part): the copy numbers (array 5) into the window's set.
+// A workgroup first gathers its slice's distinct values in a set of its own
+// in LDS, so that only those (a few hundred) reach the window's set: with
+// every element inserted there, the few hot values took a compare-and-swap
+// per element.
+__global__ __launch_bounds__(kDictPartThreads) void k_hash_insert(SplitArgs a)
+{
+    __shared__ unsigned long long lset[kHashLds];
+    __shared__ uint32_t lglob[kHashLds];
+    const uint32_t w = blockIdx.x, p = blockIdx.y, t = threadIdx.x;
+    const uint32_t m = dict_elems(a.tot, w, 5);
+    const uint32_t e0 = p * 8u * kDictPartThreads;
+    if (e0 >= m) return;
+    const size_t wk = (size_t)w * kNDict + 5;
+    for (uint32_t i = t; i < kHashLds; i += kDictPartThreads) lset[i] = kEmpty;
+    __syncthreads();
+    const uint64_t* in = a.in64 + (size_t)w * kWin;
+    uint64_t* tab = a.htab + (size_t)w * kHashSlots;
+    uint32_t* es = a.eslot + wk * kWin;
+    uint32_t* count = &a.state[3 * wk];
+    uint64_t* keys = a.hkeys + (size_t)w * kWin;
+    uint32_t* slots = a.hslot + (size_t)w * kWin;
+    uint64_t key[8];
+    uint32_t ls[8];  // the element's slot in the workgroup's set, or kNoSlot
+    constexpr uint32_t kNoSlot = 0xFFFFFFFFu;
+#pragma unroll
+    for (uint32_t u = 0; u < 8; u++) {
+        const uint32_t e = e0 + t + u * kDictPartThreads;
+        key[u] = e < m ? in[e] : kEmpty;
+    }
+#pragma unroll
+    for (uint32_t u = 0; u < 8; u++) {
+        ls[u] = kNoSlot;
+        if (key[u] == kEmpty) continue;  // past the array's end (values are never NaN)
+        uint32_t h = (uint32_t)((key[u] * 0x9E3779B97F4A7C15ull) >> (64 - kHashLdsLg));
+        for (uint32_t probe = 0; probe < kHashLdsProbes; probe++) {
+            unsigned long long cur = lset[h];
+            if (cur == kEmpty) cur = atomicCAS(&lset[h], (unsigned long long)kEmpty, (unsigned long long)key[u]);
+            if (cur == kEmpty || cur == key[u]) {
+                ls[u] = h;
+                break;
+            }
+            h = (h + 1u) & (kHashLds - 1u);
+        }
+    }
+    __syncthreads();
+    for (uint32_t i = t; i < kHashLds; i += kDictPartThreads) {
+        const uint64_t k = lset[i];
+        if (k != kEmpty) lglob[i] = hash_insert(tab, k, count, keys, slots);
+    }
+    __syncthreads();
+#pragma unroll
+    for (uint32_t u = 0; u < 8; u++) {
+        const uint32_t e = e0 + t + u * kDictPartThreads;
+        if (e >= m) break;
+        es[e] = ls[u] != kNoSlot ? lglob[ls[u]] : hash_insert(tab, key[u], count, keys, slots);
+    }
+}
+
+// grid (window): rank of each listed copy number, the dictionary in order
+__global__ __launch_bounds__(kDictThreads) void k_hash_rank(SplitArgs a)
+{
+    __shared__ uint64_t ok[kHashRankMax];
+    const uint32_t w = blockIdx.x, t = threadIdx.x;
+    const uint32_t m = dict_elems(a.tot, w, 5);
+    const size_t wk = (size_t)w * kNDict + 5;
+    uint64_t* tab = a.htab + (size_t)w * kHashSlots;
+    const uint64_t* hk = a.hkeys + (size_t)w * kWin;
+    const uint32_t* hs = a.hslot + (size_t)w * kWin;
+    const uint32_t nd = a.state[3 * wk];  // written by the previous launch only
+    if (m == 0 || nd > kHashRankMax) {  // none, or left to k_col_dict: the set is emptied
+        for (uint32_t d = t; d < nd; d += kDictThreads) tab[hs[d]] = kEmpty;
+        __syncthreads();  // every thread has read the count
+        if (t == 0) {
+            if (m == 0) a.nd[wk] = a.nw[wk] = 0;
+            a.state[3 * wk] = 0u;
+            a.state[3 * wk + 2] = m == 0 ? 1u : 0u;
+        }
+        return;
+    }
+    for (uint32_t d = t; d < nd; d += kDictThreads) ok[d] = order_key(hk[d], true);
+    __syncthreads();
+    uint32_t* rk = a.hrank + (size_t)w * kHashSlots;
+    for (uint32_t d = t; d < nd; d += kDictThreads) {
+        const uint64_t x = ok[d];
+        uint32_t r = 0;
+        for (uint32_t u = 0; u < nd; u++) r += ok[u] < x;  // all lanes read one word: a broadcast
+        rk[hs[d]] = r;
+        a.dict64[(size_t)w * kWin + r] = hk[d];
+        tab[hs[d]] = kEmpty;
+    }
+    __syncthreads();
+    if (t == 0) {
+        const uint32_t per = 32u / (uint32_t)dict_index_bits(nd, a.pow2bits);
+        a.nd[wk] = nd;
+        a.nw[wk] = (m + per - 1) / per;
+        a.state[3 * wk] = 0u;
+        a.state[3 * wk + 2] = 1u;
+    }
+}
+
+// grid (window, array, part); array 5 takes its ranks from the slots of
+// k_hash_insert, the others from their bitmaps
+__global__ __launch_bounds__(kDictPartThreads) void k_dict_pack(SplitArgs a)
+{
+    __shared__ uint32_t lb[kPackLds], lp[kPackLds], rk[kPackTile];
+    const uint32_t w = blockIdx.x, k = blockIdx.y, p = blockIdx.z, t = threadIdx.x;
+    const uint32_t m = dict_elems(a.tot, w, k);
+    const size_t wk = (size_t)w * kNDict + k;
+    if (m == 0 || !a.state[3 * wk + 2]) return;
+    const uint32_t nd = a.nd[wk], nwords = a.nw[wk];
+    const int bits = dict_index_bits(nd, a.pow2bits);
+    const uint32_t per = 32u / (uint32_t)bits;
+    const uint32_t tw = kPackTile / per;  // whole words per tile
+    const uint32_t ntiles = (nwords + tw - 1) / tw;
+    const uint32_t t0 = ntiles * p / kDictParts, t1 = ntiles * (p + 1) / kDictParts;
+    if (t0 == t1) return;
+    const bool dbl = k == 5;
+    const uint32_t* gb = a.rbits + wk * kBmpWords;
+    const uint32_t* gp = a.wpre + wk * kBmpWords;
+    const uint32_t nbw = dbl ? 0u : a.dict32[wk * kWin + nd - 1] / 32u + 1u;  // the largest value's word
+    const bool inl = nbw <= kPackLds;
+    if (inl)
+        for (uint32_t i = t; i < nbw; i += kDictPartThreads) {
+            lb[i] = gb[i];
+            lp[i] = gp[i];
+        }
+    const uint32_t* in = a.in32 + k * a.stride + (size_t)w * kWin;
+    const uint32_t* es = a.eslot + wk * kWin;
+    const uint32_t* hr = a.hrank + (size_t)w * kHashSlots;
+    uint32_t* wo = a.words + wk * kWin;
+    for (uint32_t tile = t0; tile < t1; tile++) {
+        const uint32_t q0 = tile * tw, q1 = min(q0 + tw, nwords);
+        const uint32_t e0 = q0 * per, e1 = min(q1 * per, m);
+        uint32_t v[kPackK];
+#pragma unroll
+        for (uint32_t i = 0; i < kPackK; i++) {
+            const uint32_t e = e0 + t + i * kDictPartThreads;
+            v[i] = e < e1 ? (dbl ? es[e] : in[e]) : 0u;
+        }
+        __syncthreads();  // the bitmap copy is in; the previous tile's ranks are used
+#pragma unroll
+        for (uint32_t i = 0; i < kPackK; i++) {
+            const uint32_t e = e0 + t + i * kDictPartThreads;
+            if (e >= e1) break;
+            uint32_t r;
+            if (dbl) {
+                r = hr[v[i]];
+            } else {
+                const uint32_t wd = v[i] >> 5, below = (1u << (v[i] & 31u)) - 1u;
+                r = inl ? lp[wd] + (uint32_t)__popc(lb[wd] & below) : gp[wd] + (uint32_t)__popc(gb[wd] & below);
+            }
+            rk[e - e0] = r;
+        }
+        __syncthreads();
+        // a word per thread and step, ranks MSB first; the array's last word
+        // holds the remainder, right-aligned
+        for (uint32_t q = q0 + t; q < q1; q += kDictPartThreads) {
+            const uint32_t eb = (q - q0) * per, ee = min(eb + per, e1 - e0);
+            uint32_t acc = 0;
+            for (uint32_t e = eb; e < ee; e++) acc = bits >= 32 ? rk[e] : ((acc << bits) | rk[e]);
+            wo[q] = acc;
+        }
+    }
+}
+
 struct WriteArgs {
     uint32_t n;
     const uint32_t* tot;
@@ -840,7 +1259,7 @@ struct SizeSink {
 };
 
 // one thread per (window, file): chunk sizes, then their offsets per file
-__global__ __launch_bounds__(128) void k_col_sizes(WriteArgs a, uint32_t nwin)
+__global__ __launch_bounds__(256) void k_col_sizes(WriteArgs a, uint32_t nwin)
 {
     __shared__ uint64_t sz[kSubWindows * 5];
     const uint32_t t = threadIdx.x;
@@ -950,6 +1369,14 @@ static int alloc_cols(ColBufs& b, uint32_t win)
     HIPCHK(b.nw.alloc(wk));
     HIPCHK(b.foff.alloc((size_t)win * 5));
     HIPCHK(b.ftot.alloc(5));
+    HIPCHK(b.gbits.alloc(wk * kBmpWords));
+    HIPCHK(b.rbits.alloc(wk * kBmpWords));
+    HIPCHK(b.wpre.alloc(wk * kBmpWords));
+    HIPCHK(b.gstate.alloc(3 * wk));
+    HIPCHK(b.htab.alloc((size_t)win * kHashSlots));
+    HIPCHK(b.hrank.alloc((size_t)win * kHashSlots));
+    HIPCHK(b.hkeys.alloc(S));
+    HIPCHK(b.hslot.alloc(S));
     b.win = win;
     return GSNP_OK;
 }
@@ -964,6 +1391,13 @@ int encode_columns(Ctx& c, uint32_t offset, uint32_t ws_abs, uint32_t n, const C
         HIPCHK(hipStreamSynchronize(st));
         int rc = alloc_cols(b, want);
         if (rc) return rc;
+        // the split dictionaries' bitmap and flags start at zero; each pass
+        // leaves them so
+        const size_t wk = (size_t)b.win * kNDict;
+        launch_fill_u32(st, b.gbits.p, (uint32_t)(wk * kBmpWords), 0u);
+        launch_fill_u32(st, b.gstate.p, (uint32_t)(3 * wk), 0u);
+        launch_fill_u32(st, b.htab.p, (uint32_t)(2 * (size_t)b.win * kHashSlots), 0xFFFFFFFFu);  // kEmpty
+        HIPCHK(hipGetLastError());
     }
     if (!b.copy) {
         HIPCHK(hipStreamCreateWithFlags(&b.copy, hipStreamNonBlocking));
@@ -1025,6 +1459,22 @@ int encode_columns(Ctx& c, uint32_t offset, uint32_t ws_abs, uint32_t n, const C
         da.tot = b.tot.p, da.in32 = b.in32.p, da.in64 = b.in64.p, da.stride = S;
         da.tab = b.tab.p, da.trank = b.trank.p, da.dkeys = b.dkeys.p, da.dslot = b.dslot.p, da.eslot = b.eslot.p;
         da.words = b.words.p, da.dict32 = b.dict32.p, da.dict64 = b.dict64.p, da.nd = b.nd.p, da.nw = b.nw.p;
+        da.state = nullptr;
+        if (da.bitmap) {
+            SplitArgs sa;
+            sa.tot = b.tot.p, sa.in32 = b.in32.p, sa.in64 = b.in64.p, sa.stride = S;
+            sa.gbits = b.gbits.p, sa.rbits = b.rbits.p, sa.wpre = b.wpre.p, sa.state = b.gstate.p;
+            sa.htab = b.htab.p, sa.hrank = b.hrank.p, sa.hkeys = b.hkeys.p, sa.hslot = b.hslot.p;
+            sa.eslot = b.eslot.p, sa.dict32 = b.dict32.p, sa.dict64 = b.dict64.p, sa.words = b.words.p;
+            sa.nd = b.nd.p, sa.nw = b.nw.p;
+            std::copy(da.pow2bits, da.pow2bits + 32, sa.pow2bits);
+            hipLaunchKernelGGL(k_dict_setbits, dim3(nwin, kIntDicts, kDictParts), dim3(kDictPartThreads), 0, st, sa);
+            hipLaunchKernelGGL(k_hash_insert, dim3(nwin, kHashParts), dim3(kDictPartThreads), 0, st, sa);
+            hipLaunchKernelGGL(k_dict_rank, dim3(nwin, kIntDicts), dim3(kDictThreads), 0, st, sa);
+            hipLaunchKernelGGL(k_hash_rank, dim3(nwin), dim3(kDictThreads), 0, st, sa);
+            hipLaunchKernelGGL(k_dict_pack, dim3(nwin, kNDict, kDictParts), dim3(kDictPartThreads), 0, st, sa);
+            da.state = b.gstate.p;
+        }
         hipLaunchKernelGGL(k_col_dict, dim3(nwin, kNDict), dim3(kDictThreads), 0, st, da);
         WriteArgs wa;
         wa.n = ns, wa.tot = b.tot.p, wa.nd = b.nd.p, wa.nw = b.nw.p;
@@ -1034,7 +1484,7 @@ int encode_columns(Ctx& c, uint32_t offset, uint32_t ws_abs, uint32_t n, const C
         wa.code3 = b.code3.p, wa.code10 = b.code10.p, wa.words = b.words.p, wa.dict32 = b.dict32.p;
         wa.dict64 = b.dict64.p, wa.val15 = b.val15.p, wa.foff = b.foff.p, wa.ftot = b.ftot.p;
         wa.out = nullptr, wa.cap = 0;
-        hipLaunchKernelGGL(k_col_sizes, dim3(1), dim3(128), 0, st, wa, nwin);
+        hipLaunchKernelGGL(k_col_sizes, dim3(1), dim3(256), 0, st, wa, nwin);
         c.timer.end(kStColumns, st);
         HIPCHK(hipGetLastError());
         uint64_t tot[5];

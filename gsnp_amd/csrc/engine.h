@@ -158,6 +158,14 @@ struct ColBufs {
     DevBuf<uint64_t> tab, dkeys;
     DevBuf<uint32_t> trank, dslot, eslot, words, dict32, nd, nw;
     DevBuf<uint64_t> dict64, foff, ftot;
+    // integer dictionaries spread over workgroups: presence bits (zero
+    // between passes), their copy and word prefix, and per array the largest
+    // value, an out-of-range flag and a done flag
+    DevBuf<uint32_t> gbits, rbits, wpre, gstate;
+    // the copy numbers' split path: a hash set per window (empty between
+    // passes), each slot's rank, the distinct values and their slots
+    DevBuf<uint64_t> htab, hkeys;
+    DevBuf<uint32_t> hrank, hslot;
     // the five file streams of a pass, double-buffered: the copy of one pass
     // to pinned host memory runs on `copy` while the next pass is encoded
     DevBuf<uint8_t> stream[2];

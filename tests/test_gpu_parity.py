@@ -156,10 +156,12 @@ def test_parameter_variants(datasets, tmp_path, name, flags, batched):
 def test_compressed_output(datasets, tmp_path, monkeypatch, name, per, lds):
     """-C files (encoded on the device, one chunk per reference window; with
     per > 0 the windows of a batched span are encoded together) are the
-    oracle's bytes, and decompress to the oracle's text.  By default integer
-    arrays of small range take the LDS presence bitmap; with lds set every
-    array takes the hash set, "0" building each one in HBM, "8" making most
-    of them overflow the LDS set and start again in HBM."""
+    oracle's bytes, and decompress to the oracle's text.  By default the
+    integer arrays take presence bitmaps and the copy numbers a hash set,
+    each array spread over several workgroups; with lds set every array goes
+    to the one-workgroup encoder and its hash set, "0" building each one in
+    HBM, "8" making most of them overflow the LDS set and start again in
+    HBM."""
     import gsnp_amd
 
     if lds is not None:
@@ -216,14 +218,52 @@ def test_compressed_many_distinct_values(built, tmp_path, per):
     assert gt.read_bytes() == ct.read_bytes(), _first_diff(gt, ct)
 
 
+@pytest.mark.parametrize("per", [0, 4])
+def test_compressed_uncovered_window(built, tmp_path, per):
+    """A reference window without reads: its run-length columns hold one run
+    of 262,144 sites, a value beyond the integer dictionaries' presence bitmap
+    ([0, 262144)), so those arrays go back to the one-workgroup encoder while
+    the other windows' arrays take the split bitmap path.  The -C files are
+    the oracle's bytes."""
+    import random
+
+    import gsnp_amd
+
+    rng = random.Random(11)
+    W, L = 262144, 50
+    G = 2 * W + 150000
+    ref = "".join(rng.choice("ACGT") for _ in range(G))
+    fa, so = tmp_path / "g.fa", tmp_path / "d.soap"
+    fa.write_text(">chrU\n" + "\n".join(ref[i:i + 80] for i in range(0, G, 80)) + "\n")
+    starts = sorted(p for p in (rng.randrange(1, G - L - 1) for _ in range(G * 6 // L))
+                    if p + L <= W or p > 2 * W)  # nothing reaches window 1
+    lines = []
+    for i, p in enumerate(starts):
+        seq = [b if rng.random() >= 0.01 else rng.choice("ACGT") for b in ref[p - 1:p - 1 + L]]
+        q = "".join(chr(64 + rng.randint(5, 40)) for _ in range(L))
+        lines.append(f"r{i}\t{''.join(seq)}\t{q}\t1\ta\t{L}\t{rng.choice('+-')}\tchrU\t{p}\t0\n")
+    so.write_text("".join(lines))
+    print(f"[parity] -C uncovered window batched={per}", flush=True)
+    gpu, cpu = tmp_path / "gpu", tmp_path / "cpu"
+    gsnp_amd.run(fa, so, gpu, compress=True, batched=per, read_length=L)
+    oracle_lib.run(fa, so, cpu, compress=True, read_length=L)
+    for sfx in SUFFIXES:
+        assert filecmp.cmp(str(gpu) + sfx, str(cpu) + sfx, shallow=False), f"{sfx} differs"
+    dec = tmp_path / "dec.cns"
+    gsnp_amd.decompress(gpu, dec)
+    ct = tmp_path / "cpu.cns"
+    oracle_lib.run(fa, so, ct, read_length=L)
+    assert dec.read_bytes() == ct.read_bytes(), _first_diff(dec, ct)
+
+
 def test_compressed_long_span(built, tmp_path):
-    """A 5 Mb chromosome (20 windows, more than one encoder pass of 16 windows):
+    """A 9 Mb chromosome (35 windows, more than one encoder pass of 32 windows):
     the -C files of one batched span equal those of the per-window path, and
     decompress to the batched span's text."""
     import gsnp_amd
 
     pre = tmp_path / "c2h"
-    _, rl = gsnp_amd.synth(2, 7, 0.5, pre)
+    _, rl = gsnp_amd.synth(2, 7, 0.9, pre)
     fa, so, snp = f"{pre}.fa", f"{pre}.soap", f"{pre}.snp"
     flags = dict(read_length=rl, refine_mode=1)
     print("[parity] -C long span", flush=True)
