@@ -38,9 +38,11 @@ sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, 8 TB/s
 WORKLOADS = {
+    1: "100 kb reference, 20x 100 bp reads",
     2: "10 Mb reference, 30x 100 bp pairs (insert 500+-50), 3% 2-hit reads, 1% known SNPs, refine mode",
     3: "24 contigs log-uniform 1-20 Mb, 100 bp pairs, depth per 10 kb ~ Gamma(4, 7.5), 1% repeat-like blocks "
        "at depth 200-255 with 1-3 hits, 2% uncovered blocks",
+    4: "2 Mb reference, 30x 100 bp reads, miscalibrated qualities (delta 3 above q30, cycle offset to 2)",
     5: "248 Mb reference, 30x 150 bp pairs (insert 400+-40), cycle-decaying qualities, rank-sum test",
 }
 TRAFFIC_PROFILE = {3: ROOT / "profiles" / "fused_stage_c3_hbm_traffic.md"}
@@ -121,9 +123,17 @@ def cpu_leg(args, work: Path, lens: list[int], sites_total: int) -> dict:
     pick.sort()
     pre = work / "cpu_sample"
     t0 = time.time()
-    gsnp_amd.synth_subset(args.config, args.seed, args.scale, pick, pre)
-    print(f"[bench] cpu sample: contigs {pick} ({tot / 1e6:.1f} Mb) written in {time.time() - t0:.1f}s",
-          file=sys.stderr, flush=True)
+    scale = args.scale
+    if tot > 2 * args.cpu_mb * 1e6:
+        # one contig far larger than the sample (the 248 Mb workload): the same
+        # generator at a smaller scale, which keeps its depth, read and quality
+        # model, gives a contig of about cpu_mb
+        scale = args.scale * args.cpu_mb * 1e6 / tot
+        pick = [0]
+        tot = gsnp_amd.synth_contigs(args.config, args.seed, scale)[0]
+    gsnp_amd.synth_subset(args.config, args.seed, scale, pick, pre)
+    print(f"[bench] cpu sample: contigs {pick} at scale {scale:.4g} ({tot / 1e6:.1f} Mb) written in "
+          f"{time.time() - t0:.1f}s", file=sys.stderr, flush=True)
     base = [str(oracle_lib.ORACLE_CLI), "-i", f"{pre}.soap", "-d", f"{pre}.fa", "-L", str(args.read_len), "-R"]
     if args.config == 5:
         base += ["-u"]
@@ -140,9 +150,11 @@ def cpu_leg(args, work: Path, lens: list[int], sites_total: int) -> dict:
     (work / "cpu1.cns").unlink(missing_ok=True)
     (work / "cpuN.cns").unlink(missing_ok=True)
     names = ", ".join(f"chr{i + 1}" for i in pick)
+    what = (f"contigs {names}" if scale == args.scale else
+            f"one contig of the same generator at scale {scale:.4g}")
     return {"value": round(many["sites"] / many["seconds_call"], 1), "unit": "sites/s", "cores": threads,
             "kind": "port", "cpu_model": cpu_model(),
-            "sample": f"contigs {names} ({many['sites']} sites, {many['windows']} windows) of the same workload as "
+            "sample": f"{what} ({many['sites']} sites, {many['windows']} windows) of the same workload as "
                       f"files; the CPU oracle calling windows on {threads} threads, .cns text written; call stage "
                       f"only (parsing excluded)",
             "single_thread": {"value": round(one["sites"] / one["seconds_call"], 1), "unit": "sites/s", "cores": 1,
