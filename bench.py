@@ -232,7 +232,8 @@ def main() -> int:
     value = sites / secs
     cnt = job.counters(reset=True)
     recs = cnt["records"] // max(1, args.steps + args.warmup)
-    fixups = {"host_fixups_per_pass": cnt["host_fixups"] // max(1, args.steps + args.warmup)}
+    fixups = {"host_fixups_per_pass": cnt["host_fixups"] // max(1, args.steps + args.warmup),
+              "device_normal_per_pass": cnt["device_normal"] // max(1, args.steps + args.warmup)}
     print(f"[bench] batched, rows to host: {secs * 1e3:.1f} ms/pass, {value:.4g} sites/s", file=sys.stderr,
           flush=True)
     d_secs, _, _ = timed_passes(job, args.steps, 1, per, 1)

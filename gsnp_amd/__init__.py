@@ -440,10 +440,11 @@ class Job:
         return {"smallest_network": 8 << cfg[0], "waves_per_block": list(cfg[1:7]), "grid_cap": list(cfg[7:13])}
 
     def counters(self, reset: bool = True) -> dict:
-        """Records called and sites finished on the host since the last reset (gsnp_counters)."""
-        v = (ctypes.c_uint64 * 3)()
-        check(lib().gsnp_counters(self.ctx, v, 3, int(reset)))
-        return {"records": v[0], "host_fixups": v[1], "host_fixups_normal": v[2]}
+        """Records called, sites finished on the host and rank-sum normal
+        approximations finished on the device since the last reset (gsnp_counters)."""
+        v = (ctypes.c_uint64 * 4)()
+        check(lib().gsnp_counters(self.ctx, v, 4, int(reset)))
+        return {"records": v[0], "host_fixups": v[1], "host_fixups_normal": v[2], "device_normal": v[3]}
 
     def records(self, reset: bool = True) -> int:
         v = ctypes.c_uint64()

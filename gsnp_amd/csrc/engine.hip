@@ -672,7 +672,8 @@ int gsnp_window_posterior(gsnp_ctx* c, uint32_t C)
     c->timer.end(kStPosterior, c->stream);
     if (c->p.rank_sum_mode) {
         c->timer.begin(kStRank, c->stream);
-        launch_rank(c->stream, c->tables(), a, c->jobs.p, c->job_count.p, C, res, c->fixes.p, c->fix_count.p);
+        launch_rank(c->stream, c->tables(), a, c->jobs.p, c->job_count.p, C, res, c->fixes.p, c->fix_count.p,
+                    c->stat.p + 1);
         c->timer.end(kStRank, c->stream);
         HIPCHK(hipGetLastError());
         c->fix_offset = 0;
@@ -885,13 +886,13 @@ int gsnp_stats(gsnp_ctx* c, uint64_t* n_records, int reset)
 int gsnp_counters(gsnp_ctx* c, uint64_t* out, int n, int reset)
 {
     if (!c || (n > 0 && !out)) return fail(GSNP_E_ARG, "null argument");
-    unsigned long long rec = 0;
-    HIPCHK(hipMemcpyAsync(&rec, c->stat.p, 8, hipMemcpyDeviceToHost, c->stream));
+    unsigned long long st[2] = {0, 0};  // records called, normal approximations finished on the device
+    HIPCHK(hipMemcpyAsync(st, c->stat.p, 16, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
-    const uint64_t v[3] = {rec, c->fixups_total, c->fixups_normal};
-    for (int i = 0; i < n && i < 3; i++) out[i] = v[i];
+    const uint64_t v[4] = {st[0], c->fixups_total, c->fixups_normal, st[1]};
+    for (int i = 0; i < n && i < 4; i++) out[i] = v[i];
     if (reset) {
-        HIPCHK(hipMemsetAsync(c->stat.p, 0, 8, c->stream));
+        HIPCHK(hipMemsetAsync(c->stat.p, 0, 16, c->stream));
         HIPCHK(hipStreamSynchronize(c->stream));
         c->fixups_total = c->fixups_normal = 0;
     }
@@ -947,7 +948,7 @@ void Ctx::span_enqueue(uint32_t start, uint32_t n)
     if (p.rank_sum_mode) {
         Results res = result_view(0);
         timer.begin(kStRank, stream);
-        launch_rank(stream, tables(), a, jobs.p, job_count.p, n, res, fixes.p, fix_count.p);
+        launch_rank(stream, tables(), a, jobs.p, job_count.p, n, res, fixes.p, fix_count.p, stat.p + 1);
         timer.end(kStRank, stream);
     }
 }

@@ -131,8 +131,10 @@ void launch_fused(hipStream_t s, const FusedArgs& f, const uint32_t* lists, uint
                   uint32_t max_sites, const FusedTune& tune);
 void launch_fused_one(hipStream_t s, const FusedArgs& f, const uint32_t* lists, uint32_t list_pitch,
                       const uint32_t* counts, uint32_t max_sites, uint32_t cls, uint32_t wpb, uint32_t max_blocks);
+// normal_count: incremented once per site whose normal approximation the
+// device finished
 void launch_rank(hipStream_t s, const Tables& t, const SiteArrays& a, const RankJob* jobs, const uint32_t* job_count,
-                 uint32_t max_jobs, Results& out, FixJob* fixes, uint32_t* fix_count);
+                 uint32_t max_jobs, Results& out, FixJob* fixes, uint32_t* fix_count, unsigned long long* normal_count);
 void launch_apply_fix(hipStream_t s, const FixJob* fixes, const double* rank, const int32_t* qcns, uint32_t n,
                       Results& out);
 // row lengths and block sums; rows that need the exact %#g routine are listed

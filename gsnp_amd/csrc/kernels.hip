@@ -7,6 +7,7 @@
 
 #include "gsnp_common.h"
 #include "kernels.h"
+#include "nearest.h"
 #include "walk.h"
 
 namespace gsnp {
@@ -1163,12 +1164,14 @@ void launch_fused(hipStream_t st, const FusedArgs& f, const uint32_t* lists, uin
 
 // ---------------------------------------------------------------------------
 // (e) Wilcoxon rank-sum test on het calls (kernel/posterior_kernel.cu:506-572,
-// cpp/rank_sum.cc:51-58).  One wave per job: lane q counts the records of the
+// cpp/rank_sum.cc:44-58).  One wave per job: lane q counts the records of the
 // two alleles with quality q, a wave prefix sum gives the mid-ranks, and
 // rank sums T1/T2 (exact half-integers) are reduced across lanes.  N < 64
 // uses the exact table (including its <<16 interpolation index); N >= 64
-// (normal approximation, host exp) and any log10 within 1e-9 of an integer
-// boundary go to the host fix-up list.
+// takes the normal approximation, whose exp and sqrt are evaluated so that
+// they give the host libm's doubles (below).  A site whose exp or sqrt lies
+// too close to a rounding boundary to be sure of that, or whose log10 lies
+// within 1e-9 of an integer boundary, goes to the host fix-up list.
 
 __device__ __forceinline__ double wave_sum(double v)
 {
@@ -1177,7 +1180,8 @@ __device__ __forceinline__ double wave_sum(double v)
 }
 
 __global__ __launch_bounds__(256) void k_rank(Tables t, SiteArrays a, const RankJob* jobs, const uint32_t* job_count,
-                                              Results out, FixJob* fixes, uint32_t* fix_count)
+                                              Results out, FixJob* fixes, uint32_t* fix_count,
+                                              unsigned long long* normal_count)
 {
     const uint32_t count = *job_count;
     const uint32_t lane = lane_id();
@@ -1237,7 +1241,10 @@ __global__ __launch_bounds__(256) void k_rank(Tables t, SiteArrays a, const Rank
                 rank = p[i1] + (tb - (int)tb) * (p[i2] - p[i1]);
             }
         } else {
-            need_host = true;
+            bool hard = false;
+            rank = normal_test_nearest(n1, n2, ta, tb, hard);
+            need_host = hard;
+            if (!hard) atomicAdd(normal_count, 1ull);
         }
         int q_cns = 0;
         if (!need_host) {
@@ -1284,12 +1291,13 @@ __global__ __launch_bounds__(256) void k_rank(Tables t, SiteArrays a, const Rank
 }
 
 void launch_rank(hipStream_t st, const Tables& t, const SiteArrays& a, const RankJob* jobs, const uint32_t* job_count,
-                 uint32_t max_jobs, Results& out, FixJob* fixes, uint32_t* fix_count)
+                 uint32_t max_jobs, Results& out, FixJob* fixes, uint32_t* fix_count, unsigned long long* normal_count)
 {
     uint32_t blocks = (max_jobs + 3) / 4;
     if (blocks > 4096) blocks = 4096;
     if (blocks == 0) blocks = 1;
-    hipLaunchKernelGGL(k_rank, dim3(blocks), dim3(256), 0, st, t, a, jobs, job_count, out, fixes, fix_count);
+    hipLaunchKernelGGL(k_rank, dim3(blocks), dim3(256), 0, st, t, a, jobs, job_count, out, fixes, fix_count,
+                       normal_count);
 }
 
 __global__ void k_apply_fix(const FixJob* fixes, const double* rank, const int32_t* qcns, uint32_t n, Results out)

@@ -299,9 +299,11 @@ int gsnp_profile(gsnp_ctx* ctx, int enable, double* stage_ms, uint64_t* stage_co
 int gsnp_stats(gsnp_ctx* ctx, uint64_t* n_records, int reset);
 
 /* Counters since the last reset, the first n of: [0] observation records
- * called, [1] sites finished on the host with the host libm (rank-sum normal
- * approximation or a consensus quality within the guard band of an integer),
- * [2] those of [1] that took the normal approximation (n1 + n2 >= 64). */
+ * called, [1] sites finished on the host with the host libm (a rank-sum
+ * normal approximation whose exp or sqrt lies too close to a rounding
+ * boundary, or a consensus quality within the guard band of an integer),
+ * [2] those of [1] that took the normal approximation (n1 + n2 >= 64),
+ * [3] normal approximations the device finished. */
 int gsnp_counters(gsnp_ctx* ctx, uint64_t* out, int n, int reset);
 
 /* Session over input files: parses FASTA, SOAP and known SNPs, trains (or

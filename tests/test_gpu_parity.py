@@ -571,6 +571,32 @@ def test_whole_pass_rows_reach_host_memory(datasets, name):
         job.close()
 
 
+@pytest.mark.parametrize("name", ["stress", "c3s"])
+@pytest.mark.parametrize("per", [0, 4])
+def test_rank_sum_normal_approximation_on_device(datasets, tmp_path, name, per):
+    """Het sites with n1 + n2 >= 64 (stress: about 490, c3s: about 40, in its
+    repeat-like blocks) take the rank-sum test's normal approximation; the
+    device finishes it, with exp and sqrt giving the host libm's doubles,
+    except where a value lies within 2 % of an ulp of a rounding boundary.
+    The text is the oracle's, and the counters show both paths."""
+    import gsnp_amd
+
+    ds = datasets(name)
+    ref = ds.oracle_text()
+    job = gsnp_amd.Job(ds.fasta, ds.soap, dbsnp=ds.dbsnp, **ds.flags)
+    try:
+        job.counters(reset=True)
+        out = tmp_path / f"{name}.n{per}.cns"
+        job.write(out, batched=per)
+        c = job.counters(reset=True)
+        print(f"[parity] {name} batched={per} rank-sum counters {c}", flush=True)
+        assert out.read_bytes() == ref.read_bytes(), _first_diff(out, ref)
+        assert c["device_normal"] > 0
+        assert c["host_fixups_normal"] < c["device_normal"]
+    finally:
+        job.close()
+
+
 REGIONS = {
     # c2w3 (one 600 kb chromosome, windows 0-2): a region inside window 0, one
     # whose read-length extension crosses into window 1 (window 2 is not
