@@ -183,6 +183,45 @@ def test_compressed_output(datasets, tmp_path, monkeypatch, name, per, lds):
     assert txt.read_bytes() == ref.read_bytes(), _first_diff(txt, ref)
 
 
+@pytest.mark.parametrize("per", [0, 2])
+def test_longest_reads_and_capped_sites_match_oracle(built, tmp_path, per):
+    """Reads of the largest length (256: cycles up to 255 fill the record's
+    cycle field), shorter reads among them, 2- and 3-hit reads, and a pile-up
+    of 300 reads at one position, whose sites pass the 255-record cap: the
+    calibration, the text (per-window and batched) and the -C files are the
+    oracle's."""
+    import random
+
+    import gsnp_amd
+
+    rng = random.Random(23)
+    G, L = 300000, 256
+    ref = "".join(rng.choice("ACGT") for _ in range(G))
+    fa, so = tmp_path / "g.fa", tmp_path / "d.soap"
+    fa.write_text(">chrL\n" + "\n".join(ref[i:i + 80] for i in range(0, G, 80)) + "\n")
+    starts = [rng.randrange(1, G - L - 1) for _ in range(G * 25 // L)] + [150001] * 300
+    lines = []
+    for i, p in enumerate(sorted(starts)):
+        n = L if rng.random() < 0.8 else rng.randint(40, L - 1)
+        seq = [b if rng.random() >= 0.01 else rng.choice("ACGT") for b in ref[p - 1:p - 1 + n]]
+        q = "".join(chr(64 + rng.randint(2, 40)) for _ in range(n))
+        hits = 1 if rng.random() < 0.9 else rng.randint(2, 3)
+        lines.append(f"r{i}\t{''.join(seq)}\t{q}\t{hits}\ta\t{n}\t{rng.choice('+-')}\tchrL\t{p}\t0\n")
+    so.write_text("".join(lines))
+    print(f"[parity] longest reads and capped sites batched={per}", flush=True)
+    gt, ct = tmp_path / "gpu.cns", tmp_path / "cpu.cns"
+    gsnp_amd.run(fa, so, gt, batched=per, read_length=L)
+    oracle_lib.run(fa, so, ct, read_length=L)
+    assert gt.read_bytes() == ct.read_bytes(), _first_diff(gt, ct)
+    deep = [ln for ln in ct.read_text().splitlines() if int(ln.split("\t")[13]) > 255]
+    assert deep, "no site past the 255-record cap"
+    gpu, cpu = tmp_path / "gpu", tmp_path / "cpu"
+    gsnp_amd.run(fa, so, gpu, compress=True, batched=per, read_length=L)
+    oracle_lib.run(fa, so, cpu, compress=True, read_length=L)
+    for sfx in SUFFIXES:
+        assert filecmp.cmp(str(gpu) + sfx, str(cpu) + sfx, shallow=False), f"{sfx} differs"
+
+
 @pytest.mark.parametrize("per", [0, 4])
 def test_compressed_many_distinct_values(built, tmp_path, per):
     """Reads with random hit counts give a window tens of thousands of
