@@ -940,13 +940,26 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(S <= 128 ? 
             for (uint32_t k0 = 0; k0 < NP; k0 += CH) {
                 uint32_t v[CH];
                 uint32_t pad = 0;  // bit k: my slot of pass k is padding
+                // every lane loads (a padding lane's slot lies in its site's
+                // row, an idle group's in site 0's, and on cache lines the
+                // site's records touch anyway), so that no load waits behind
+                // a branch; padding slots are set to all ones afterwards
+                uint32_t nk[CH];
 #pragma unroll
                 for (uint32_t k = 0; k < CH; k++) {
                     const uint32_t gi = (k0 + k) * P + sub;
-                    const uint32_t st = (uint32_t)__shfl((int)site, (int)gi);
-                    const uint32_t nn = (uint32_t)__shfl((int)n, (int)gi);
-                    v[k] = e < nn ? __builtin_nontemporal_load(f.a.rec + (size_t)st * kSlots + e) : 0xFFFFFFFFu;
+                    uint32_t st;
+                    if constexpr (P == 1) {  // one site per pass: its index and count in scalar registers
+                        st = (uint32_t)__builtin_amdgcn_readlane((int)site, (int)(k0 + k));
+                        nk[k] = (uint32_t)__builtin_amdgcn_readlane((int)n, (int)(k0 + k));
+                    } else {
+                        st = (uint32_t)__shfl((int)site, (int)gi);
+                        nk[k] = (uint32_t)__shfl((int)n, (int)gi);
+                    }
+                    v[k] = __builtin_nontemporal_load(f.a.rec + (size_t)st * kSlots + e);
                 }
+#pragma unroll
+                for (uint32_t k = 0; k < CH; k++) v[k] = e < nk[k] ? v[k] : 0xFFFFFFFFu;
 #pragma unroll
                 for (uint32_t k = 0; k < CH; k += 2) bitonic_all2<2, S>(v[k], v[k + 1], e);  // CH is even
 #pragma unroll
@@ -1017,12 +1030,13 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(S <= 128 ? 
             uint32_t v[G][R];
 #pragma unroll
             for (uint32_t gi = 0; gi < G; gi++) {
-                const uint32_t st = (uint32_t)__shfl((int)site, (int)gi);
-                const uint32_t nn = (uint32_t)__shfl((int)n, (int)gi);
+                const uint32_t st = (uint32_t)__builtin_amdgcn_readlane((int)site, (int)gi);
+                const uint32_t nn = (uint32_t)__builtin_amdgcn_readlane((int)n, (int)gi);
 #pragma unroll
-                for (int r = 0; r < R; r++) {
+                for (int r = 0; r < R; r++) {  // unconditional, as above: e < S <= kSlots
                     const uint32_t e = (uint32_t)r * 64u + lane;
-                    v[gi][r] = e < nn ? f.a.rec[(size_t)st * kSlots + e] : 0xFFFFFFFFu;
+                    const uint32_t w = f.a.rec[(size_t)st * kSlots + e];
+                    v[gi][r] = e < nn ? w : 0xFFFFFFFFu;
                 }
             }
             const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
