@@ -148,6 +148,7 @@ def cpu_leg(args, work: Path, lens: list[int], sites_total: int) -> dict:
                          check=True)
     many = json.loads(res.stdout.strip().splitlines()[-1])
     (work / "cpu1.cns").unlink(missing_ok=True)
+    cli = cli_leg(args, pre, work / "cpuN.cns", many["sites"])
     (work / "cpuN.cns").unlink(missing_ok=True)
     names = ", ".join(f"chr{i + 1}" for i in pick)
     what = (f"contigs {names}" if scale == args.scale else
@@ -158,7 +159,37 @@ def cpu_leg(args, work: Path, lens: list[int], sites_total: int) -> dict:
                       f"files; the CPU oracle calling windows on {threads} threads, .cns text written; call stage "
                       f"only (parsing excluded)",
             "single_thread": {"value": round(one["sites"] / one["seconds_call"], 1), "unit": "sites/s", "cores": 1,
-                              "sample": f"first {one['sites']} sites ({one['windows']} windows) of the same sample"}}
+                              "sample": f"first {one['sites']} sites ({one['windows']} windows) of the same sample"},
+            "gpu_cli": cli}
+
+
+def cli_leg(args, pre: Path, want: Path, sites: int) -> dict:
+    """The drop-in command line end to end on the CPU sample's files: parse
+    the FASTA and SOAP files, calibrate, upload, call in batched spans and
+    write the .cns text; its output is compared with the CPU oracle's."""
+    import filecmp
+
+    import gsnp_amd
+
+    exe = gsnp_amd.LIB_PATH.parent / "gsnp"
+    out = pre.parent / "cli.cns"
+    cmd = [str(exe), "-i", f"{pre}.soap", "-d", f"{pre}.fa", "-o", str(out), "-L", str(args.read_len),
+           "-B", str(args.per if args.per > 0 else 1 << 20)]
+    if args.config == 5:
+        cmd += ["-u"]
+    snp = Path(f"{pre}.snp")
+    if snp.exists():
+        cmd += ["-s", str(snp), "-2"]
+    t0 = time.perf_counter()
+    subprocess.run(cmd, check=True, capture_output=True)
+    wall = time.perf_counter() - t0
+    same = filecmp.cmp(out, want, shallow=False)
+    out.unlink(missing_ok=True)
+    print(f"[bench] gsnp command line on the CPU sample: {wall:.2f}s end to end, output equals the oracle's: {same}",
+          file=sys.stderr, flush=True)
+    return {"wall_s": round(wall, 3), "sites": sites, "sites_per_s": round(sites / wall, 1), "matches_oracle": same,
+            "note": "the gsnp binary end to end (parse, calibrate, upload, call in batched spans, write .cns) on the "
+                    "CPU sample's files, output compared with the oracle's"}
 
 
 def main() -> int:
@@ -276,6 +307,17 @@ def main() -> int:
                     "t_call": {"achieved": round(t_call_gbs, 2), "frac": round(t_call_gbs / HBM_PEAK_GBS, 4),
                                "note": "the same 4n + 80 bytes per site over the whole timed pass, rows to host"}}
 
+    # the window build (reads -> records), as its own figure: 2 B per
+    # observed base read (base and quality), 4 B per record written, 12 B of
+    # metadata per read
+    site_build = None
+    if stage_ms.get("site_build"):
+        sb_bytes = 6 * recs + 12 * reads
+        site_build = {"ms_per_pass": round(stage_ms["site_build"], 4), "bytes_per_pass": int(sb_bytes),
+                      "achieved": round(sb_bytes / (stage_ms["site_build"] / 1e3) / 1e9, 2), "unit": "GB/s",
+                      "note": "2 B per base and 4 B per record over the reads' 12 B of metadata each; k_read_ranges + "
+                              "k_site_build"}
+
     columns = None
     if not args.no_columns:
         c_secs, c_bytes, _ = timed_passes(job, args.steps, 1, per, 3)
@@ -311,6 +353,7 @@ def main() -> int:
         "other_mode": plain,
         "roofline": roofline,
         "stage_ms_per_step": {k: round(v, 4) for k, v in stage_ms.items()},
+        "site_build": site_build,
         "rank_sum": fixups,
         "cpu_baseline": cpu,
     }
