@@ -215,15 +215,19 @@ struct ChromState {
 // into one of two device buffers while the copy of the previous span's rows
 // to pinned host memory runs on a stream of its own.
 struct TextPipe {
-    DevBuf<char> dev[2];
-    char* host[2] = {nullptr, nullptr};
-    uint64_t host_cap[2] = {0, 0};
-    uint64_t len[2] = {0, 0};
-    bool copying[2] = {false, false};  // a copy into host[i] is queued and not yet handed on
-    bool recorded[2] = {false, false};  // ev_done[i] has been recorded
-    int cur = 0;                        // buffer the next span is formatted into
+    // spans whose rows may be in flight at once: while one span is computed,
+    // the copies of up to three earlier ones stay queued, so that a short span
+    // followed by a long one leaves no gap on the copy engine
+    static constexpr int kBufs = 4;
+    DevBuf<char> dev[kBufs];
+    char* host[kBufs] = {};
+    uint64_t host_cap[kBufs] = {};
+    uint64_t len[kBufs] = {};
+    bool recorded[kBufs] = {};  // ev_done[i] has been recorded
+    int cur = 0;                // buffer the next span is formatted into
+    int pending = 0;            // spans queued and not yet handed on, the oldest in buffer cur - pending
     hipStream_t copy = nullptr;
-    hipEvent_t ev_done[2] = {nullptr, nullptr};  // [i]: host[i] holds its span's rows
+    hipEvent_t ev_done[kBufs] = {};  // [i]: host[i] holds its span's rows
     hipEvent_t ev_fmt = nullptr;
     TextPipe() = default;
     TextPipe(const TextPipe&) = delete;
@@ -233,8 +237,9 @@ struct TextPipe {
         if (copy) (void)hipStreamSynchronize(copy);
         for (char* h : host)
             if (h) (void)hipHostFree(h);
-        for (hipEvent_t e : {ev_done[0], ev_done[1], ev_fmt})
+        for (hipEvent_t e : ev_done)
             if (e) (void)hipEventDestroy(e);
+        if (ev_fmt) (void)hipEventDestroy(ev_fmt);
         if (copy) (void)hipStreamDestroy(copy);
     }
 };
@@ -329,8 +334,9 @@ struct Ctx {
     void swap_chrom(ChromState& s);
     // rows of the span just called (gsnp_batched_call without text) formatted
     // into the pipe's device buffer and their copy to pinned host memory queued
-    // on the copy stream; then the rows of the previous span, landed in host
-    // memory, are handed to consume(ptr, len).  finish_text hands on the last.
+    // on the copy stream; the rows of the span TextPipe::kBufs spans back are
+    // handed to consume(ptr, len) first, once landed, when the pipe is full.
+    // finish_text hands on the spans still queued, in order.
     using TextSink = std::function<int(const char*, uint64_t)>;
     int text_to_host(uint32_t start, uint32_t n, uint64_t* nbytes, const TextSink& consume);
     int finish_text(const TextSink& consume);

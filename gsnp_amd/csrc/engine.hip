@@ -1092,15 +1092,29 @@ int Ctx::format_text(uint32_t offset, uint32_t ws_abs, uint32_t n_sites, uint32_
     return GSNP_OK;
 }
 
+// Hands the oldest queued span's rows to consume(), once its copy has landed.
+static int hand_over_oldest(TextPipe& tp, const Ctx::TextSink& consume)
+{
+    const int o = (tp.cur - tp.pending + TextPipe::kBufs) % TextPipe::kBufs;
+    HIPCHK(hipEventSynchronize(tp.ev_done[o]));
+    tp.pending--;
+    return consume(tp.host[o], tp.len[o]);
+}
+
 int Ctx::text_to_host(uint32_t start, uint32_t n, uint64_t* nbytes, const TextSink& consume)
 {
     TextPipe& tp = pipe;
     if (!tp.copy) {
         HIPCHK(hipStreamCreateWithFlags(&tp.copy, hipStreamNonBlocking));
-        for (hipEvent_t* e : {&tp.ev_done[0], &tp.ev_done[1], &tp.ev_fmt})
-            HIPCHK(hipEventCreateWithFlags(e, hipEventDisableTiming));
+        for (hipEvent_t& e : tp.ev_done) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        HIPCHK(hipEventCreateWithFlags(&tp.ev_fmt, hipEventDisableTiming));
     }
-    const int i = tp.cur, j = i ^ 1;
+    // buffer cur last held the span kBufs spans back: hand it on first
+    if (tp.pending == TextPipe::kBufs) {
+        int rc = hand_over_oldest(tp, consume);
+        if (rc) return rc;
+    }
+    const int i = tp.cur;
     const uint64_t need = text_bound(n, chr_name.size());
     if (tp.dev[i].n < need) {
         // nothing may still read the old buffer: its copy, or a graph that writes it
@@ -1109,13 +1123,12 @@ int Ctx::text_to_host(uint32_t start, uint32_t n, uint64_t* nbytes, const TextSi
         HIPCHK(tp.dev[i].alloc(need + need / 8));
         buf_gen++;
     }
-    // the copy of the span formatted into this buffer two spans ago reads it
+    // the copy of the span formatted into this buffer kBufs spans ago reads it
     if (tp.recorded[i]) HIPCHK(hipStreamWaitEvent(stream, tp.ev_done[i], 0));
     uint64_t len = 0;
     int rc = format_text(0, start, n, start + 1, &len, true, tp.dev[i].p);
     if (rc) return rc;
-    if (tp.host_cap[i] < len) {
-        if (tp.recorded[i]) HIPCHK(hipEventSynchronize(tp.ev_done[i]));
+    if (tp.host_cap[i] < len) {  // that span has been handed on, and its copy has landed
         if (tp.host[i]) HIPCHK(hipHostFree(tp.host[i]));
         tp.host[i] = nullptr;
         tp.host_cap[i] = 0;
@@ -1129,15 +1142,9 @@ int Ctx::text_to_host(uint32_t start, uint32_t n, uint64_t* nbytes, const TextSi
     HIPCHK(hipEventRecord(tp.ev_done[i], tp.copy));
     tp.recorded[i] = true;
     tp.len[i] = len;
-    tp.copying[i] = true;
-    tp.cur = j;
+    tp.cur = (i + 1) % TextPipe::kBufs;
+    tp.pending++;
     if (nbytes) *nbytes = len;
-    // the previous span's rows: landed by now or soon (its copy ran beside this span's kernels)
-    if (tp.copying[j]) {
-        HIPCHK(hipEventSynchronize(tp.ev_done[j]));
-        tp.copying[j] = false;
-        return consume(tp.host[j], tp.len[j]);
-    }
     return GSNP_OK;
 }
 
@@ -1145,7 +1152,7 @@ int Ctx::reset_text()
 {
     TextPipe& tp = pipe;
     if (tp.copy) HIPCHK(hipStreamSynchronize(tp.copy));
-    tp.copying[0] = tp.copying[1] = false;
+    tp.pending = 0;
     tp.cur = 0;
     return GSNP_OK;
 }
@@ -1170,11 +1177,11 @@ int Ctx::window_text_to_host(const TextSink& consume)
 int Ctx::finish_text(const TextSink& consume)
 {
     TextPipe& tp = pipe;
-    const int j = tp.cur ^ 1;  // the last span queued
-    if (!tp.copying[j]) return GSNP_OK;
-    HIPCHK(hipEventSynchronize(tp.ev_done[j]));
-    tp.copying[j] = false;
-    return consume(tp.host[j], tp.len[j]);
+    while (tp.pending) {  // the spans still queued, in order
+        int rc = hand_over_oldest(tp, consume);
+        if (rc) return rc;
+    }
+    return GSNP_OK;
 }
 
 }  // namespace gsnp
