@@ -222,6 +222,50 @@ def test_longest_reads_and_capped_sites_match_oracle(built, tmp_path, per):
         assert filecmp.cmp(str(gpu) + sfx, str(cpu) + sfx, shallow=False), f"{sfx} differs"
 
 
+@pytest.mark.parametrize("per", [0, 3])
+def test_small_empty_and_edge_chromosomes_match_oracle(built, tmp_path, per):
+    """Chromosomes of 40 bp (shorter than -L), of 1000 bp without reads, of
+    one window plus 10 bp with reads at both ends, and of 500 N: the text
+    (per-window and batched) and the -C files are the oracle's."""
+    import random
+
+    import gsnp_amd
+
+    rng = random.Random(31)
+    W, L = 262144, 50
+    seqs = {
+        "chrA": "".join(rng.choice("ACGT") for _ in range(40)),
+        "chrB": "".join(rng.choice("ACGT") for _ in range(1000)),
+        "chrC": "".join(rng.choice("ACGT") for _ in range(W + 10)),
+        "chrD": "N" * 500,
+    }
+    fa, so = tmp_path / "g.fa", tmp_path / "d.soap"
+    fa.write_text("".join(f">{k}\n" + "\n".join(v[i:i + 60] for i in range(0, len(v), 60)) + "\n"
+                          for k, v in seqs.items()))
+    reads = []  # (chromosome, 1-based position, length)
+    reads += [("chrA", p, 30) for p in (1, 1, 5, 11, 11)]
+    C = len(seqs["chrC"])
+    reads += [("chrC", p, L) for p in sorted([1, 1, 2] + [rng.randrange(1, C - L + 2) for _ in range(W * 8 // L)]
+                                             + [C - L + 1, C - L + 1, C - L])]
+    lines = []
+    for i, (c, p, n) in enumerate(reads):
+        ref = seqs[c][p - 1:p - 1 + n]
+        seq = "".join(b if rng.random() >= 0.02 else rng.choice("ACGT") for b in ref)
+        q = "".join(chr(64 + rng.randint(2, 40)) for _ in range(n))
+        lines.append(f"r{i}\t{seq}\t{q}\t1\ta\t{n}\t{rng.choice('+-')}\t{c}\t{p}\t0\n")
+    so.write_text("".join(lines))
+    print(f"[parity] small, empty and edge chromosomes batched={per}", flush=True)
+    gt, ct = tmp_path / "gpu.cns", tmp_path / "cpu.cns"
+    gsnp_amd.run(fa, so, gt, batched=per, read_length=L)
+    oracle_lib.run(fa, so, ct, read_length=L)
+    assert gt.read_bytes() == ct.read_bytes(), _first_diff(gt, ct)
+    gpu, cpu = tmp_path / "gpu", tmp_path / "cpu"
+    gsnp_amd.run(fa, so, gpu, compress=True, batched=per, read_length=L)
+    oracle_lib.run(fa, so, cpu, compress=True, read_length=L)
+    for sfx in SUFFIXES:
+        assert filecmp.cmp(str(gpu) + sfx, str(cpu) + sfx, shallow=False), f"{sfx} differs"
+
+
 @pytest.mark.parametrize("per", [0, 4])
 def test_compressed_many_distinct_values(built, tmp_path, per):
     """Reads with random hit counts give a window tens of thousands of
