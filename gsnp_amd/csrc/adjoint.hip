@@ -15,9 +15,10 @@
 //
 // Accumulation is deterministic: the records are sorted once by score-row key
 // (stable LSD radix sort), each evaluation writes the per-record
-// contributions in key order, and a segmented reduce sums them in a fixed
-// order (pieces of at most 512 contiguous records, then the pieces of a key
-// in order).  No floating-point atomics.
+// contributions in key order (one thread per record), and a segmented reduce
+// sums them in a fixed order: each piece of at most 512 contiguous records by
+// one wave (a strided lane sum, then a fixed xor tree), then the pieces of a
+// key in order.  No floating-point atomics.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -329,54 +330,62 @@ __global__ __launch_bounds__(256) void k_adj_site(const double* T, const uint32_
     dh[s] = d;
 }
 
-// per record: its contribution to the four ln p entries of its row, stored at
-// the record's place in key order
-__global__ __launch_bounds__(256) void k_adj_contrib(const double* pm, const uint32_t* key, const uint64_t* off,
-                                                     const uint32_t* rank, const double* wsite, uint32_t n,
-                                                     double* c_sorted)
+// per record: the site that owns it (record order; sites own contiguous runs)
+__global__ __launch_bounds__(256) void k_adj_rsite(const uint64_t* off, uint32_t n, uint32_t* rsite)
 {
     const uint32_t s = blockIdx.x * 256u + threadIdx.x;
     if (s >= n) return;
-    const uint64_t r0 = off[s], r1 = off[s + 1];
-    if (r0 == r1) return;
-    double w[10];
-#pragma unroll
-    for (int g = 0; g < 10; g++) w[g] = wsite[(size_t)s * 10 + g];
-    for (uint64_t r = r0; r < r1; r++) {
-        const uint32_t k = key[r];
-        const uint32_t q = k >> 10, cyc = (k >> 2) & 255u, o = k & 3u;
-        const double* prow = pm + ((size_t)q << 12) + ((size_t)cyc << 4) + o;
-        double p[4];
-#pragma unroll
-        for (int a = 0; a < 4; a++) p[a] = prow[a << 2];
-        double c[4] = {0.0, 0.0, 0.0, 0.0};
-        int g = 0;
-#pragma unroll
-        for (int a1 = 0; a1 < 4; a1++)
-#pragma unroll
-            for (int a2 = a1; a2 < 4; a2++) {
-                if (a1 == a2) {
-                    c[a1] += w[g];
-                } else {
-                    const double inv = 1.0 / (p[a1] + p[a2]);  // one division per pair
-                    c[a1] += w[g] * (p[a1] * inv);
-                    c[a2] += w[g] * (p[a2] * inv);
-                }
-                g++;
-            }
-        double2* dst = reinterpret_cast<double2*>(c_sorted + (size_t)rank[r] * 4);
-        dst[0] = make_double2(c[0], c[1]);
-        dst[1] = make_double2(c[2], c[3]);
-    }
+    for (uint64_t r = off[s]; r < off[s + 1]; r++) rsite[r] = s;
 }
 
+// per record: its contribution to the four ln p entries of its row, stored at
+// the record's place in key order.  One thread per record (a thread per site
+// walked its records one after another, each waiting on its own loads); the
+// records of a site share its ten weights, which come from cache.
+__global__ __launch_bounds__(256) void k_adj_contrib(const double* pm, const uint32_t* key, const uint32_t* rsite,
+                                                     const uint32_t* rank, const double* wsite, uint32_t nr,
+                                                     double* c_sorted)
+{
+    const uint32_t r = blockIdx.x * 256u + threadIdx.x;
+    if (r >= nr) return;
+    const uint32_t s = rsite[r], k = key[r];
+    const uint32_t q = k >> 10, cyc = (k >> 2) & 255u, o = k & 3u;
+    const double* prow = pm + ((size_t)q << 12) + ((size_t)cyc << 4) + o;
+    double p[4], w[10];
+#pragma unroll
+    for (int a = 0; a < 4; a++) p[a] = prow[a << 2];
+#pragma unroll
+    for (int g = 0; g < 10; g++) w[g] = wsite[(size_t)s * 10 + g];
+    double c[4] = {0.0, 0.0, 0.0, 0.0};
+    int g = 0;
+#pragma unroll
+    for (int a1 = 0; a1 < 4; a1++)
+#pragma unroll
+        for (int a2 = a1; a2 < 4; a2++) {
+            if (a1 == a2) {
+                c[a1] += w[g];
+            } else {
+                const double inv = 1.0 / (p[a1] + p[a2]);  // one division per pair
+                c[a1] += w[g] * (p[a1] * inv);
+                c[a2] += w[g] * (p[a2] * inv);
+            }
+            g++;
+        }
+    double2* dst = reinterpret_cast<double2*>(c_sorted + (size_t)rank[r] * 4);
+    dst[0] = make_double2(c[0], c[1]);
+    dst[1] = make_double2(c[2], c[3]);
+}
+
+// one wave per piece: lane l adds records l, l + 64, ... of the piece, then
+// a fixed xor tree over the wave (a fixed order: the sums are the same on
+// every run)
 __global__ __launch_bounds__(256) void k_adj_pieces(const double* c_sorted, const uint32_t* pstart, uint32_t npieces,
                                                     double* psum)
 {
-    const uint32_t p = blockIdx.x * 256u + threadIdx.x;
-    if (p >= npieces) return;
+    const uint32_t p = blockIdx.x * 4u + (threadIdx.x >> 6), lane = threadIdx.x & 63u;
+    if (p >= npieces) return;  // wave-uniform
     double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-    for (uint32_t j = pstart[p]; j < pstart[p + 1]; j++) {
+    for (uint32_t j = pstart[p] + lane; j < pstart[p + 1]; j += 64u) {
         const double2* src = reinterpret_cast<const double2*>(c_sorted + (size_t)j * 4);
         const double2 x = src[0], y = src[1];
         s0 += x.x;
@@ -384,9 +393,17 @@ __global__ __launch_bounds__(256) void k_adj_pieces(const double* c_sorted, cons
         s2 += y.x;
         s3 += y.y;
     }
-    double2* dst = reinterpret_cast<double2*>(psum + (size_t)p * 4);
-    dst[0] = make_double2(s0, s1);
-    dst[1] = make_double2(s2, s3);
+    for (int off = 32; off > 0; off >>= 1) {
+        s0 += __shfl_xor(s0, off);
+        s1 += __shfl_xor(s1, off);
+        s2 += __shfl_xor(s2, off);
+        s3 += __shfl_xor(s3, off);
+    }
+    if (lane == 0) {
+        double2* dst = reinterpret_cast<double2*>(psum + (size_t)p * 4);
+        dst[0] = make_double2(s0, s1);
+        dst[1] = make_double2(s2, s3);
+    }
 }
 
 // gradient of one key's four entries, p_matrix layout q<<12 | c<<4 | a<<2 | o
@@ -521,6 +538,9 @@ extern "C" int gsnp_adjoint_prepare(gsnp_ctx* c, uint32_t start, uint32_t n)
     if (nrec >= 0xFFFFFFFFull) return fail(GSNP_E_RANGE, "span holds too many records for one adjoint");
     const uint32_t nr = (uint32_t)nrec;
     HIPCHK(c->adj_key.alloc(nr + 1));
+    HIPCHK(c->adj_rsite.alloc(nr + 1));
+    if (n)
+        hipLaunchKernelGGL(k_adj_rsite, dim3(blocks_for(n)), dim3(256), 0, c->stream, c->adj_off.p, n, c->adj_rsite.p);
     HIPCHK(c->adj_rank.alloc(nr + 1));
     HIPCHK(c->adj_skey.alloc(nr + 1));
     HIPCHK(c->adj_perm.alloc(nr + 1));
@@ -628,11 +648,12 @@ extern "C" int gsnp_adjoint_eval(gsnp_ctx* c, const double* p_matrix, double log
                            c->adj_T.p, c->adj_key.p, c->adj_off.p,
                            c->adj_prow.p, c->adj_lp.p, c->adj_mask.p, n, c->p.is_monoploid, c->adj_ll.p, c->adj_dh.p,
                            c->adj_w.p);
-        hipLaunchKernelGGL(k_adj_contrib, dim3(blocks_for(n)), dim3(256), 0, st, c->adj_pm.p, c->adj_key.p, c->adj_off.p,
-                           c->adj_rank.p, c->adj_w.p, n, c->adj_c.p);
+        if (nr)
+            hipLaunchKernelGGL(k_adj_contrib, dim3(blocks_for(nr)), dim3(256), 0, st, c->adj_pm.p, c->adj_key.p,
+                               c->adj_rsite.p, c->adj_rank.p, c->adj_w.p, nr, c->adj_c.p);
     }
     if (c->adj_npieces)
-        hipLaunchKernelGGL(k_adj_pieces, dim3(blocks_for(c->adj_npieces)), dim3(256), 0, st, c->adj_c.p,
+        hipLaunchKernelGGL(k_adj_pieces, dim3((c->adj_npieces + 3) / 4), dim3(256), 0, st, c->adj_c.p,
                            c->adj_pstart.p, c->adj_npieces, c->adj_psum.p);
     hipLaunchKernelGGL(k_adj_grad, dim3(blocks_for(kKeys)), dim3(256), 0, st, c->adj_psum.p, c->adj_kpiece.p,
                        c->adj_grad.p);
@@ -648,7 +669,6 @@ extern "C" int gsnp_adjoint_eval(gsnp_ctx* c, const double* p_matrix, double log
     HIPCHK(hipStreamSynchronize(st));
     if (loglik) *loglik = out2[0];
     if (grad_log_het) *grad_log_het = out2[1];
-    (void)nr;
     return GSNP_OK;
 }
 

@@ -320,6 +320,7 @@ struct Ctx {
     uint32_t adj_start = 0, adj_n = 0, adj_npieces = 0;
     uint64_t adj_nrec = 0;
     DevBuf<uint32_t> adj_cnt, adj_prow, adj_key, adj_skey, adj_perm, adj_rank, adj_tmp, adj_pstart, adj_kpiece;
+    DevBuf<uint32_t> adj_rsite;  // per record (record order): its site
     DevBuf<uint32_t> adj_hist, adj_mask;
     DevBuf<uint64_t> adj_off, adj_pidx, adj_hoff, adj_bsum;
     DevBuf<double> adj_T, adj_w, adj_ll, adj_dh, adj_c, adj_psum, adj_part, adj_out, adj_pm, adj_grad, adj_lp;
