@@ -223,6 +223,76 @@ def test_longest_reads_and_capped_sites_match_oracle(built, tmp_path, per):
 
 
 @pytest.mark.parametrize("per", [0, 3])
+def test_shared_anchor_reads_match_oracle(built, tmp_path, per):
+    """Reads that share their anchor (the start on '+', the last base on '-')
+    give a site records of one strand and cycle, which the dependency count
+    (kernel/likelihood_kernel.cu:168-256) counts.  chrA has such pairs and
+    triples here and there; chrB a block with a pair starting every 4 bp on
+    each strand (25 such anchors meet at a site); chrC pairs at every
+    position of a block (100 meet).  The text, per-window and batched, is
+    the oracle's."""
+    import random
+
+    import gsnp_amd
+
+    rng = random.Random(41)
+    L, G = 100, 30000
+    seqs = {c: "".join(rng.choice("ACGT") for _ in range(G)) for c in ("chrA", "chrB", "chrC")}
+    fa, so = tmp_path / "g.fa", tmp_path / "d.soap"
+    fa.write_text("".join(f">{k}\n" + "\n".join(v[i:i + 60] for i in range(0, len(v), 60)) + "\n"
+                          for k, v in seqs.items()))
+    reads = []  # (chromosome, 1-based position, length, strand)
+
+    def scatter(c, n):
+        for _ in range(n):
+            reads.append((c, rng.randrange(1, G - L - 1), L if rng.random() < 0.8 else rng.randint(30, L), rng.choice("+-")))
+
+    scatter("chrA", G * 20 // L)
+    for _ in range(60):  # pairs and triples on one anchor
+        p, st, k = rng.randrange(200, G - 300), rng.choice("+-"), rng.choice([2, 2, 3])
+        for _ in range(k):
+            n = L if st == "+" else rng.randint(60, L)  # '-' reads share their last base
+            reads.append(("chrA", p if st == "+" else p + L - n, n, st))
+    for c, step in (("chrB", 4), ("chrC", 1)):
+        scatter(c, G * 10 // L)
+        for a in range(10000, 10400, step):
+            for st in "+-":
+                for _ in range(2):
+                    n = rng.randint(70, L)
+                    reads.append((c, a if st == "+" else a + L - n, n, st))
+    reads.sort(key=lambda r: (r[0], r[1]))
+    # the most anchors held by two or more reads of one strand that meet at a
+    # site
+    for c, lo, hi in (("chrA", 1, 7), ("chrB", 8, 63), ("chrC", 64, 10 ** 6)):
+        most = 0
+        for st in "+-":
+            held = {}
+            for cc, p, n, s2 in reads:
+                if cc == c and s2 == st:
+                    a = p if st == "+" else p + n - 1
+                    held[a] = held.get(a, 0) + 1
+            dup = sorted(a for a, k in held.items() if k >= 2)
+            j = 0
+            for i, a in enumerate(dup):
+                while dup[j] <= a - L:
+                    j += 1
+                most = max(most, i - j + 1)
+        assert lo <= most <= hi, (c, most)
+    lines = []
+    for i, (c, p, n, st) in enumerate(reads):
+        ref = seqs[c][p - 1:p - 1 + n]
+        seq = "".join(b if rng.random() >= 0.03 else rng.choice("ACGT") for b in ref)
+        q = "".join(chr(64 + rng.randint(2, 40)) for _ in range(n))
+        lines.append(f"r{i}\t{seq}\t{q}\t1\ta\t{n}\t{st}\t{c}\t{p}\t0\n")
+    so.write_text("".join(lines))
+    print(f"[parity] shared-anchor reads batched={per}", flush=True)
+    gt, ct = tmp_path / "gpu.cns", tmp_path / "cpu.cns"
+    gsnp_amd.run(fa, so, gt, batched=per, read_length=L)
+    oracle_lib.run(fa, so, ct, read_length=L)
+    assert gt.read_bytes() == ct.read_bytes(), _first_diff(gt, ct)
+
+
+@pytest.mark.parametrize("per", [0, 3])
 def test_small_empty_and_edge_chromosomes_match_oracle(built, tmp_path, per):
     """Chromosomes of 40 bp (shorter than -L), of 1000 bp without reads, of
     one window plus 10 bp with reads at both ends, and of 500 N: the text
