@@ -761,14 +761,17 @@ __global__ __launch_bounds__(kDictThreads) void k_col_dict(DictArgs a)
 //                  elements are loaded coalesced, 16 per thread in flight, and
 //                  ranked (the word prefix plus the set bits below the value
 //                  in its word) into LDS, then each thread assembles words.
-// The copy numbers (doubles), with at most kHashRankMax distinct values per
+// The copy numbers (doubles), with at most kHashSortMax distinct values per
 // window:
 //  k_hash_insert   workgroups per array insert the values into an open-
 //                  addressed set in HBM (compare-and-swap), each element
 //                  keeping its slot and each new value appended to a list;
-//  k_hash_rank     one workgroup per array ranks the listed values (the count
-//                  of smaller ones), writes the dictionary and each slot's
-//                  rank, and empties the used slots for the next pass;
+//  k_hash_rank     one workgroup per array ranks the listed values (up to
+//                  kHashRankMax the count of smaller ones; above that the
+//                  position after a bitonic sort of (order key, list index)
+//                  pairs in LDS: the keys are distinct, so the two agree),
+//                  writes the dictionary and each slot's rank, and empties
+//                  the used slots for the next pass;
 //  k_dict_pack     as above, with an element's rank from its slot.
 // An array these do not cover is left (done flag 0) to k_col_dict.
 constexpr uint32_t kDictParts = 16;
@@ -777,7 +780,8 @@ constexpr uint32_t kIntDicts = kNDict - 1;  // all but the copy numbers (doubles
 constexpr uint32_t kPackK = 16;             // elements per thread of a pack tile
 constexpr uint32_t kPackTile = kPackK * kDictPartThreads;
 constexpr uint32_t kPackLds = 2048;         // bitmap words a pack workgroup keeps in LDS
-constexpr uint32_t kHashRankMax = 1024;     // distinct copy numbers ranked by k_hash_rank
+constexpr uint32_t kHashRankMax = 1024;     // distinct copy numbers ranked by counting
+constexpr uint32_t kHashSortMax = 8192;     // distinct copy numbers ranked by k_hash_rank (sorted above kHashRankMax)
 constexpr uint32_t kHashLg = 19;            // slots of a window's set: 2 kWin (load at most 1/2)
 constexpr uint32_t kHashSlots = 1u << kHashLg;
 static_assert(kHashSlots >= 2 * kWin, "the set holds a window's values at load 1/2");
@@ -1054,7 +1058,8 @@ __global__ __launch_bounds__(kDictPartThreads) void k_hash_insert(SplitArgs a)
 // grid (window): rank of each listed copy number, the dictionary in order
 __global__ __launch_bounds__(kDictThreads) void k_hash_rank(SplitArgs a)
 {
-    __shared__ uint64_t ok[kHashRankMax];
+    __shared__ uint64_t ok[kHashSortMax];
+    __shared__ uint16_t ix[kHashSortMax];
     const uint32_t w = blockIdx.x, t = threadIdx.x;
     const uint32_t m = dict_elems(a.tot, w, 5);
     const size_t wk = (size_t)w * kNDict + 5;
@@ -1062,7 +1067,7 @@ __global__ __launch_bounds__(kDictThreads) void k_hash_rank(SplitArgs a)
     const uint64_t* hk = a.hkeys + (size_t)w * kWin;
     const uint32_t* hs = a.hslot + (size_t)w * kWin;
     const uint32_t nd = a.state[3 * wk];  // written by the previous launch only
-    if (m == 0 || nd > kHashRankMax) {  // none, or left to k_col_dict: the set is emptied
+    if (m == 0 || nd > kHashSortMax) {  // none, or left to k_col_dict: the set is emptied
         for (uint32_t d = t; d < nd; d += kDictThreads) tab[hs[d]] = kEmpty;
         __syncthreads();  // every thread has read the count
         if (t == 0) {
@@ -1072,16 +1077,50 @@ __global__ __launch_bounds__(kDictThreads) void k_hash_rank(SplitArgs a)
         }
         return;
     }
-    for (uint32_t d = t; d < nd; d += kDictThreads) ok[d] = order_key(hk[d], true);
-    __syncthreads();
     uint32_t* rk = a.hrank + (size_t)w * kHashSlots;
-    for (uint32_t d = t; d < nd; d += kDictThreads) {
-        const uint64_t x = ok[d];
-        uint32_t r = 0;
-        for (uint32_t u = 0; u < nd; u++) r += ok[u] < x;  // all lanes read one word: a broadcast
-        rk[hs[d]] = r;
-        a.dict64[(size_t)w * kWin + r] = hk[d];
-        tab[hs[d]] = kEmpty;
+    if (nd <= kHashRankMax) {
+        for (uint32_t d = t; d < nd; d += kDictThreads) ok[d] = order_key(hk[d], true);
+        __syncthreads();
+        for (uint32_t d = t; d < nd; d += kDictThreads) {
+            const uint64_t x = ok[d];
+            uint32_t r = 0;
+            for (uint32_t u = 0; u < nd; u++) r += ok[u] < x;  // all lanes read one word: a broadcast
+            rk[hs[d]] = r;
+            a.dict64[(size_t)w * kWin + r] = hk[d];
+            tab[hs[d]] = kEmpty;
+        }
+    } else {
+        // (order key, list index) pairs over the next power of two, padded
+        // with keys above every value (no copy number is NaN), sorted
+        // ascending; the position of a value is its rank
+        uint32_t np = kHashRankMax;
+        while (np < nd) np <<= 1;  // <= kHashSortMax
+        for (uint32_t d = t; d < np; d += kDictThreads) {
+            ok[d] = d < nd ? order_key(hk[d], true) : ~0ull;
+            ix[d] = (uint16_t)d;
+        }
+        __syncthreads();
+        for (uint32_t k = 2; k <= np; k <<= 1)
+            for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+                for (uint32_t q = t; q < np / 2; q += kDictThreads) {
+                    const uint32_t i = 2u * j * (q / j) + q % j, l = i + j;  // pair (i, i + j), i & j == 0
+                    const uint64_t x = ok[i], y = ok[l];
+                    if ((x > y) == ((i & k) == 0)) {
+                        ok[i] = y;
+                        ok[l] = x;
+                        const uint16_t u = ix[i];
+                        ix[i] = ix[l];
+                        ix[l] = u;
+                    }
+                }
+                __syncthreads();
+            }
+        for (uint32_t r = t; r < nd; r += kDictThreads) {
+            const uint32_t d = ix[r];
+            rk[hs[d]] = r;
+            a.dict64[(size_t)w * kWin + r] = hk[d];
+            tab[hs[d]] = kEmpty;
+        }
     }
     __syncthreads();
     if (t == 0) {

@@ -372,6 +372,58 @@ def test_compressed_many_distinct_values(built, tmp_path, per):
 
 
 @pytest.mark.parametrize("per", [0, 4])
+def test_compressed_sorted_copy_number_ranks(built, tmp_path, per):
+    """Windows with a few thousand distinct copy numbers (column 16): about
+    3,400 on chrA and 5,900 on chrB, above the 1,024 the dictionary ranks by
+    counting and within the 8,192 it ranks by a sort in LDS.  The -C files and
+    the text are the oracle's."""
+    import random
+
+    import gsnp_amd
+
+    rng = random.Random(11)
+    G, L = 100000, 50
+    seqs = {c: "".join(rng.choice("ACGT") for _ in range(G)) for c in ("chrA", "chrB")}
+    fa, so = tmp_path / "g.fa", tmp_path / "d.soap"
+    fa.write_text("".join(f">{k}\n" + "\n".join(v[i:i + 80] for i in range(0, G, 80)) + "\n"
+                          for k, v in seqs.items()))
+    lines = []
+    for c, hmax in (("chrA", 30), ("chrB", 60)):
+        starts = sorted(rng.randrange(1, G - L - 1) for _ in range(G * 20 // L))
+        hits = [1 if rng.random() < 0.5 else rng.randint(2, hmax) for _ in starts]
+        # the window's distinct copy numbers, repeat / depth over covered sites
+        depth, rep = [0] * (G + 1), [0] * (G + 1)
+        for p, h in zip(starts, hits):
+            depth[p - 1] += 1
+            depth[p - 1 + L] -= 1
+            rep[p - 1] += h
+            rep[p - 1 + L] -= h
+        vals, dd, rr = set(), 0, 0
+        for x in range(G):
+            dd += depth[x]
+            rr += rep[x]
+            if dd:
+                vals.add(rr / dd)
+        assert 1024 < len(vals) + 1 <= 8192, (c, len(vals))
+        for p, h in zip(starts, hits):
+            ref = seqs[c][p - 1:p - 1 + L]
+            seq = "".join(b if rng.random() >= 0.01 else rng.choice("ACGT") for b in ref)
+            q = "".join(chr(64 + rng.randint(5, 40)) for _ in range(L))
+            lines.append(f"r{len(lines)}\t{seq}\t{q}\t{h}\ta\t{L}\t{rng.choice('+-')}\t{c}\t{p}\t0\n")
+    so.write_text("".join(lines))
+    print(f"[parity] -C sorted copy-number ranks batched={per}", flush=True)
+    gpu, cpu = tmp_path / "gpu", tmp_path / "cpu"
+    gsnp_amd.run(fa, so, gpu, compress=True, batched=per, read_length=L)
+    oracle_lib.run(fa, so, cpu, compress=True, read_length=L)
+    for sfx in SUFFIXES:
+        assert filecmp.cmp(str(gpu) + sfx, str(cpu) + sfx, shallow=False), f"{sfx} differs"
+    gt, ct = tmp_path / "gpu.cns", tmp_path / "cpu.cns"
+    gsnp_amd.run(fa, so, gt, batched=per, read_length=L)
+    oracle_lib.run(fa, so, ct, read_length=L)
+    assert gt.read_bytes() == ct.read_bytes(), _first_diff(gt, ct)
+
+
+@pytest.mark.parametrize("per", [0, 4])
 def test_compressed_uncovered_window(built, tmp_path, per):
     """A reference window without reads: its run-length columns hold one run
     of 262,144 sites, a value beyond the integer dictionaries' presence bitmap
