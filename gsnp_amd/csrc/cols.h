@@ -4,6 +4,7 @@
 #include <stdint.h>
 
 #include <functional>
+#include <vector>
 
 namespace gsnp {
 
@@ -23,9 +24,27 @@ struct ColumnPass {
 };
 using ColumnSink = std::function<int(const ColumnPass&)>;
 
+// A queued pass whose copy to pinned host memory may still be in flight, and
+// the sink it goes to.
+struct ColumnPending {
+    ColumnPass pass{};
+    int buf = 0;
+    std::vector<uint64_t> starts;
+    std::vector<uint32_t> sizes;
+    std::vector<uint64_t> offsets;
+    ColumnSink sink;
+};
+
 // Encodes the called sites [offset, offset+n) of the context, which hold the
 // chromosome positions [ws_abs, ws_abs+n) (ws_abs on a window boundary, n > 0),
-// one chunk per reference window, and hands every pass to `sink`.
+// one chunk per reference window, and hands every pass to `sink` in order.
+// The last pass stays queued, its copy running beside whatever the caller
+// queues next, until the next call hands it over or finish_columns does: a
+// caller calls finish_columns before it writes anything else to the same
+// files and before its sink's captures go out of scope, or drop_columns after
+// an error.
 int encode_columns(Ctx& c, uint32_t offset, uint32_t ws_abs, uint32_t n, const ColumnSink& sink);
+int finish_columns(Ctx& c);
+void drop_columns(Ctx& c);
 
 }  // namespace gsnp

@@ -1427,8 +1427,14 @@ int encode_columns(Ctx& c, uint32_t offset, uint32_t ws_abs, uint32_t n, const C
     ColBufs& b = c.cols;
     const uint32_t want = std::min<uint32_t>(kSubWindows, (n + kWin - 1) / kWin);
     if (b.win < want) {
+        // a queued pass reads the encoder arrays until its writes are done
+        int rc = finish_columns(c);
+        if (rc) {
+            drop_columns(c);
+            return rc;
+        }
         HIPCHK(hipStreamSynchronize(st));
-        int rc = alloc_cols(b, want);
+        rc = alloc_cols(b, want);
         if (rc) return rc;
         // the split dictionaries' bitmap and flags start at zero; each pass
         // leaves them so
@@ -1443,25 +1449,6 @@ int encode_columns(Ctx& c, uint32_t offset, uint32_t ws_abs, uint32_t n, const C
         HIPCHK(hipEventCreateWithFlags(&b.ev_written, hipEventDisableTiming));
         for (hipEvent_t& e : b.ev_copied) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     }
-    // a pass whose copy to host memory is in flight; handed to `sink` once the
-    // next pass has been queued (its buffers are reused two passes later)
-    struct Done {
-        ColumnPass pass;
-        int buf = 0;
-        std::vector<uint64_t> starts;
-        std::vector<uint32_t> sizes;
-        std::vector<uint64_t> offsets;
-    };
-    Done prev;
-    bool have_prev = false;
-    auto hand_over = [&](Done& d) -> int {
-        HIPCHK(hipEventSynchronize(b.ev_copied[d.buf]));
-        d.pass.starts = d.starts.data();
-        d.pass.sizes = d.sizes.data();
-        d.pass.offsets = d.offsets.data();
-        d.pass.n_windows = (uint32_t)d.sizes.size();
-        return sink(d.pass);
-    };
     DictArgs da{};
     for (int e = 0; e < 32; e++) da.pow2bits[e] = (uint8_t)dict_bits(1ull << e);
     // GSNP_DICT_LDS_MAX: distinct values a dictionary keeps in LDS (at most
@@ -1472,7 +1459,7 @@ int encode_columns(Ctx& c, uint32_t offset, uint32_t ws_abs, uint32_t n, const C
     da.bitmap = 1;
     if (const char* v = std::getenv("GSNP_DICT_BITMAP")) da.bitmap = (uint32_t)std::strtoul(v, nullptr, 10);
     const size_t S = (size_t)b.win * kWin;
-    int cur = 0;
+    int& cur = b.cur;
     for (uint32_t s0 = 0; s0 < n; s0 += kSubWindows * kWin) {
         const uint32_t ns = std::min<uint32_t>(n - s0, kSubWindows * kWin);
         const uint32_t nwin = (ns + kWin - 1) / kWin;
@@ -1531,8 +1518,8 @@ int encode_columns(Ctx& c, uint32_t offset, uint32_t ws_abs, uint32_t n, const C
         HIPCHK(hipMemcpyAsync(tot, b.ftot.p, sizeof tot, hipMemcpyDeviceToHost, st));
         HIPCHK(hipMemcpyAsync(offsets.data(), b.foff.p, 8ull * nwin * 5, hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
-        // buffer `cur` was last copied from two passes ago, and that pass has
-        // been handed over, so it is free
+        // buffer `cur` was last copied from two passes ago (this call's or an
+        // earlier one's), and that pass has been handed over, so it is free
         const uint64_t need = (*std::max_element(tot, tot + 5) + 255) & ~255ull;  // files stay 256-byte aligned
         if (need > b.stream_cap[cur]) {
             HIPCHK(b.stream[cur].alloc(5 * need));
@@ -1553,8 +1540,9 @@ int encode_columns(Ctx& c, uint32_t offset, uint32_t ws_abs, uint32_t n, const C
             HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&b.host[cur]), all, hipHostMallocDefault));
             b.host_cap[cur] = all;
         }
-        Done d;
+        ColumnPending d;
         d.buf = cur;
+        d.sink = sink;
         d.offsets.swap(offsets);
         uint64_t at = 0;
         for (int f = 0; f < 5; f++) {
@@ -1570,18 +1558,42 @@ int encode_columns(Ctx& c, uint32_t offset, uint32_t ws_abs, uint32_t n, const C
             d.starts.push_back((uint64_t)ws_abs + s0 + (uint64_t)w * kWin + 1);
             d.sizes.push_back(std::min<uint32_t>(kWin, ns - w * kWin));
         }
-        if (have_prev) {
-            int rc = hand_over(prev);
-            if (rc) {
-                (void)hipStreamSynchronize(b.copy);
-                return rc;
-            }
+        // the pass before this one (of this call or an earlier one) goes to
+        // its sink now that this pass is queued
+        int rc = finish_columns(c);
+        if (rc) {
+            drop_columns(c);
+            return rc;
         }
-        prev = std::move(d);
-        have_prev = true;
+        b.prev = std::move(d);
+        b.have_prev = true;
         cur ^= 1;
     }
-    return have_prev ? hand_over(prev) : GSNP_OK;
+    return GSNP_OK;
+}
+
+int finish_columns(Ctx& c)
+{
+    ColBufs& b = c.cols;
+    if (!b.have_prev) return GSNP_OK;
+    b.have_prev = false;
+    ColumnPending& d = b.prev;
+    HIPCHK(hipEventSynchronize(b.ev_copied[d.buf]));
+    d.pass.starts = d.starts.data();
+    d.pass.sizes = d.sizes.data();
+    d.pass.offsets = d.offsets.data();
+    d.pass.n_windows = (uint32_t)d.sizes.size();
+    const int rc = d.sink(d.pass);
+    d.sink = nullptr;
+    return rc;
+}
+
+void drop_columns(Ctx& c)
+{
+    ColBufs& b = c.cols;
+    if (b.copy) (void)hipStreamSynchronize(b.copy);
+    b.have_prev = false;
+    b.prev.sink = nullptr;
 }
 
 }  // namespace gsnp

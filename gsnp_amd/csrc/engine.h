@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "../../include/gsnp_abi.h"
+#include "cols.h"
 #include "kernels.h"
 
 namespace gsnp {
@@ -174,6 +175,9 @@ struct ColBufs {
     uint64_t host_cap[2] = {0, 0};
     hipStream_t copy = nullptr;
     hipEvent_t ev_written = nullptr, ev_copied[2] = {nullptr, nullptr};
+    int cur = 0;            // the buffer the next pass takes
+    bool have_prev = false;  // prev: the last pass queued, not yet handed over
+    ColumnPending prev;
     ColBufs() = default;
     ColBufs(const ColBufs&) = delete;
     ColBufs& operator=(const ColBufs&) = delete;

@@ -292,8 +292,11 @@ int job_write(gsnp_job* job, const char* out, int compress, int batched)
         const uint32_t N = (uint32_t)chr.code.size();
         auto on_results = [&](uint32_t offset, uint32_t ws, uint32_t n) -> int {
             if (!compress) return GSNP_OK;
-            if (n == 0)  // the empty window after a chromosome that fills its last window
+            if (n == 0) {  // the empty window after a chromosome that fills its last window
+                const int r = finish_columns(*ctx);  // the queued chunks go first
+                if (r) return r;
                 return cw.write_empty_window(chr.name, ws, err) ? fail(GSNP_E_IO, err) : GSNP_OK;
+            }
             // one -C chunk per reference window of the span, encoded on the device
             return encode_columns(*ctx, offset, ws, n, [&](const ColumnPass& pass) -> int {
                 return cw.write_chunks(chr.name, pass, err) ? fail(GSNP_E_IO, err) : GSNP_OK;
@@ -301,7 +304,11 @@ int job_write(gsnp_job* job, const char* out, int compress, int batched)
         };
         rc = call_chromosome(ctx, b.pos, N, batched, compress ? kTextNone : kTextHost, on_results, nullptr, sink,
                              job->mask(b.chr));
-        if (rc) return rc;
+        if (!rc && compress) rc = finish_columns(*ctx);  // the sink refers to this chromosome
+        if (rc) {
+            drop_columns(*ctx);
+            return rc;
+        }
     }
     if (!compress) {
         rc = ctx->finish_text(sink);
@@ -399,6 +406,8 @@ int gsnp_job_columns(gsnp_job* job, int batched, uint64_t* nbytes)
     };
     int rc = gsnp::call_chromosome(job->ctx, b.pos, N, batched, gsnp::kTextNone, on_results, nullptr, nullptr,
                                    job->mask(b.chr));
+    if (!rc) rc = gsnp::finish_columns(*job->ctx);
+    if (rc) gsnp::drop_columns(*job->ctx);
     if (nbytes) *nbytes = total;
     return rc;
 }
@@ -419,6 +428,8 @@ int gsnp_job_columns_to(gsnp_job* job, int batched, gsnp_chunk_fn fn, void* user
         };
         auto on_results = [&](uint32_t offset, uint32_t ws, uint32_t n) -> int {
             if (n == 0) {
+                const int r = gsnp::finish_columns(*job->ctx);  // the queued chunks go first
+                if (r) return r;
                 const gsnp::EmptyChunks e;
                 return hand((uint64_t)ws + 1, 0, e.chunk, e.bytes);
             }
@@ -435,6 +446,8 @@ int gsnp_job_columns_to(gsnp_job* job, int batched, gsnp_chunk_fn fn, void* user
         };
         int rc = gsnp::call_chromosome(job->ctx, b.pos, N, batched, gsnp::kTextNone, on_results, nullptr, nullptr,
                                        job->mask(b.chr));
+        if (!rc) rc = gsnp::finish_columns(*job->ctx);
+        if (rc) gsnp::drop_columns(*job->ctx);
         if (nbytes) *nbytes = total;
         return rc;
     });
@@ -462,12 +475,15 @@ int gsnp_window_columns(gsnp_ctx* c, uint32_t call_length, uint8_t* const* dst, 
             const gsnp::EmptyChunks e;
             return deliver(e.chunk, e.bytes);
         }
-        return gsnp::encode_columns(*c, 0, c->ws, call_length, [&](const gsnp::ColumnPass& pass) -> int {
+        int rc = gsnp::encode_columns(*c, 0, c->ws, call_length, [&](const gsnp::ColumnPass& pass) -> int {
             const uint8_t* chunk[5];
             uint64_t bytes[5];
             gsnp::pass_window(pass, 0, chunk, bytes);
             return deliver(chunk, bytes);
         });
+        if (!rc) rc = gsnp::finish_columns(*c);  // the window's chunks are the caller's on return
+        if (rc) gsnp::drop_columns(*c);
+        return rc;
     });
 }
 
@@ -553,6 +569,10 @@ int gsnp_job_pass(gsnp_job* job, int batched, int emit_text, uint64_t* nbytes, u
         rc = gsnp::call_chromosome(job->ctx, b.pos, N, (int)per, columns ? (int)gsnp::kTextNone : emit_text,
                                    on_results, &total, landed, job->mask(b.chr));
         sites += N;
+    }
+    if (columns) {  // the last pass's chunks, queued beside the next chromosome's calls until now
+        if (!rc) rc = gsnp::finish_columns(*job->ctx);
+        if (rc) gsnp::drop_columns(*job->ctx);
     }
     if (!rc && emit_text == gsnp::kTextHost) rc = job->ctx->finish_text(landed);
     if (nbytes) *nbytes = total;
