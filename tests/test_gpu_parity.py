@@ -916,6 +916,43 @@ def test_window_and_callback_columns_match_oracle(datasets, tmp_path, name):
             assert filecmp.cmp(a, b, shallow=False), f"{label}: {sfx} differs"
 
 
+def test_chunk_callback_error_leaves_no_queued_pass(datasets, tmp_path):
+    """A chunk callback that fails on its second chunk ends gsnp_job_columns_to
+    with an I/O error while a later pass is still queued (spans of one
+    window); that pass is dropped, so a full callback pass on the same job
+    afterwards gives the oracle's -C files, and gsnp_job_pass with -C chunks
+    counts the same bytes."""
+    import gsnp_amd
+
+    ds = datasets("c2w3")
+    cpu = tmp_path / "c2w3.cpu"
+    oracle_lib.run(ds.fasta, ds.soap, cpu, dbsnp=ds.dbsnp, compress=True, **ds.flags)
+    job = gsnp_amd.Job(ds.fasta, ds.soap, ds.dbsnp, **ds.flags)
+    L = gsnp_amd.lib()
+    seen = []
+
+    def fail_second(user, chr_, start, n, chunk, nbytes):
+        seen.append(start)
+        return 1 if len(seen) == 2 else 0
+
+    print("[parity] chunk callback error, then a full pass", flush=True)
+    windows = []
+    for i in range(len(job)):
+        job.select(i)
+        if i == 0:
+            cb = gsnp_amd.CHUNK_FN(fail_second)
+            nb = ctypes.c_uint64()
+            rc = L.gsnp_job_columns_to(job.h, 1, cb, None, ctypes.byref(nb))
+            assert rc == 1 and len(seen) == 2, (rc, seen)  # GSNP_E_IO
+        windows += job.column_chunks(batched=1)
+    pre = tmp_path / "c2w3.after"
+    _assemble_cfiles(pre, windows)
+    for sfx in SUFFIXES:
+        assert filecmp.cmp(str(pre) + sfx, str(cpu) + sfx, shallow=False), f"{sfx} differs"
+    nbytes, nsites = job.run_pass(batched=1, emit_text=3)
+    assert nbytes == sum(len(c) for w in windows for c in w[3])
+
+
 def test_caller_stream_gives_the_same_rows(datasets, tmp_path):
     """gsnp_set_stream queues the context's work on a caller's stream (one
     created through the HIP runtime the library uses, as a C caller would);
