@@ -219,7 +219,7 @@ def main() -> int:
     args = ap.parse_args()
 
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU: --gpus must be 1 (sharding across GPUs is designed "
+        print(json.dumps({"error": "gsnp_amd runs on one GPU: --gpus must be 1 (sharding across GPUs is designed "
                                    "in the documentation, not built)"}))
         return 2
 

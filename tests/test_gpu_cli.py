@@ -6,7 +6,7 @@ byte for byte: -u, -2 -s, -t, -g/-p (given as values, stored log10), -z (with
 q_max = q_min + 40) and -Q, -L, -m and the prior flags, -M/-I round trips,
 -C with a decompression round trip, the batched path (-B) and -T target
 regions (which the reference refuses).  The flags the reference refuses and
-this build does too (-q, -F) exit 0 with its message and write nothing."""
+the gsnp binary refuses too (-q, -F) exit 0 with its message and write nothing."""
 from __future__ import annotations
 
 import filecmp
