@@ -14,11 +14,11 @@
 //   dl/d ln het = sum_s sum_g w_s(g) [prior_s(g) carries one factor of het]
 //
 // Accumulation is deterministic: the records are sorted once by score-row key
-// (stable LSD radix sort), each evaluation writes the per-record
-// contributions in key order (one thread per record), and a segmented reduce
-// sums them in a fixed order: each piece of at most 512 contiguous records by
-// one wave (a strided lane sum, then a fixed xor tree), then the pieces of a
-// key in order.  No floating-point atomics.
+// (stable LSD radix sort), and each evaluation sums their contributions in a
+// fixed order: each piece of at most 512 records of one key (in key order) by
+// one wave, whose lane l computes the contributions of records l, l + 64, ...
+// from their sites' weights and adds them in that order before a fixed xor
+// tree, then the pieces of a key in order.  No floating-point atomics.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -338,60 +338,58 @@ __global__ __launch_bounds__(256) void k_adj_rsite(const uint64_t* off, uint32_t
     for (uint64_t r = off[s]; r < off[s + 1]; r++) rsite[r] = s;
 }
 
-// per record: its contribution to the four ln p entries of its row, stored at
-// the record's place in key order.  One thread per record (a thread per site
-// walked its records one after another, each waiting on its own loads); the
-// records of a site share its ten weights, which come from cache.
-__global__ __launch_bounds__(256) void k_adj_contrib(const double* pm, const uint32_t* key, const uint32_t* rsite,
-                                                     const uint32_t* rank, const double* wsite, uint32_t nr,
-                                                     double* c_sorted)
+// the site of each record in key order (prepare): ssite[j] = rsite[perm[j]]
+__global__ __launch_bounds__(256) void k_adj_ssite(const uint32_t* perm, const uint32_t* rsite, uint32_t nr,
+                                                   uint32_t* ssite)
 {
-    const uint32_t r = blockIdx.x * 256u + threadIdx.x;
-    if (r >= nr) return;
-    const uint32_t s = rsite[r], k = key[r];
-    const uint32_t q = k >> 10, cyc = (k >> 2) & 255u, o = k & 3u;
-    const double* prow = pm + ((size_t)q << 12) + ((size_t)cyc << 4) + o;
-    double p[4], w[10];
-#pragma unroll
-    for (int a = 0; a < 4; a++) p[a] = prow[a << 2];
-#pragma unroll
-    for (int g = 0; g < 10; g++) w[g] = wsite[(size_t)s * 10 + g];
-    double c[4] = {0.0, 0.0, 0.0, 0.0};
-    int g = 0;
-#pragma unroll
-    for (int a1 = 0; a1 < 4; a1++)
-#pragma unroll
-        for (int a2 = a1; a2 < 4; a2++) {
-            if (a1 == a2) {
-                c[a1] += w[g];
-            } else {
-                const double inv = 1.0 / (p[a1] + p[a2]);  // one division per pair
-                c[a1] += w[g] * (p[a1] * inv);
-                c[a2] += w[g] * (p[a2] * inv);
-            }
-            g++;
-        }
-    double2* dst = reinterpret_cast<double2*>(c_sorted + (size_t)rank[r] * 4);
-    dst[0] = make_double2(c[0], c[1]);
-    dst[1] = make_double2(c[2], c[3]);
+    const uint32_t j = blockIdx.x * 256u + threadIdx.x;
+    if (j < nr) ssite[j] = rsite[perm[j]];
 }
 
-// one wave per piece: lane l adds records l, l + 64, ... of the piece, then
-// a fixed xor tree over the wave (a fixed order: the sums are the same on
-// every run)
-__global__ __launch_bounds__(256) void k_adj_pieces(const double* c_sorted, const uint32_t* pstart, uint32_t npieces,
+// One wave per piece (at most 512 records of one key, in key order): lane l
+// computes the contributions of records l, l + 64, ... of the piece to the
+// four ln p entries of the key's row, adds them in that order, then a fixed
+// xor tree over the wave (a fixed order: the sums are the same on every run).
+// A record's contribution is sum_g w_s(g) share_g(a) from its site's ten
+// weights and the key's four p values, one division per genotype pair.
+__global__ __launch_bounds__(256) void k_adj_pieces(const double* pm, const uint32_t* skey, const uint32_t* ssite,
+                                                    const double* wsite, const uint32_t* pstart, uint32_t npieces,
                                                     double* psum)
 {
     const uint32_t p = blockIdx.x * 4u + (threadIdx.x >> 6), lane = threadIdx.x & 63u;
     if (p >= npieces) return;  // wave-uniform
+    const uint32_t j0 = pstart[p], j1 = pstart[p + 1];
+    const uint32_t k = skey[j0];  // one key per piece
+    const uint32_t q = k >> 10, cyc = (k >> 2) & 255u, o = k & 3u;
+    const double* prow = pm + ((size_t)q << 12) + ((size_t)cyc << 4) + o;
+    double pa[4];
+#pragma unroll
+    for (int a = 0; a < 4; a++) pa[a] = prow[a << 2];
     double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-    for (uint32_t j = pstart[p] + lane; j < pstart[p + 1]; j += 64u) {
-        const double2* src = reinterpret_cast<const double2*>(c_sorted + (size_t)j * 4);
-        const double2 x = src[0], y = src[1];
-        s0 += x.x;
-        s1 += x.y;
-        s2 += y.x;
-        s3 += y.y;
+    for (uint32_t j = j0 + lane; j < j1; j += 64u) {
+        const double* ws = wsite + (size_t)ssite[j] * 10;
+        double w[10];
+#pragma unroll
+        for (int g = 0; g < 10; g++) w[g] = ws[g];
+        double c[4] = {0.0, 0.0, 0.0, 0.0};
+        int g = 0;
+#pragma unroll
+        for (int a1 = 0; a1 < 4; a1++)
+#pragma unroll
+            for (int a2 = a1; a2 < 4; a2++) {
+                if (a1 == a2) {
+                    c[a1] += w[g];
+                } else {
+                    const double inv = 1.0 / (pa[a1] + pa[a2]);  // one division per pair
+                    c[a1] += w[g] * (pa[a1] * inv);
+                    c[a2] += w[g] * (pa[a2] * inv);
+                }
+                g++;
+            }
+        s0 += c[0];
+        s1 += c[1];
+        s2 += c[2];
+        s3 += c[3];
     }
     for (int off = 32; off > 0; off >>= 1) {
         s0 += __shfl_xor(s0, off);
@@ -555,6 +553,10 @@ extern "C" int gsnp_adjoint_prepare(gsnp_ctx* c, uint32_t start, uint32_t n)
     rc = radix_sort16(*c, c->adj_skey.p, c->adj_perm.p, c->adj_tmp.p, c->adj_tmp.p + nr + 1, nr);
     if (rc) return rc;
     if (nr) hipLaunchKernelGGL(k_invert, dim3(blocks_for(nr)), dim3(256), 0, c->stream, c->adj_perm.p, nr, c->adj_rank.p);
+    HIPCHK(c->adj_ssite.alloc(nr + 1));
+    if (nr)
+        hipLaunchKernelGGL(k_adj_ssite, dim3(blocks_for(nr)), dim3(256), 0, c->stream, c->adj_perm.p, c->adj_rsite.p,
+                           nr, c->adj_ssite.p);
     // reduction pieces
     uint32_t* flag = c->adj_tmp.p;
     HIPCHK(c->adj_pidx.alloc((size_t)nr + 1));
@@ -577,7 +579,6 @@ extern "C" int gsnp_adjoint_prepare(gsnp_ctx* c, uint32_t start, uint32_t n)
     HIPCHK(c->adj_w.alloc((size_t)n * 10 + 1));
     HIPCHK(c->adj_ll.alloc(n + 1));
     HIPCHK(c->adj_dh.alloc(n + 1));
-    HIPCHK(c->adj_c.alloc(4 * (size_t)nr + 4));
     HIPCHK(c->adj_psum.alloc(4 * (size_t)npieces + 4));
     HIPCHK(c->adj_part.alloc(2 * (size_t)blocks_for(n) + 2));
     HIPCHK(c->adj_out.alloc(2));
@@ -648,13 +649,11 @@ extern "C" int gsnp_adjoint_eval(gsnp_ctx* c, const double* p_matrix, double log
                            c->adj_T.p, c->adj_key.p, c->adj_off.p,
                            c->adj_prow.p, c->adj_lp.p, c->adj_mask.p, n, c->p.is_monoploid, c->adj_ll.p, c->adj_dh.p,
                            c->adj_w.p);
-        if (nr)
-            hipLaunchKernelGGL(k_adj_contrib, dim3(blocks_for(nr)), dim3(256), 0, st, c->adj_pm.p, c->adj_key.p,
-                               c->adj_rsite.p, c->adj_rank.p, c->adj_w.p, nr, c->adj_c.p);
     }
     if (c->adj_npieces)
-        hipLaunchKernelGGL(k_adj_pieces, dim3((c->adj_npieces + 3) / 4), dim3(256), 0, st, c->adj_c.p,
-                           c->adj_pstart.p, c->adj_npieces, c->adj_psum.p);
+        hipLaunchKernelGGL(k_adj_pieces, dim3((c->adj_npieces + 3) / 4), dim3(256), 0, st, c->adj_pm.p,
+                           c->adj_skey.p, c->adj_ssite.p, c->adj_w.p, c->adj_pstart.p, c->adj_npieces,
+                           c->adj_psum.p);
     hipLaunchKernelGGL(k_adj_grad, dim3(blocks_for(kKeys)), dim3(256), 0, st, c->adj_psum.p, c->adj_kpiece.p,
                        c->adj_grad.p);
     const uint32_t nb = blocks_for(n) ? blocks_for(n) : 1;

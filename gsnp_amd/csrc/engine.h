@@ -325,9 +325,10 @@ struct Ctx {
     uint64_t adj_nrec = 0;
     DevBuf<uint32_t> adj_cnt, adj_prow, adj_key, adj_skey, adj_perm, adj_rank, adj_tmp, adj_pstart, adj_kpiece;
     DevBuf<uint32_t> adj_rsite;  // per record (record order): its site
+    DevBuf<uint32_t> adj_ssite;  // per record in key order: its site
     DevBuf<uint32_t> adj_hist, adj_mask;
     DevBuf<uint64_t> adj_off, adj_pidx, adj_hoff, adj_bsum;
-    DevBuf<double> adj_T, adj_w, adj_ll, adj_dh, adj_c, adj_psum, adj_part, adj_out, adj_pm, adj_grad, adj_lp;
+    DevBuf<double> adj_T, adj_w, adj_ll, adj_dh, adj_psum, adj_part, adj_out, adj_pm, adj_grad, adj_lp;
     // device dictionary encoder scratch
     DevBuf<int32_t> dict_mm, dict_vals;
     DevBuf<uint32_t> dict_bits, dict_wpre, dict_out;
