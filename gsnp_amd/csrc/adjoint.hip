@@ -195,12 +195,6 @@ __global__ __launch_bounds__(256) void k_iota(uint32_t* v, uint32_t n)
     if (i < n) v[i] = i;
 }
 
-__global__ __launch_bounds__(256) void k_invert(const uint32_t* perm, uint32_t n, uint32_t* rank)
-{
-    const uint32_t j = blockIdx.x * 256u + threadIdx.x;
-    if (j < n) rank[perm[j]] = j;
-}
-
 // piece heads: a new piece starts at every key change and every kPiece-th item
 __global__ __launch_bounds__(256) void k_piece_flags(const uint32_t* skey, uint32_t n, uint32_t* flag)
 {
@@ -539,7 +533,6 @@ extern "C" int gsnp_adjoint_prepare(gsnp_ctx* c, uint32_t start, uint32_t n)
     HIPCHK(c->adj_rsite.alloc(nr + 1));
     if (n)
         hipLaunchKernelGGL(k_adj_rsite, dim3(blocks_for(n)), dim3(256), 0, c->stream, c->adj_off.p, n, c->adj_rsite.p);
-    HIPCHK(c->adj_rank.alloc(nr + 1));
     HIPCHK(c->adj_skey.alloc(nr + 1));
     HIPCHK(c->adj_perm.alloc(nr + 1));
     HIPCHK(c->adj_tmp.alloc(2 * (size_t)nr + 2));
@@ -552,7 +545,6 @@ extern "C" int gsnp_adjoint_prepare(gsnp_ctx* c, uint32_t start, uint32_t n)
     if (nr) hipLaunchKernelGGL(k_iota, dim3(blocks_for(nr)), dim3(256), 0, c->stream, c->adj_perm.p, nr);
     rc = radix_sort16(*c, c->adj_skey.p, c->adj_perm.p, c->adj_tmp.p, c->adj_tmp.p + nr + 1, nr);
     if (rc) return rc;
-    if (nr) hipLaunchKernelGGL(k_invert, dim3(blocks_for(nr)), dim3(256), 0, c->stream, c->adj_perm.p, nr, c->adj_rank.p);
     HIPCHK(c->adj_ssite.alloc(nr + 1));
     if (nr)
         hipLaunchKernelGGL(k_adj_ssite, dim3(blocks_for(nr)), dim3(256), 0, c->stream, c->adj_perm.p, c->adj_rsite.p,

@@ -323,7 +323,7 @@ struct Ctx {
     // their place in key order and the reduction pieces
     uint32_t adj_start = 0, adj_n = 0, adj_npieces = 0;
     uint64_t adj_nrec = 0;
-    DevBuf<uint32_t> adj_cnt, adj_prow, adj_key, adj_skey, adj_perm, adj_rank, adj_tmp, adj_pstart, adj_kpiece;
+    DevBuf<uint32_t> adj_cnt, adj_prow, adj_key, adj_skey, adj_perm, adj_tmp, adj_pstart, adj_kpiece;
     DevBuf<uint32_t> adj_rsite;  // per record (record order): its site
     DevBuf<uint32_t> adj_ssite;  // per record in key order: its site
     DevBuf<uint32_t> adj_hist, adj_mask;
